@@ -1,0 +1,139 @@
+/*
+ * knnj.h -- C ABI of the MI355X exact kNN-join engine (libknnj.so).
+ *
+ * The reference (gychen1991/knn_sweet) has no library API: its join is the
+ * program `knnjoin` (src/knnjoin.cu:983-1172), which reads
+ *   query_nb source_nb dim qrep_nb srep_nb K qfile sfile
+ * from argv (knnjoin.cu:990-1001), builds a clustered index of both sets
+ * (clusterReps, knnjoin.cu:166-411), bounds and prunes cluster pairs
+ * (RepsUpperBound / FilterReps, knnjoin.cu:459-525), scans the survivors
+ * (KNNQuery_base / KNNQuery + final, knnjoin.cu:762-979) and leaves
+ * IndexDist knearest[query_nb][K] (include/common8.h:37-40) on the device.
+ * The entry points below replace that program's compute path; each cites the
+ * part it replaces.
+ *
+ * Conventions
+ *  - Matrices are row-major FP32, row stride `ld` floats (ld >= d).
+ *  - Output row q belongs to the q-th row of Q.  Each row lists the k
+ *    reference rows with the smallest (squared distance, row index) keys,
+ *    ascending; the squared distance is the binary64 value
+ *    sum_j ((double)q_j - (double)r_j)^2 accumulated in ascending j without
+ *    fused multiply-add, so ties and near-ties are ordered reproducibly.
+ *    out_dist holds (float)sqrt of that value, or the value itself rounded
+ *    to float when knnj_opts.out_squared is set.
+ *  - Output pointers are device pointers when Q.on_device is set, host
+ *    pointers otherwise.  Inputs may be host or device independently.
+ *  - Every call is synchronous and returns a status; knnj_last_error() gives
+ *    the message of the calling thread's last failure.  No exception crosses
+ *    the ABI.  A knnj_ctx / knnj_index is used by one host thread at a time;
+ *    several contexts may share a device.
+ */
+#ifndef KNNJ_H
+#define KNNJ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  KNNJ_OK = 0,
+  KNNJ_EINVAL = 1,       /* bad argument: k < 1, k > |R|, d < 1, null pointer, non-finite input */
+  KNNJ_ENOMEM = 2,       /* device memory or the context's memory budget exhausted */
+  KNNJ_EDEVICE = 3,      /* HIP runtime error */
+  KNNJ_EUNSUPPORTED = 4, /* valid request outside what this build implements (e.g. k > 1024) */
+  KNNJ_EOVERFLOW = 5     /* candidate list overflowed even at the largest capacity */
+} knnj_status;
+
+typedef enum {
+  KNNJ_PREC_AUTO = 0, /* BF16 filter for d >= 64 real-valued data, FP32 otherwise */
+  KNNJ_PREC_FP32 = 1, /* plain FP32 distance tiles */
+  KNNJ_PREC_BF16 = 2, /* BF16 MFMA filter + exact rerank */
+  KNNJ_PREC_FP8 = 3   /* reserved: FP8 filter (not built) */
+} knnj_precision;
+
+typedef struct knnj_ctx knnj_ctx;
+typedef struct knnj_index knnj_index;
+
+typedef struct {
+  const float *data;
+  int64_t n;         /* rows */
+  int32_t d;         /* dimension */
+  int64_t ld;        /* row stride in floats, >= d (0 means d) */
+  int32_t on_device; /* 1: `data` is a device pointer on the context's device */
+} knnj_matrix;
+
+/* Options; a NULL pointer or zero fields select tuned defaults. */
+typedef struct {
+  int32_t m_q;         /* query representatives (reference: qrep_nb, knnjoin.cu:993) */
+  int32_t m_r;         /* reference representatives (srep_nb, knnjoin.cu:994) */
+  int32_t trials;      /* representative draws per side (reference: 10, knnjoin.cu:195) */
+  int32_t precision;   /* knnj_precision */
+  uint64_t seed;       /* representative sampling seed (reference: 2015, knnjoin.cu:199) */
+  int32_t out_squared; /* 1: out_dist holds squared distances */
+  int32_t out_aos;     /* 1: out_idx is IndexDist{int32 index; float dist}[nQ][k]
+                          (include/common8.h:37-40); out_dist is then ignored */
+  int32_t brute_force; /* 1: one cluster per side (no pruning) */
+  int32_t tile_q;      /* queries per wave tile, 0 = auto */
+  int32_t reserved[6];
+} knnj_opts;
+
+/* Device-side timings (hipEvents on the context's stream) and counters. */
+typedef struct {
+  double t_total_ms;        /* whole call, including host<->device copies */
+  double t_index_ms;        /* representative sampling, assignment, sorting, permuted copies */
+  double t_bounds_ms;       /* theta_A and surviving cluster-pair lists */
+  double t_join_ms;         /* distance tiles, k-select and rerank (the join proper) */
+  double t_fallback_ms;     /* overflow re-run, if any */
+  int64_t n_dist_evals;     /* executed (query, reference) distance evaluations */
+  double pruning_ratio;     /* 1 - n_dist_evals / (|Q| |R|) */
+  int64_t n_cluster_pairs;  /* surviving (query cluster, reference cluster) pairs */
+  int64_t n_overflow;       /* queries re-run at a larger candidate capacity */
+  int32_t exact_mode;       /* 1: integer inputs, FP32 keys are exact */
+  int32_t precision_used;   /* knnj_precision actually run */
+  int32_t m_q, m_r;         /* representative counts used */
+  int32_t tile_q;           /* queries per wave tile used */
+  int32_t kcap;             /* candidate capacity per query used */
+  int64_t n_tiles;          /* query tiles launched */
+  double filter_max_abs_err;/* largest |filter distance^2 - binary64 distance^2| seen in rerank */
+  int32_t reserved[8];
+} knnj_stats;
+
+/* One independent join of a batch (knnj_join_batch). */
+typedef struct {
+  knnj_matrix R, Q;
+  int32_t k;
+  int32_t *out_idx;
+  float *out_dist;
+} knnj_instance;
+
+/* Library version string. */
+const char *knnj_version(void);
+
+/* Thread-local message of the last failed call on this thread ("" if none). */
+const char *knnj_last_error(void);
+
+/* Context on HIP device `device`: stream, events, grow-only workspace.
+ * Replaces the reference's implicit CUDA context and its warm-up thread
+ * (knnjoin.cu:980-986). */
+knnj_status knnj_ctx_create(int device, knnj_ctx **out);
+void knnj_ctx_destroy(knnj_ctx *ctx);
+
+/* Caps the context's workspace in bytes (0 = no cap).  A call whose plan
+ * needs more returns KNNJ_ENOMEM before touching the device. */
+knnj_status knnj_ctx_set_budget(knnj_ctx *ctx, int64_t bytes);
+
+/* Exact kNN join of Q against R: the whole reference program
+ * (knnjoin.cu:1043-1141: clusterReps, RepsUpperBound, FilterReps, the
+ * replist sort and KNNQuery_base / KNNQuery+final), results in original
+ * query order.  out_idx/out_dist: [Q.n][k]. */
+knnj_status knnj_join(knnj_ctx *ctx, knnj_matrix R, knnj_matrix Q, int32_t k,
+                      const knnj_opts *opts, int32_t *out_idx, float *out_dist,
+                      knnj_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KNNJ_H */

@@ -1,0 +1,194 @@
+"""knn_sweet_amd -- MI355X-native exact kNN join (Python binding of libknnj.so).
+
+The library is the C ABI declared in include/knnj.h; this module mirrors it
+with ctypes.  It has no CPU fallback: if libknnj.so is missing or no GPU is
+visible, the calls raise.
+
+    import knn_sweet_amd as kj
+    ctx = kj.Context()
+    idx, dist, stats = ctx.join(R, Q, k)     # numpy (host) or torch CUDA tensors
+
+``R`` and ``Q`` are 2-D float32 arrays (row-major).  For torch tensors on the
+GPU the outputs are torch tensors on the same device; for numpy inputs they
+are numpy arrays.  ``idx[q]`` lists the k reference rows nearest to ``Q[q]``
+in ascending (squared distance, row index) order; ``dist`` is the Euclidean
+distance (or the squared one with ``out_squared=True``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libknnj.so")
+
+KNNJ_OK, KNNJ_EINVAL, KNNJ_ENOMEM, KNNJ_EDEVICE, KNNJ_EUNSUPPORTED, KNNJ_EOVERFLOW = range(6)
+PREC_AUTO, PREC_FP32, PREC_BF16, PREC_FP8 = range(4)
+_STATUS_NAMES = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EUNSUPPORTED", 5: "EOVERFLOW"}
+
+
+class KnnjError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"knnj {_STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class Matrix(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("n", ctypes.c_int64), ("d", ctypes.c_int32),
+                ("ld", ctypes.c_int64), ("on_device", ctypes.c_int32)]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("m_q", ctypes.c_int32), ("m_r", ctypes.c_int32), ("trials", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("seed", ctypes.c_uint64),
+                ("out_squared", ctypes.c_int32), ("out_aos", ctypes.c_int32),
+                ("brute_force", ctypes.c_int32), ("tile_q", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 6)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("t_total_ms", ctypes.c_double), ("t_index_ms", ctypes.c_double),
+                ("t_bounds_ms", ctypes.c_double), ("t_join_ms", ctypes.c_double),
+                ("t_fallback_ms", ctypes.c_double), ("n_dist_evals", ctypes.c_int64),
+                ("pruning_ratio", ctypes.c_double), ("n_cluster_pairs", ctypes.c_int64),
+                ("n_overflow", ctypes.c_int64), ("exact_mode", ctypes.c_int32),
+                ("precision_used", ctypes.c_int32), ("m_q", ctypes.c_int32), ("m_r", ctypes.c_int32),
+                ("tile_q", ctypes.c_int32), ("kcap", ctypes.c_int32), ("n_tiles", ctypes.c_int64),
+                ("filter_max_abs_err", ctypes.c_double), ("reserved", ctypes.c_int32 * 8)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class Instance(ctypes.Structure):
+    _fields_ = [("R", Matrix), ("Q", Matrix), ("k", ctypes.c_int32),
+                ("out_idx", ctypes.c_void_p), ("out_dist", ctypes.c_void_p)]
+
+
+EXPORTS = ("knnj_version", "knnj_last_error", "knnj_ctx_create", "knnj_ctx_destroy",
+           "knnj_ctx_set_budget", "knnj_join")
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libknnj.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m knn_sweet_amd.build`")
+    lib = ctypes.CDLL(path)
+    lib.knnj_version.restype = ctypes.c_char_p
+    lib.knnj_last_error.restype = ctypes.c_char_p
+    lib.knnj_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    lib.knnj_ctx_create.restype = ctypes.c_int
+    lib.knnj_ctx_destroy.argtypes = [ctypes.c_void_p]
+    lib.knnj_ctx_destroy.restype = None
+    lib.knnj_ctx_set_budget.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+    lib.knnj_ctx_set_budget.restype = ctypes.c_int
+    lib.knnj_join.argtypes = [ctypes.c_void_p, Matrix, Matrix, ctypes.c_int32, ctypes.POINTER(Opts),
+                              ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Stats)]
+    lib.knnj_join.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _check(status: int):
+    if status != KNNJ_OK:
+        raise KnnjError(status, _lib.knnj_last_error().decode())
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _matrix(x):
+    """(Matrix, keepalive, on_device) for a numpy array or a torch tensor."""
+    if _is_torch(x):
+        import torch
+        if x.dtype != torch.float32 or x.dim() != 2:
+            raise ValueError("expected a 2-D float32 tensor")
+        if x.stride(1) != 1:
+            x = x.contiguous()
+        dev = 1 if x.is_cuda else 0
+        if not dev:
+            x = x.contiguous()
+        return Matrix(x.data_ptr(), x.shape[0], x.shape[1], x.stride(0), dev), x, bool(dev)
+    a = np.asarray(x)
+    if a.ndim != 2:
+        raise ValueError("expected a 2-D array")
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return Matrix(a.ctypes.data, a.shape[0], a.shape[1], a.shape[1], 0), a, False
+
+
+def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
+              brute_force=False, tile_q=0) -> Opts:
+    return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
+                tile_q)
+
+
+class Context:
+    """A device context (stream, events, workspace) on one GPU."""
+
+    def __init__(self, device: int = 0):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.knnj_ctx_create(device, ctypes.byref(h)))
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.knnj_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_budget(self, nbytes: int):
+        _check(_lib.knnj_ctx_set_budget(self._h, nbytes))
+
+    def join(self, R, Q, k: int, **opts):
+        """Exact kNN join; returns (idx, dist, stats-dict)."""
+        mR, keepR, _ = _matrix(R)
+        mQ, keepQ, qdev = _matrix(Q)
+        if R is Q:
+            mQ = mR
+        o = make_opts(**opts)
+        aos = bool(o.out_aos)
+        nq = mQ.n
+        if qdev:
+            import torch
+            dev = keepQ.device
+            idx = torch.empty((nq, k, 2) if aos else (nq, k), dtype=torch.int32, device=dev)
+            dist = None if aos else torch.empty((nq, k), dtype=torch.float32, device=dev)
+            pi, pd = idx.data_ptr(), (dist.data_ptr() if dist is not None else None)
+        else:
+            idx = np.empty((nq, k, 2) if aos else (nq, k), np.int32)
+            dist = None if aos else np.empty((nq, k), np.float32)
+            pi, pd = idx.ctypes.data, (dist.ctypes.data if dist is not None else None)
+        st = Stats()
+        _check(_lib.knnj_join(self._h, mR, mQ, k, ctypes.byref(o), pi, pd, ctypes.byref(st)))
+        del keepR, keepQ
+        return idx, dist, st.as_dict()
+
+
+_default_ctx = {}
+
+
+def join(R, Q, k: int, device: int = 0, **opts):
+    """Exact kNN join on a per-process default context of `device`."""
+    ctx = _default_ctx.get(device)
+    if ctx is None:
+        ctx = _default_ctx[device] = Context(device)
+    return ctx.join(R, Q, k, **opts)
+
+
+def version() -> str:
+    return load_library().knnj_version().decode()

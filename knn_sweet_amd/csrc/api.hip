@@ -1,0 +1,512 @@
+// api.hip -- the C ABI (include/knnj.h) and the host-side planner.
+//
+// The planner replaces the reference driver main() (src/knnjoin.cu:983-1172):
+// argument checks instead of print-and-continue (knnjoin.cu:38-41), one
+// grow-only device workspace per context instead of 3*m_Q + m_R cudaMallocs
+// that are never freed (knnjoin.cu:305-309, 347-349), one stream with hipEvent
+// stage timers instead of CLOCK_REALTIME around cudaDeviceSynchronize
+// (common8.h:83-90), and device-side sorts instead of the OpenMP
+// cudaMemcpy round trips (knnjoin.cu:380-400, 1076-1094).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/knnj.h"
+#include "knnj_common.h"
+#include "knnj_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+};
+
+uint64_t splitmix64(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+struct Status {
+  knnj_status code = KNNJ_OK;
+  bool ok() const { return code == KNNJ_OK; }
+};
+
+}  // namespace
+
+struct knnj_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<std::string, DevBuf> ws;
+  int64_t budget = 0;
+  std::vector<int32_t> host_cand[2];
+  hipEvent_t ev[8] = {};
+};
+
+#define HIPCHK(expr)                                                             \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return KNNJ_EDEVICE;                                                       \
+    }                                                                            \
+  } while (0)
+
+namespace {
+
+size_t ws_total(const knnj_ctx *c) {
+  size_t t = 0;
+  for (auto &kv : c->ws) t += kv.second.n;
+  return t;
+}
+
+// Device buffer of at least `bytes` under `name`; grows, never shrinks.
+knnj_status ws_get(knnj_ctx *c, const std::string &name, size_t bytes, void **out) {
+  bytes = std::max<size_t>(bytes, 256);
+  DevBuf &b = c->ws[name];
+  if (b.n >= bytes) {
+    *out = b.p;
+    return KNNJ_OK;
+  }
+  const size_t cur = ws_total(c) - b.n;
+  if (c->budget > 0 && (int64_t)(cur + bytes) > c->budget) {
+    set_err("workspace budget exceeded: %s needs %zu bytes, %zu in use, budget %lld", name.c_str(), bytes,
+            cur, (long long)c->budget);
+    return KNNJ_ENOMEM;
+  }
+  if (b.p) {
+    hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+  }
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    set_err("hipMalloc(%zu) for %s failed: %s", bytes, name.c_str(), hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? KNNJ_ENOMEM : KNNJ_EDEVICE;
+  }
+  b.n = bytes;
+  *out = b.p;
+  return KNNJ_OK;
+}
+
+template <class T>
+knnj_status ws(knnj_ctx *c, const std::string &name, size_t count, T **out) {
+  void *p = nullptr;
+  knnj_status s = ws_get(c, name, count * sizeof(T), &p);
+  *out = (T *)p;
+  return s;
+}
+
+#define TRY(expr)                      \
+  do {                                 \
+    knnj_status s_ = (expr);           \
+    if (s_ != KNNJ_OK) return s_;      \
+  } while (0)
+
+int pow2_at_least(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+int default_reps(int64_t n) {
+  int64_t m = (int64_t)std::llround(2.0 * std::sqrt((double)n));
+  return (int)std::max<int64_t>(1, std::min<int64_t>({m, (int64_t)KNNJ_MAX_REPS, n}));
+}
+
+// Builds one side's cluster index from device rows X.
+knnj_status build_side(knnj_ctx *c, int side, const float *X, int64_t n, int64_t ld, int d, int dpad, int m,
+                       int trials, uint64_t seed, SideIndex *out) {
+  const std::string p = side == 0 ? "r_" : "q_";
+  hipStream_t s = c->stream;
+  float *reps, *d2r, *pdist, *px;
+  u64 *radbits, *gkeys;
+  int32_t *off, *perm, *cl, *cnt, *fill, *cand, *rep_rows;
+  double *partial;
+  const int64_t nn = std::max<int64_t>(n, 1);
+  TRY(ws(c, p + "reps", (size_t)m * dpad, &reps));
+  TRY(ws(c, p + "rad", (size_t)m, &radbits));
+  TRY(ws(c, p + "off", (size_t)m + 1, &off));
+  TRY(ws(c, p + "cnt", (size_t)m, &cnt));
+  TRY(ws(c, p + "fill", (size_t)m, &fill));
+  TRY(ws(c, p + "perm", (size_t)nn, &perm));
+  TRY(ws(c, p + "cl", (size_t)nn, &cl));
+  TRY(ws(c, p + "d2r", (size_t)nn, &d2r));
+  TRY(ws(c, p + "pdist", (size_t)nn, &pdist));
+  TRY(ws(c, p + "gkeys", (size_t)nn, &gkeys));
+  TRY(ws(c, p + "px", (size_t)nn * dpad, &px));
+  TRY(ws(c, p + "cand", (size_t)m * trials, &cand));
+  TRY(ws(c, p + "reprows", (size_t)m, &rep_rows));
+  const int64_t npairs = (int64_t)m * (m - 1) / 2;
+  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(256, (npairs + 255) / 256));
+  TRY(ws(c, p + "partial", (size_t)nblk * trials, &partial));
+
+  std::vector<int32_t> &hc = c->host_cand[side];
+  hc.resize((size_t)m * trials);
+  uint64_t st = seed;
+  for (auto &v : hc) v = (int32_t)(splitmix64(st) % (uint64_t)n);
+  HIPCHK(hipMemcpyAsync(cand, hc.data(), hc.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemsetAsync(radbits, 0, sizeof(u64) * m, s));
+  HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * m, s));
+  HIPCHK(hipMemsetAsync(fill, 0, sizeof(int32_t) * m, s));
+  launch_rep_select(X, ld, d, dpad, cand, m, trials, partial, nblk, reps, rep_rows, s);
+  launch_assign(X, n, ld, d, dpad, reps, m, cl, d2r, radbits, cnt, s);
+  launch_bucket_sort_permute(X, n, ld, d, dpad, m, cl, cnt, off, fill, perm, d2r, pdist, gkeys, px, s);
+  HIPCHK(hipGetLastError());
+  out->n = n;
+  out->m = m;
+  out->dpad = dpad;
+  out->reps = reps;
+  out->rad = (const double *)radbits;
+  out->off = off;
+  out->perm = perm;
+  out->pdist = pdist;
+  out->px = px;
+  return KNNJ_OK;
+}
+
+knnj_status check_matrix(const knnj_matrix &M, const char *name) {
+  if (M.n < 0) {
+    set_err("%s.n < 0", name);
+    return KNNJ_EINVAL;
+  }
+  if (M.d < 1) {
+    set_err("%s.d < 1", name);
+    return KNNJ_EINVAL;
+  }
+  if (M.ld != 0 && M.ld < M.d) {
+    set_err("%s.ld < %s.d", name, name);
+    return KNNJ_EINVAL;
+  }
+  if (M.n > 0 && !M.data) {
+    set_err("%s.data is NULL", name);
+    return KNNJ_EINVAL;
+  }
+  if (M.n > (int64_t)INT32_MAX - 1) {
+    set_err("%s.n exceeds 2^31-2 rows", name);
+    return KNNJ_EUNSUPPORTED;
+  }
+  return KNNJ_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+  return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *knnj_version(void) { return "knnj 0.1.0 (gfx950)"; }
+
+const char *knnj_last_error(void) { return g_err.c_str(); }
+
+knnj_status knnj_ctx_create(int device, knnj_ctx **out) {
+  g_err.clear();
+  if (!out) {
+    set_err("out is NULL");
+    return KNNJ_EINVAL;
+  }
+  *out = nullptr;
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) {
+    set_err("device %d out of range (%d devices)", device, n);
+    return KNNJ_EINVAL;
+  }
+  HIPCHK(hipSetDevice(device));
+  knnj_ctx *c = new knnj_ctx();
+  c->device = device;
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    set_err("hipStreamCreate failed");
+    return KNNJ_EDEVICE;
+  }
+  for (auto &e : c->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      set_err("hipEventCreate failed");
+      knnj_ctx_destroy(c);
+      return KNNJ_EDEVICE;
+    }
+  }
+  *out = c;
+  return KNNJ_OK;
+}
+
+void knnj_ctx_destroy(knnj_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto &kv : c->ws)
+    if (kv.second.p) hipFree(kv.second.p);
+  for (auto &e : c->ev)
+    if (e) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+knnj_status knnj_ctx_set_budget(knnj_ctx *c, int64_t bytes) {
+  if (!c || bytes < 0) {
+    set_err("bad context or budget");
+    return KNNJ_EINVAL;
+  }
+  c->budget = bytes;
+  return KNNJ_OK;
+}
+
+knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, const knnj_opts *opts,
+                      int32_t *out_idx, float *out_dist, knnj_stats *stats) {
+  g_err.clear();
+  if (!c) {
+    set_err("context is NULL");
+    return KNNJ_EINVAL;
+  }
+  TRY(check_matrix(R, "R"));
+  TRY(check_matrix(Q, "Q"));
+  if (R.d != Q.d) {
+    set_err("R.d (%d) != Q.d (%d)", R.d, Q.d);
+    return KNNJ_EINVAL;
+  }
+  if (k < 1 || k > R.n) {
+    set_err("k = %d outside [1, |R| = %lld]", k, (long long)R.n);
+    return KNNJ_EINVAL;
+  }
+  if (k > KNNJ_MAX_K) {
+    set_err("k = %d above the supported maximum %d", k, KNNJ_MAX_K);
+    return KNNJ_EUNSUPPORTED;
+  }
+  knnj_opts o{};
+  if (opts) o = *opts;
+  const bool aos = o.out_aos != 0;
+  if (Q.n > 0 && (!out_idx || (!aos && !out_dist))) {
+    set_err("output pointer is NULL");
+    return KNNJ_EINVAL;
+  }
+  if (o.precision == KNNJ_PREC_FP8) {
+    set_err("FP8 filter is not built");
+    return KNNJ_EUNSUPPORTED;
+  }
+  if (o.tile_q != 0 && o.tile_q != 1 && o.tile_q != 2 && o.tile_q != 4 && o.tile_q != 8) {
+    set_err("tile_q must be 0, 1, 2, 4 or 8");
+    return KNNJ_EINVAL;
+  }
+  knnj_stats st{};
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const int d = R.d;
+  const int64_t ldR = R.ld ? R.ld : d, ldQ = Q.ld ? Q.ld : d;
+  const bool self = R.data == Q.data && R.n == Q.n && ldR == ldQ && R.on_device == Q.on_device;
+  HIPCHK(hipEventRecord(c->ev[0], s));
+
+  // device-resident inputs
+  const float *dR = R.data, *dQ = Q.data;
+  int64_t dldR = ldR, dldQ = ldQ;
+  if (!R.on_device) {
+    float *b;
+    TRY(ws(c, "in_r", (size_t)std::max<int64_t>(R.n, 1) * d, &b));
+    HIPCHK(hipMemcpy2DAsync(b, sizeof(float) * d, R.data, sizeof(float) * ldR, sizeof(float) * d, R.n,
+                            hipMemcpyHostToDevice, s));
+    dR = b;
+    dldR = d;
+  }
+  if (self) {
+    dQ = dR;
+    dldQ = dldR;
+  } else if (!Q.on_device && Q.n > 0) {
+    float *b;
+    TRY(ws(c, "in_q", (size_t)Q.n * d, &b));
+    HIPCHK(hipMemcpy2DAsync(b, sizeof(float) * d, Q.data, sizeof(float) * ldQ, sizeof(float) * d, Q.n,
+                            hipMemcpyHostToDevice, s));
+    dQ = b;
+    dldQ = d;
+  }
+
+  // input scan: finiteness, integrality, magnitude
+  u32 *flags;
+  TRY(ws(c, "flags", 8, &flags));
+  HIPCHK(hipMemsetAsync(flags, 0, 8 * sizeof(u32), s));
+  launch_scan_input(dR, R.n, d, dldR, flags, s);
+  if (!self && Q.n > 0) launch_scan_input(dQ, Q.n, d, dldQ, flags, s);
+  u32 hflags[3];
+  HIPCHK(hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (hflags[0]) {
+    set_err("non-finite value in R or Q");
+    return KNNJ_EINVAL;
+  }
+  float maxabs;
+  std::memcpy(&maxabs, &hflags[1], 4);
+  const double span = 2.0 * (double)maxabs;
+  const bool exact = !hflags[2] && (double)d * span * span < 16777216.0;
+  st.exact_mode = exact ? 1 : 0;
+  st.precision_used = KNNJ_PREC_FP32;
+  if (Q.n == 0) {
+    if (stats) *stats = st;
+    return KNNJ_OK;
+  }
+
+  const int dpad = (d + 3) & ~3;
+  const bool brute = o.brute_force != 0;
+  int m_r = brute ? 1 : (o.m_r > 0 ? std::min<int64_t>(o.m_r, R.n) : default_reps(R.n));
+  int m_q = brute ? 1 : (o.m_q > 0 ? std::min<int64_t>(o.m_q, Q.n) : default_reps(Q.n));
+  m_r = std::min(m_r, KNNJ_MAX_REPS);
+  m_q = std::min(m_q, KNNJ_MAX_REPS);
+  if (self) m_q = m_r;
+  const int trials = o.trials > 0 ? o.trials : 10;
+  const uint64_t seed = o.seed ? o.seed : 2015;
+  st.m_q = m_q;
+  st.m_r = m_r;
+
+  HIPCHK(hipEventRecord(c->ev[1], s));
+  SideIndex Ri{}, Qi{};
+  TRY(build_side(c, 0, dR, R.n, dldR, d, dpad, m_r, trials, seed, &Ri));
+  if (self)
+    Qi = Ri;
+  else
+    TRY(build_side(c, 1, dQ, Q.n, dldQ, d, dpad, m_q, trials, seed, &Qi));
+  HIPCHK(hipEventRecord(c->ev[2], s));
+
+  double *theta;
+  int32_t *plist, *pcnt;
+  TRY(ws(c, "theta", (size_t)m_q, &theta));
+  TRY(ws(c, "plist", (size_t)m_q * m_r, &plist));
+  TRY(ws(c, "pcnt", (size_t)m_q, &pcnt));
+  launch_bounds(Qi, Ri, d, k, theta, plist, pcnt, s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->ev[3], s));
+
+  // capacity, tile size, tiles
+  int kcap = exact ? pow2_at_least(std::max(k, 64)) : pow2_at_least(std::max(k + std::max(8, k / 8), 64));
+  kcap = std::min(kcap, KNNJ_MAX_K);
+  const int tq = o.tile_q ? std::min(o.tile_q, join_tile_q(kcap)) : join_tile_q(kcap);
+  const int64_t max_tiles = Q.n / tq + m_q + 1;
+  Tile *tiles, *ovf_tiles, *ovf2_tiles;
+  int32_t *ntiles, *counters;
+  u64 *n_evals;
+  TRY(ws(c, "tiles", (size_t)max_tiles, &tiles));
+  TRY(ws(c, "ovf_tiles", (size_t)Q.n, &ovf_tiles));
+  TRY(ws(c, "ovf2_tiles", (size_t)Q.n, &ovf2_tiles));
+  TRY(ws(c, "counters", 8, &counters));  // [0] ntiles, [1] ovf, [2] ovf2, [3] max_err bits
+  TRY(ws(c, "evals", 2, &n_evals));
+  ntiles = counters;
+  HIPCHK(hipMemsetAsync(counters, 0, 8 * sizeof(int32_t), s));
+  HIPCHK(hipMemsetAsync(n_evals, 0, 2 * sizeof(u64), s));
+  launch_make_tiles(Qi.off, m_q, tq, tiles, ntiles, s);
+
+  // outputs
+  int32_t *oidx = out_idx;
+  float *odist = out_dist;
+  const size_t nout = (size_t)Q.n * k;
+  if (!Q.on_device) {
+    TRY(ws(c, "out_idx", nout * (aos ? 2 : 1), &oidx));
+    if (!aos) TRY(ws(c, "out_dist", nout, &odist));
+  }
+
+  JoinArgs a{};
+  a.Q = Qi;
+  a.R = Ri;
+  a.d = d;
+  a.k = k;
+  a.kcap = kcap;
+  a.exact = exact ? 1 : 0;
+  const double delta = exact ? 0.0 : (double)(d + 4) * 0x1p-23 + 0x1p-40;
+  a.delta = delta;
+  a.fac = (1.0 + delta) / (1.0 - delta) * (1.0 + 0x1p-20);
+  a.tiles = tiles;
+  a.ntiles = ntiles;
+  a.theta = theta;
+  a.plist = plist;
+  a.pcnt = pcnt;
+  a.out_idx = oidx;
+  a.out_dist = odist;
+  a.out_mode = aos ? 2 : (o.out_squared ? 1 : 0);
+  a.n_evals = n_evals;
+  a.ovf_tiles = ovf_tiles;
+  a.ovf_count = counters + 1;
+  a.max_err_bits = (u32 *)(counters + 3);
+  HIPCHK(launch_join(a, tq, max_tiles, s));
+  HIPCHK(hipEventRecord(c->ev[4], s));
+
+  int32_t hc[4];
+  HIPCHK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  st.n_tiles = hc[0];
+  st.n_overflow = hc[1];
+  HIPCHK(hipEventRecord(c->ev[5], s));
+  if (hc[1] > 0) {
+    JoinArgs b = a;
+    b.kcap = KNNJ_MAX_K;
+    b.tiles = ovf_tiles;
+    b.ntiles = counters + 1;
+    b.ovf_tiles = ovf2_tiles;
+    b.ovf_count = counters + 2;
+    HIPCHK(launch_join(b, 1, hc[1], s));
+    int32_t h2 = 0;
+    HIPCHK(hipMemcpyAsync(&h2, counters + 2, sizeof(h2), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (h2 > 0) {
+      set_err("%d queries overflowed the largest candidate capacity (%d)", h2, KNNJ_MAX_K);
+      return KNNJ_EOVERFLOW;
+    }
+  }
+  HIPCHK(hipEventRecord(c->ev[6], s));
+  if (!Q.on_device) {
+    HIPCHK(hipMemcpyAsync(out_idx, oidx, nout * sizeof(int32_t) * (aos ? 2 : 1), hipMemcpyDeviceToHost, s));
+    if (!aos) HIPCHK(hipMemcpyAsync(out_dist, odist, nout * sizeof(float), hipMemcpyDeviceToHost, s));
+  }
+  u64 hev = 0;
+  u32 herr = 0;
+  HIPCHK(hipMemcpyAsync(&hev, n_evals, sizeof(hev), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&herr, counters + 3, sizeof(herr), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(c->ev[7], s));
+  HIPCHK(hipStreamSynchronize(s));
+
+  st.t_total_ms = elapsed(c->ev[0], c->ev[7]);
+  st.t_index_ms = elapsed(c->ev[1], c->ev[2]);
+  st.t_bounds_ms = elapsed(c->ev[2], c->ev[3]);
+  st.t_join_ms = elapsed(c->ev[3], c->ev[4]);
+  st.t_fallback_ms = hc[1] > 0 ? elapsed(c->ev[5], c->ev[6]) : 0.0;
+  st.n_dist_evals = (int64_t)hev;
+  const double tot = (double)R.n * (double)Q.n;
+  st.pruning_ratio = tot > 0 ? 1.0 - (double)hev / tot : 0.0;
+  st.tile_q = tq;
+  st.kcap = kcap;
+  float ferr;
+  std::memcpy(&ferr, &herr, 4);
+  st.filter_max_abs_err = ferr;
+  // surviving cluster pairs
+  {
+    std::vector<int32_t> hp(m_q);
+    HIPCHK(hipMemcpy(hp.data(), pcnt, sizeof(int32_t) * m_q, hipMemcpyDeviceToHost));
+    int64_t t = 0;
+    for (int32_t v : hp) t += v;
+    st.n_cluster_pairs = t;
+  }
+  if (stats) *stats = st;
+  return KNNJ_OK;
+}
+
+}  // extern "C"

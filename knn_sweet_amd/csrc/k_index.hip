@@ -1,0 +1,348 @@
+// k_index.hip -- cluster index build on gfx950.
+//
+// Replaces clusterReps (reference src/knnjoin.cu:166-411): representative
+// sampling (selectReps_cuda/_max/_copy, knnjoin.cu:125-154), the
+// cuBLAS + AddAll + findQCluster/findTCluster assignment (knnjoin.cu:43-97,
+// 270-342), the atomic member bucketing (fillQMembers/fillTMembers/
+// reorderMembers, knnjoin.cu:98-124) and the host OpenMP sort of every
+// reference cluster by distance to its representative (knnjoin.cu:380-400).
+// Here: no N x m distance matrix is materialised (argmin is fused into the
+// distance loop), radii are binary64 maxima for both sides, bucketing is a
+// counting sort, the per-cluster sort runs on the device, and the rows are
+// physically permuted into cluster order so the join streams contiguous
+// memory.
+#include "knnj_common.h"
+#include "knnj_kernels.h"
+#include "wave_sort.h"
+
+// ---------------------------------------------------------------------------
+// Input scan: non-finite values, non-integers, max |x|.
+// flags[0]: any non-finite; flags[1]: float bits of max |x|; flags[2]: any non-integer.
+__global__ void k_scan_input(const float *__restrict__ X, int64_t n, int d, int64_t ld, u32 *flags) {
+  u32 bad = 0, nonint = 0;
+  float mx = 0.f;
+  const int64_t total = n * (int64_t)d;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / d;
+    const int c = (int)(e - r * d);
+    const float x = X[r * ld + c];
+    if (!isfinite(x)) {
+      bad = 1;
+    } else {
+      mx = fmaxf(mx, fabsf(x));
+      if (x != rintf(x)) nonint = 1;
+    }
+  }
+  bad = __ballot(bad) != 0;
+  nonint = __ballot(nonint) != 0;
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) {
+    if (bad) atomicOr(&flags[0], 1u);
+    atomicMax(&flags[1], __float_as_uint(mx));
+    if (nonint) atomicOr(&flags[2], 1u);
+  }
+}
+
+void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags, hipStream_t s) {
+  const int64_t total = n * (int64_t)d;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_scan_input, dim3((unsigned)blocks), dim3(256), 0, s, X, n, d, ld, flags);
+}
+
+// ---------------------------------------------------------------------------
+// Representative sampling score: for trial t, the sum over unordered pairs
+// i < j of the FP32 distance between candidate rows (the reference sums
+// int(distance) over ordered pairs, knnjoin.cu:125-133; the argmax is the
+// same up to that truncation).  Partial sums per block, reduced in a fixed
+// order so the choice is deterministic.
+__global__ __launch_bounds__(256) void k_rep_score(const float *__restrict__ X, int64_t ld, int d,
+                                                   const int32_t *__restrict__ cand, int m,
+                                                   double *partial) {
+  const int t = blockIdx.y;
+  const int64_t npairs = (int64_t)m * (m - 1) / 2;
+  const int32_t *c = cand + (int64_t)t * m;
+  double acc = 0.0;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npairs;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    // p -> (i, j), j < i, p = i(i-1)/2 + j
+    int64_t i = (int64_t)((1.0 + sqrt(1.0 + 8.0 * (double)p)) * 0.5);
+    while (i * (i - 1) / 2 > p) --i;
+    while ((i + 1) * i / 2 <= p) ++i;
+    const int64_t j = p - i * (i - 1) / 2;
+    const float *a = X + (int64_t)c[i] * ld;
+    const float *b = X + (int64_t)c[j] * ld;
+    float s = 0.f;
+    for (int q = 0; q < d; ++q) {
+      const float df = a[q] - b[q];
+      s = __builtin_fmaf(df, df, s);
+    }
+    acc += (double)sqrtf(s);
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(int64_t)t * gridDim.x + blockIdx.x] = red[0];
+}
+
+// Picks the best trial (first maximum, as the reference's strict compare,
+// knnjoin.cu:135-148) and gathers its rows into reps[m][dpad] (zero padded).
+__global__ void k_rep_pick(const double *__restrict__ partial, int nblk, int trials,
+                           const int32_t *__restrict__ cand, int m, const float *__restrict__ X,
+                           int64_t ld, int d, int dpad, float *reps, int32_t *rep_rows) {
+  __shared__ int best;
+  if (threadIdx.x == 0) {
+    double bs = -1.0;
+    int bt = 0;
+    for (int t = 0; t < trials; ++t) {
+      double s = 0.0;
+      for (int b = 0; b < nblk; ++b) s += partial[(int64_t)t * nblk + b];
+      if (s > bs) {
+        bs = s;
+        bt = t;
+      }
+    }
+    best = bt;
+  }
+  __syncthreads();
+  const int32_t *c = cand + (int64_t)best * m;
+  for (int64_t e = threadIdx.x; e < (int64_t)m * dpad; e += blockDim.x) {
+    const int r = (int)(e / dpad), q = (int)(e - (int64_t)r * dpad);
+    reps[e] = q < d ? X[(int64_t)c[r] * ld + q] : 0.f;
+    if (q == 0 && rep_rows) rep_rows[r] = c[r];
+  }
+}
+
+void launch_rep_select(const float *X, int64_t ld, int d, int dpad, const int32_t *cand, int m,
+                       int trials, double *partial, int nblk, float *reps, int32_t *rep_rows,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_rep_score, dim3(nblk, trials), dim3(256), 0, s, X, ld, d, cand, m, partial);
+  hipLaunchKernelGGL(k_rep_pick, dim3(1), dim3(256), 0, s, partial, nblk, trials, cand, m, X, ld, d,
+                     dpad, reps, rep_rows);
+}
+
+// ---------------------------------------------------------------------------
+// Assignment: nearest representative (FP32, lowest index on ties, as
+// findQCluster's strict compare, knnjoin.cu:64-80), the binary64 distance to
+// it, the binary64 cluster radius (atomic max on the bit pattern of a
+// non-negative double) and the cluster histogram.  Representatives are
+// staged in LDS, 32 at a time, 32 dimensions at a time.
+#define AS_RB 32
+template <int AS_DC>
+__global__ __launch_bounds__(256) void k_assign(const float *__restrict__ X, int64_t n, int64_t ld, int d,
+                                                int dpad, const float *__restrict__ reps, int m,
+                                                int32_t *cl, float *d2r, u64 *radbits, int32_t *cnt) {
+  __shared__ float sr[AS_RB][AS_DC + 1];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool live = i < n;
+  const float *x = X + (live ? i : 0) * ld;
+  float best = INFINITY;
+  int bi = 0;
+  for (int r0 = 0; r0 < m; r0 += AS_RB) {
+    const int rn = min(AS_RB, m - r0);
+    float acc[AS_RB];
+#pragma unroll
+    for (int r = 0; r < AS_RB; ++r) acc[r] = 0.f;
+    for (int t0 = 0; t0 < dpad; t0 += AS_DC) {
+      const int tn = min(AS_DC, dpad - t0);
+      __syncthreads();
+      for (int e = threadIdx.x; e < AS_RB * AS_DC; e += blockDim.x) {
+        const int r = e / AS_DC, t = e % AS_DC;
+        sr[r][t] = (r < rn && t < tn) ? reps[(int64_t)(r0 + r) * dpad + t0 + t] : 0.f;
+      }
+      __syncthreads();
+      float xv[AS_DC];
+#pragma unroll
+      for (int t = 0; t < AS_DC; ++t) xv[t] = (live && t < tn && t0 + t < d) ? x[t0 + t] : 0.f;
+#pragma unroll
+      for (int r = 0; r < AS_RB; ++r) {
+        float a = acc[r];
+#pragma unroll
+        for (int t = 0; t < AS_DC; ++t) {
+          const float df = xv[t] - sr[r][t];
+          a = __builtin_fmaf(df, df, a);
+        }
+        acc[r] = a;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < AS_RB; ++r)
+      if (r < rn && acc[r] < best) {
+        best = acc[r];
+        bi = r0 + r;
+      }
+  }
+  if (!live) return;
+  const double dd = sqrt(sq64_pair(x, reps + (int64_t)bi * dpad, d));
+  cl[i] = bi;
+  d2r[i] = (float)dd;
+  atomicMax(&radbits[bi], (u64)__double_as_longlong(dd));
+  atomicAdd(&cnt[bi], 1);
+}
+
+void launch_assign(const float *X, int64_t n, int64_t ld, int d, int dpad, const float *reps, int m,
+                   int32_t *cl, float *d2r, u64 *radbits, int32_t *cnt, hipStream_t s) {
+  if (n <= 0) return;
+  const dim3 g((unsigned)((n + 255) / 256));
+  if (dpad <= 4)
+    hipLaunchKernelGGL(k_assign<4>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, m, cl, d2r, radbits, cnt);
+  else if (dpad <= 16)
+    hipLaunchKernelGGL(k_assign<16>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, m, cl, d2r, radbits, cnt);
+  else
+    hipLaunchKernelGGL(k_assign<32>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, m, cl, d2r, radbits, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of m counts (one workgroup of 1024); off[m] = total.
+__global__ __launch_bounds__(1024) void k_scan_counts(const int32_t *__restrict__ cnt, int m, int32_t *off) {
+  __shared__ int32_t part[1024];
+  const int per = (m + 1023) / 1024;
+  const int b = threadIdx.x * per;
+  int32_t s = 0;
+  for (int j = 0; j < per; ++j)
+    if (b + j < m) s += cnt[b + j];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 1; w < 1024; w <<= 1) {
+    const int32_t v = threadIdx.x >= (unsigned)w ? part[threadIdx.x - w] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int32_t run = part[threadIdx.x] - s;
+  for (int j = 0; j < per; ++j)
+    if (b + j < m) {
+      off[b + j] = run;
+      run += cnt[b + j];
+    }
+  if (threadIdx.x == 1023) off[m] = part[1023];
+}
+
+// Places each row in its cluster's segment (slot order is arbitrary here;
+// the segment sort below makes it deterministic).
+__global__ void k_scatter(const int32_t *__restrict__ cl, int64_t n, const int32_t *__restrict__ off,
+                          int32_t *fill, int32_t *perm) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int c = cl[i];
+  perm[off[c] + atomicAdd(&fill[c], 1)] = (int32_t)i;
+}
+
+// Sorts every cluster segment ascending by (distance to rep, original row)
+// and writes pdist.  Segments up to SEG_LDS keys sort in LDS; longer ones
+// sort in place in `gkeys` (global, same indexing as perm).
+#define SEG_LDS 8192
+__device__ __forceinline__ void cex(u64 *a, int i, int j) {
+  const u64 x = a[i], y = a[j];
+  if (y < x) {
+    a[i] = y;
+    a[j] = x;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_segsort(const int32_t *__restrict__ off, int32_t *perm,
+                                                  const float *__restrict__ d2r, float *pdist,
+                                                  u64 *gkeys) {
+  __shared__ u64 sk[SEG_LDS];
+  const int c = blockIdx.x;
+  const int s0 = off[c], n = off[c + 1] - s0;
+  if (n <= 0) return;
+  int p2 = 1;
+  while (p2 < n) p2 <<= 1;
+  const bool in_lds = p2 <= SEG_LDS;
+  u64 *a = in_lds ? sk : gkeys + s0;
+  for (int j = threadIdx.x; j < (in_lds ? p2 : n); j += blockDim.x) {
+    if (j < n) {
+      const int row = perm[s0 + j];
+      a[j] = ((u64)__float_as_uint(d2r[row]) << 32) | (u32)row;
+    } else {
+      a[j] = KEY_MAX;
+    }
+  }
+  __syncthreads();
+  for (int size = 2; size <= p2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int mask = (stride == (size >> 1)) ? (size - 1) : stride;
+      for (int i = threadIdx.x; i < p2; i += blockDim.x) {
+        const int j = i ^ mask;
+        if (j > i && j < (in_lds ? p2 : n)) cex(a, i, j);
+      }
+      __syncthreads();
+    }
+  }
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const u64 key = a[j];
+    perm[s0 + j] = (int32_t)(u32)key;
+    pdist[s0 + j] = __uint_as_float((u32)(key >> 32));
+  }
+}
+
+// Permuted, zero-padded copy of the rows: px[p][0..dpad).
+__global__ void k_permute(const float *__restrict__ X, int64_t ld, int d, int dpad,
+                          const int32_t *__restrict__ perm, int64_t n, float *px) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= n * dpad) return;
+  const int64_t p = e / dpad;
+  const int t = (int)(e - p * dpad);
+  px[e] = t < d ? X[(int64_t)perm[p] * ld + t] : 0.f;
+}
+
+void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, int dpad, int m,
+                                const int32_t *cl, const int32_t *cnt, int32_t *off, int32_t *fill,
+                                int32_t *perm, const float *d2r, float *pdist, u64 *gkeys, float *px,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, cnt, m, off);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cl, n, off, fill,
+                     perm);
+  hipLaunchKernelGGL(k_segsort, dim3(m), dim3(1024), 0, s, off, perm, d2r, pdist, gkeys);
+  const int64_t tot = n * dpad;
+  hipLaunchKernelGGL(k_permute, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, X, ld, d, dpad,
+                     perm, n, px);
+}
+
+// ---------------------------------------------------------------------------
+// Query tiles: cluster A of the query side is cut into ceil(|A| / tq) tiles
+// of consecutive permuted queries.  One workgroup of 1024.
+__global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__ off, int m, int tq,
+                                                     Tile *tiles, int32_t *ntiles) {
+  __shared__ int32_t part[1024];
+  const int per = (m + 1023) / 1024;
+  const int b = threadIdx.x * per;
+  int32_t s = 0;
+  for (int j = 0; j < per; ++j)
+    if (b + j < m) s += (off[b + j + 1] - off[b + j] + tq - 1) / tq;
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 1; w < 1024; w <<= 1) {
+    const int32_t v = threadIdx.x >= (unsigned)w ? part[threadIdx.x - w] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int32_t run = part[threadIdx.x] - s;
+  for (int j = 0; j < per; ++j) {
+    const int c = b + j;
+    if (c >= m) break;
+    const int a0 = off[c], na = off[c + 1] - a0;
+    for (int q = 0; q < na; q += tq) {
+      Tile t;
+      t.start = a0 + q;
+      t.count = min(tq, na - q);
+      t.cluster = c;
+      t.pad = 0;
+      tiles[run++] = t;
+    }
+  }
+  if (threadIdx.x == 1023) *ntiles = part[1023];
+}
+
+void launch_make_tiles(const int32_t *off, int m, int tq, Tile *tiles, int32_t *ntiles, hipStream_t s) {
+  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, s, off, m, tq, tiles, ntiles);
+}

@@ -1,0 +1,312 @@
+// k_join.hip -- the join proper: VALU distance tiles, per-query wave k-select,
+// exact rerank.
+//
+// Replaces KNNQuery_base (reference src/knnjoin.cu:762-858) and the
+// small-batch KNNQuery_theta / KNNQuery / final path (knnjoin.cu:859-979).
+// The reference runs one thread per query over a random gather of R rows,
+// with an insertion-sorted K-list in per-thread scratch (IndexDist
+// knearest[1000]).  Here one wave64 owns a tile of TQ queries of one query
+// cluster and streams the surviving reference clusters (contiguous,
+// permuted rows) 64 candidates at a time, one candidate per lane:
+//   - per (tile, cluster): distance of each query to the representative,
+//     the cluster lower bound d(q,b) - r_b > theta_q skips the query, and the
+//     point-level bound |d(q,b) - d(s,b)| <= theta_q (the reference's
+//     break/continue, knnjoin.cu:796-800) is applied per 64-row chunk
+//     through the ascending member distances;
+//   - candidate keys (squared distance, row) below the query's threshold are
+//     appended to a per-query LDS buffer (ballot + mbcnt, no atomics); a full
+//     buffer is bitonic-sorted and rank-merged into the query's sorted
+//     KCAP-key list, and the threshold drops to the list's k-th key.
+// Integer inputs (exact mode): binary32 squared distances are exact, the key
+// row is the original row, the list is the answer.  Real-valued inputs: the
+// binary32 key is a filter with relative error <= delta; every candidate
+// within fac = (1+delta)/(1-delta) of the k-th filter key is kept, then
+// re-scored with the oracle's binary64 sequence and sorted by (binary64
+// squared distance, original row).  A list whose last key is still within
+// that window may have dropped a needed candidate: the query is reported as
+// overflowed and re-run at a larger capacity by the host.
+#include "knnj_common.h"
+#include "knnj_kernels.h"
+#include "wave_sort.h"
+
+template <int TQ, int KCAP, bool EXACT>
+__global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
+  extern __shared__ u64 smem[];
+  u64 *lists = smem;                       // [TQ][KCAP]
+  u64 *bufs = smem + TQ * KCAP;            // [TQ][BCAP]
+  u64 *tmp = bufs + TQ * KNNJ_BCAP;        // [KCAP]
+  const int lane = threadIdx.x;
+  if ((int)blockIdx.x >= *a.ntiles) return;
+  const Tile tile = a.tiles[blockIdx.x];
+  const int A = tile.cluster;
+  const int nq = tile.count;
+  const int d = a.d, dpR = a.R.dpad, dpQ = a.Q.dpad, k = a.k;
+  const float eps = dist_relerr(d);
+
+#pragma unroll
+  for (int i = 0; i < TQ; ++i)
+#pragma unroll
+    for (int u = 0; u < KCAP / 64; ++u) lists[i * KCAP + lane + 64 * u] = KEY_MAX;
+  __syncthreads();
+
+  const double th = a.theta[A];
+  const float T0 = (float)(th * th * (1.0 + a.delta) * (1.0 + 1e-6));
+  u64 thr[TQ];
+  int cnt[TQ];
+  const float *qrow[TQ];
+#pragma unroll
+  for (int i = 0; i < TQ; ++i) {
+    thr[i] = ((u64)__float_as_uint(T0) << 32) | 0xFFFFFFFFull;
+    cnt[i] = 0;
+    qrow[i] = a.Q.px + (int64_t)(tile.start + (i < nq ? i : 0)) * dpQ;
+  }
+  u64 evals = 0;
+
+  auto merge = [&](int i) {
+    wave_merge<KCAP>(lists + i * KCAP, bufs + i * KNNJ_BCAP, cnt[i], tmp);
+    cnt[i] = 0;
+    const u64 kth = lists[i * KCAP + k - 1];
+    if (kth != KEY_MAX) {
+      u64 t;
+      if (EXACT) {
+        t = kth - 1;
+      } else {
+        const float T = (float)((double)key_dist(kth) * a.fac);
+        t = ((u64)__float_as_uint(T) << 32) | 0xFFFFFFFFull;
+      }
+      if (t < thr[i]) thr[i] = t;
+    }
+  };
+
+  const int np = a.pcnt[A];
+  const int32_t *pl = a.plist + (int64_t)A * a.R.m;
+  for (int pi = 0; pi < np; ++pi) {
+    const int b = pl[pi];
+    const int s0 = a.R.off[b], s1 = a.R.off[b + 1];
+    const float rb = (float)a.R.rad[b] * (1.f + 2.4e-7f);
+    const float *rep = a.R.reps + (int64_t)b * dpR;
+    bool use[TQ];
+    bool any = false;
+    float lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+    for (int i = 0; i < TQ; ++i) {
+      use[i] = false;
+      if (i < nq) {
+        float part = 0.f;
+        for (int j = lane; j < d; j += 64) {
+          const float df = qrow[i][j] - rep[j];
+          part = __builtin_fmaf(df, df, part);
+        }
+        const float dqb = sqrtf(wave_sum(part));
+        const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
+        const float lb = dqb * (1.f - 2.f * eps) - rb;
+        if (!(lb > thi)) {
+          use[i] = true;
+          any = true;
+          lo = fminf(lo, dqb * (1.f - 2.f * eps) - thi);
+          hi = fmaxf(hi, dqb * (1.f + 2.f * eps) + thi);
+        }
+      }
+    }
+    if (!any) continue;
+    lo = lo > 0.f ? lo * (1.f - eps) : -INFINITY;
+    hi = hi * (1.f + eps);
+
+    const int nch = (s1 - s0 + 63) >> 6;
+    for (int cg = 0; cg < nch; cg += 64) {
+      const int c = cg + lane;
+      const float cmin = c < nch ? a.R.pdist[s0 + 64 * c] : INFINITY;
+      float nmin = __shfl_down(cmin, 1);
+      if (lane == 63) nmin = (c + 1 < nch) ? a.R.pdist[s0 + 64 * (c + 1)] : INFINITY;
+      const bool need = (c < nch) && (cmin <= hi) && (nmin >= lo);
+      const u64 m = wave_ballot(need);
+      if (m == 0) {
+        if (__shfl(cmin, 0) > hi) break;
+        continue;
+      }
+      const int cf = cg + __builtin_ctzll(m);
+      const int cl = cg + 63 - __builtin_clzll(m);
+      for (int c2 = cf; c2 <= cl; ++c2) {
+        const int p = s0 + 64 * c2 + lane;
+        const bool valid = p < s1;
+        const int pp = valid ? p : s0;
+        const u32 row = EXACT ? (u32)a.R.perm[pp] : (u32)pp;
+        const float *rp = a.R.px + (int64_t)pp * dpR;
+        float acc[TQ];
+#pragma unroll
+        for (int i = 0; i < TQ; ++i) acc[i] = 0.f;
+        for (int t = 0; t < dpR; t += 4) {
+          const float4 r4 = *(const float4 *)(rp + t);
+#pragma unroll
+          for (int i = 0; i < TQ; ++i) {
+            if (use[i]) {
+              const float4 q4 = *(const float4 *)(qrow[i] + t);
+              float df = q4.x - r4.x;
+              acc[i] = __builtin_fmaf(df, df, acc[i]);
+              df = q4.y - r4.y;
+              acc[i] = __builtin_fmaf(df, df, acc[i]);
+              df = q4.z - r4.z;
+              acc[i] = __builtin_fmaf(df, df, acc[i]);
+              df = q4.w - r4.w;
+              acc[i] = __builtin_fmaf(df, df, acc[i]);
+            }
+          }
+        }
+        const int nvalid = __popcll(wave_ballot(valid));
+#pragma unroll
+        for (int i = 0; i < TQ; ++i) {
+          if (use[i]) {
+            const u64 key = valid ? make_key(acc[i], row) : KEY_MAX;
+            const bool pass = key <= thr[i];
+            const u64 pm = wave_ballot(pass);
+            if (pm) {
+              if (pass) bufs[i * KNNJ_BCAP + cnt[i] + (int)lanes_below(pm)] = key;
+              cnt[i] += __popcll(pm);
+            }
+            evals += nvalid;
+          }
+        }
+        bool need_merge = false;
+#pragma unroll
+        for (int i = 0; i < TQ; ++i) need_merge |= cnt[i] >= 64;
+        if (need_merge) {
+          __syncthreads();
+#pragma unroll
+          for (int i = 0; i < TQ; ++i)
+            if (cnt[i] >= 64) merge(i);
+        }
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < TQ; ++i)
+    if (cnt[i] > 0) merge(i);
+  __syncthreads();
+
+  // ---- results
+#pragma unroll
+  for (int i = 0; i < TQ; ++i) {
+    if (i >= nq) break;
+    const int64_t qo = a.Q.perm[tile.start + i];
+    const u64 *L = lists + i * KCAP;
+    if (EXACT) {
+      for (int j = lane; j < k; j += 64) {
+        const u64 key = L[j];
+        const float s = key_dist(key);
+        const int32_t row = (int32_t)key_row(key);
+        const float dist = a.out_mode == 1 ? s : (float)sqrt((double)s);
+        if (a.out_mode == 2) {
+          a.out_idx[(qo * k + j) * 2] = row;
+          a.out_idx[(qo * k + j) * 2 + 1] = __float_as_int(dist);
+        } else {
+          a.out_idx[qo * k + j] = row;
+          a.out_dist[qo * k + j] = dist;
+        }
+      }
+    } else {
+      const float Tf = key_dist(thr[i]);
+      const u64 last = L[KCAP - 1];
+      if (last != KEY_MAX && key_dist(last) <= Tf) {
+        if (lane == 0) {
+          const int slot = atomicAdd(a.ovf_count, 1);
+          Tile t;
+          t.start = tile.start + i;
+          t.count = 1;
+          t.cluster = A;
+          t.pad = 0;
+          a.ovf_tiles[slot] = t;
+        }
+        continue;
+      }
+      constexpr int E = KCAP / 64;
+      u64 hs[E];
+      u32 rs[E];
+      float err = 0.f;
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        const u64 key = L[lane + 64 * u];
+        const bool ok = key != KEY_MAX && key_dist(key) <= Tf;
+        hs[u] = KEY_MAX;
+        rs[u] = 0xFFFFFFFFu;
+        if (ok) {
+          const int p = (int)key_row(key);
+          const double s64 = sq64_pair(qrow[i], a.R.px + (int64_t)p * dpR, d);
+          hs[u] = (u64)__double_as_longlong(s64);
+          rs[u] = (u32)a.R.perm[p];
+          err = fmaxf(err, (float)fabs((double)key_dist(key) - s64));
+        }
+      }
+      wave_sort_pair<E>(hs, rs);
+      err = wave_max(err);
+      if (lane == 0 && err > 0.f) atomicMax(a.max_err_bits, __float_as_uint(err));
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        const int j = lane + 64 * u;
+        if (j < k) {
+          const double s64 = __longlong_as_double((long long)hs[u]);
+          const float dist = a.out_mode == 1 ? (float)s64 : (float)sqrt(s64);
+          if (a.out_mode == 2) {
+            a.out_idx[(qo * k + j) * 2] = (int32_t)rs[u];
+            a.out_idx[(qo * k + j) * 2 + 1] = __float_as_int(dist);
+          } else {
+            a.out_idx[qo * k + j] = (int32_t)rs[u];
+            a.out_dist[qo * k + j] = dist;
+          }
+        }
+      }
+    }
+  }
+  if (lane == 0 && evals) atomicAdd(a.n_evals, evals);
+}
+
+int join_tile_q(int kcap) {
+  if (kcap <= 64) return 8;
+  if (kcap <= 256) return 4;
+  if (kcap <= 512) return 2;
+  return 1;
+}
+
+size_t join_lds_bytes(int tq, int kcap) {
+  return sizeof(u64) * ((size_t)tq * kcap + (size_t)tq * KNNJ_BCAP + (size_t)kcap);
+}
+
+template <int TQ, int KCAP>
+static hipError_t launch_tq_kcap(const JoinArgs &a, int64_t grid, hipStream_t s) {
+  const size_t lds = join_lds_bytes(TQ, KCAP);
+  if (a.exact)
+    hipLaunchKernelGGL((k_join_valu<TQ, KCAP, true>), dim3((unsigned)grid), dim3(64), lds, s, a);
+  else
+    hipLaunchKernelGGL((k_join_valu<TQ, KCAP, false>), dim3((unsigned)grid), dim3(64), lds, s, a);
+  return hipGetLastError();
+}
+
+template <int KCAP>
+static hipError_t launch_kcap(const JoinArgs &a, int tq, int64_t grid, hipStream_t s) {
+  switch (tq) {
+    case 8:
+      if constexpr (KCAP <= 256) return launch_tq_kcap<8, KCAP>(a, grid, s);
+      break;
+    case 4:
+      if constexpr (KCAP <= 256) return launch_tq_kcap<4, KCAP>(a, grid, s);
+      break;
+    case 2:
+      if constexpr (KCAP <= 512) return launch_tq_kcap<2, KCAP>(a, grid, s);
+      break;
+    case 1:
+      return launch_tq_kcap<1, KCAP>(a, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_join(const JoinArgs &a, int tq, int64_t max_tiles, hipStream_t s) {
+  if (max_tiles <= 0) return hipSuccess;
+  switch (a.kcap) {
+    case 64: return launch_kcap<64>(a, tq, max_tiles, s);
+    case 128: return launch_kcap<128>(a, tq, max_tiles, s);
+    case 256: return launch_kcap<256>(a, tq, max_tiles, s);
+    case 512: return launch_kcap<512>(a, tq, max_tiles, s);
+    case 1024: return launch_kcap<1024>(a, tq, max_tiles, s);
+  }
+  return hipErrorInvalidValue;
+}

@@ -1,0 +1,104 @@
+// knnj_common.h -- types and wave64 helpers shared by the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef unsigned long long u64;
+typedef uint32_t u32;
+
+#define KNNJ_WAVE 64
+#define KEY_MAX 0xFFFFFFFFFFFFFFFFull
+#define KNNJ_MAX_REPS 4096  // representatives per side (LDS-resident lists in the bounds kernel)
+#define KNNJ_MAX_K 1024
+#define KNNJ_BCAP 128       // per-query candidate buffer (two 64-candidate chunks)
+
+// Candidate key: the squared distance (a non-negative binary32, whose bit
+// pattern orders like its value) in the high word, a row id in the low word.
+// Lexicographic u64 order is then the (squared distance, row) order.
+__device__ __forceinline__ u64 make_key(float s, u32 row) {
+  return ((u64)__float_as_uint(s) << 32) | (u64)row;
+}
+__device__ __forceinline__ float key_dist(u64 k) { return __uint_as_float((u32)(k >> 32)); }
+__device__ __forceinline__ u32 key_row(u64 k) { return (u32)k; }
+
+__device__ __forceinline__ u32 lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// number of set bits of `mask` below this lane
+__device__ __forceinline__ u32 lanes_below(u64 mask) {
+  return __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
+}
+
+__device__ __forceinline__ u64 wave_ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ u64 shfl_xor_u64(u64 v, int m) {
+  u32 lo = (u32)v, hi = (u32)(v >> 32);
+  lo = __shfl_xor(lo, m);
+  hi = __shfl_xor(hi, m);
+  return ((u64)hi << 32) | lo;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m));
+  return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m));
+  return v;
+}
+
+// Relative error bound of a binary32 Euclidean distance sqrt(sum_j (a_j-b_j)^2)
+// over d dimensions (terms non-negative, any summation order, with or without
+// FMA): (d+2) u for the squared sum, half that plus u after sqrt.  Doubled
+// for margin.
+__host__ __device__ __forceinline__ float dist_relerr(int d) { return (float)(d + 4) * 1.1920929e-7f; }
+
+// The binary64 key of one pair, in the oracle's fixed operation sequence
+// (ascending dimension, no fused multiply-add).
+__device__ __forceinline__ double sq64_pair(const float *__restrict__ q, const float *__restrict__ r, int d) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+  for (int j = 0; j < d; ++j) {
+    double t = (double)q[j] - (double)r[j];
+    s = s + t * t;
+  }
+  return s;
+}
+
+// Per-side cluster index in device memory (all arrays device pointers).
+struct SideIndex {
+  int64_t n;          // rows
+  int32_t m;          // clusters (representatives)
+  int32_t dpad;       // padded row length of the permuted copy
+  const float *reps;  // [m][dpad] representative rows (FP32, zero padded)
+  const double *rad;  // [m] cluster radius (binary64, max member distance to rep)
+  const int32_t *off; // [m+1] cluster start in permuted order
+  const int32_t *perm;// [n] permuted position -> original row
+  const float *pdist; // [n] distance to rep in permuted order, ascending per cluster
+  const float *px;    // [n][dpad] permuted rows
+};
+
+// A query tile: `count` consecutive permuted queries of cluster `cluster`.
+struct Tile {
+  int32_t start;
+  int32_t count;
+  int32_t cluster;
+  int32_t pad;
+};

@@ -1,0 +1,50 @@
+// knnj_kernels.h -- host-side launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "knnj_common.h"
+
+// k_index.hip
+void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags, hipStream_t s);
+void launch_rep_select(const float *X, int64_t ld, int d, int dpad, const int32_t *cand, int m, int trials,
+                       double *partial, int nblk, float *reps, int32_t *rep_rows, hipStream_t s);
+void launch_assign(const float *X, int64_t n, int64_t ld, int d, int dpad, const float *reps, int m,
+                   int32_t *cl, float *d2r, u64 *radbits, int32_t *cnt, hipStream_t s);
+void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, int dpad, int m,
+                                const int32_t *cl, const int32_t *cnt, int32_t *off, int32_t *fill,
+                                int32_t *perm, const float *d2r, float *pdist, u64 *gkeys, float *px,
+                                hipStream_t s);
+void launch_make_tiles(const int32_t *off, int m, int tq, Tile *tiles, int32_t *ntiles, hipStream_t s);
+
+// k_bounds.hip
+void launch_bounds(const SideIndex &Qi, const SideIndex &Ri, int d, int k, double *theta, int32_t *plist,
+                   int32_t *pcnt, hipStream_t s);
+
+// k_join.hip
+struct JoinArgs {
+  SideIndex Q, R;
+  int d;
+  int k;
+  int kcap;
+  int exact;             // 1: integer mode (FP32 keys exact, key row = original row)
+  double fac;            // real mode: threshold factor (1+delta)/(1-delta), with margin
+  double delta;          // real mode: relative error bound of the FP32 squared distance
+  const Tile *tiles;
+  const int32_t *ntiles;
+  const double *theta;   // [mQ]
+  const int32_t *plist;  // [mQ][mR]
+  const int32_t *pcnt;   // [mQ]
+  int32_t *out_idx;      // [nQ][k] or IndexDist [nQ][k] when out_mode == 2
+  float *out_dist;
+  int out_mode;          // 0 Euclidean, 1 squared, 2 AoS IndexDist (Euclidean)
+  u64 *n_evals;
+  Tile *ovf_tiles;       // overflowed queries, as one-query tiles
+  int32_t *ovf_count;
+  u32 *max_err_bits;     // max |s32 - s64| seen in rerank (float bits)
+};
+
+// Picks the wave tile size for a candidate capacity; returns the dynamic LDS bytes.
+int join_tile_q(int kcap);
+size_t join_lds_bytes(int tq, int kcap);
+hipError_t launch_join(const JoinArgs &a, int tq, int64_t max_tiles, hipStream_t s);

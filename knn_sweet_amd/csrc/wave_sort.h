@@ -1,0 +1,122 @@
+// wave_sort.h -- wave64 bitonic sorting and merging of u64 keys.
+//
+// A wave holds E*64 elements, element e = lane + 64*u in register v[u].
+// The network is the direction-free bitonic sort: within each merge of size
+// `size`, the first step pairs e with e ^ (size-1) (the mirror element) and
+// later steps pair e with e ^ stride; every compare-exchange puts the
+// smaller key at the smaller index, so padding with KEY_MAX at the end of the
+// array never moves a real key into the padding.
+#pragma once
+
+#include "knnj_common.h"
+
+template <int E>
+__device__ __forceinline__ void wave_sort_u64(u64 (&v)[E]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int size = 2; size <= 64 * E; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int mask = (stride == (size >> 1)) ? (size - 1) : stride;
+      const int lm = mask & 63, um = mask >> 6;
+      u64 nv[E];
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        u64 other = v[u ^ um];
+        if (lm) other = shfl_xor_u64(other, lm);
+        const int e = lane + 64 * u;
+        const bool lower = e < (e ^ mask);
+        const u64 mn = v[u] < other ? v[u] : other;
+        const u64 mx = v[u] < other ? other : v[u];
+        nv[u] = lower ? mn : mx;
+      }
+#pragma unroll
+      for (int u = 0; u < E; ++u) v[u] = nv[u];
+    }
+  }
+}
+
+// Same network on (hi, lo) pairs compared lexicographically.
+template <int E>
+__device__ __forceinline__ void wave_sort_pair(u64 (&hi)[E], u32 (&lo)[E]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int size = 2; size <= 64 * E; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int mask = (stride == (size >> 1)) ? (size - 1) : stride;
+      const int lm = mask & 63, um = mask >> 6;
+      u64 nh[E];
+      u32 nl[E];
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        u64 oh = hi[u ^ um];
+        u32 ol = lo[u ^ um];
+        if (lm) {
+          oh = shfl_xor_u64(oh, lm);
+          ol = __shfl_xor(ol, lm);
+        }
+        const int e = lane + 64 * u;
+        const bool lower = e < (e ^ mask);
+        const bool mine_less = (hi[u] < oh) || (hi[u] == oh && lo[u] < ol);
+        const bool take_mine = (lower == mine_less);
+        nh[u] = take_mine ? hi[u] : oh;
+        nl[u] = take_mine ? lo[u] : ol;
+      }
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        hi[u] = nh[u];
+        lo[u] = nl[u];
+      }
+    }
+  }
+}
+
+// number of elements of sorted a[0..n) that are < x (LT) or <= x (!LT);
+// p2 is a power of two >= n.
+template <bool LT>
+__device__ __forceinline__ int count_below(const u64 *a, int n, int p2, u64 x) {
+  int c = 0;
+  for (int s = p2; s > 0; s >>= 1) {
+    const int t = c + s;
+    if (t <= n) {
+      const u64 y = a[t - 1];
+      if (LT ? (y < x) : (y <= x)) c = t;
+    }
+  }
+  return c;
+}
+
+// Merges the unsorted candidate buffer buf[0..cnt) (cnt <= KNNJ_BCAP) into
+// the sorted list[0..KCAP), keeping the KCAP smallest keys.  Keys in list and
+// buf are distinct except for KEY_MAX padding.  tmp: KCAP scratch keys.  The
+// caller's workgroup is this single wave.
+template <int KCAP>
+__device__ __forceinline__ void wave_merge(u64 *list, u64 *buf, int cnt, u64 *tmp) {
+  const int lane = threadIdx.x & 63;
+  u64 v[2];
+  v[0] = lane < cnt ? buf[lane] : KEY_MAX;
+  v[1] = lane + 64 < cnt ? buf[lane + 64] : KEY_MAX;
+  wave_sort_u64<2>(v);
+  buf[lane] = v[0];
+  buf[lane + 64] = v[1];
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < KCAP / 64; ++u) {
+    const int j = lane + 64 * u;
+    const u64 x = list[j];
+    const int pos = j + count_below<true>(buf, KNNJ_BCAP, KNNJ_BCAP, x);
+    if (pos < KCAP) tmp[pos] = x;
+  }
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int j = lane + 64 * u;
+    const u64 x = v[u];
+    const int pos = j + count_below<false>(list, KCAP, KCAP, x);
+    if (pos < KCAP) tmp[pos] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < KCAP / 64; ++u) list[lane + 64 * u] = tmp[lane + 64 * u];
+  __syncthreads();
+}
