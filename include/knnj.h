@@ -49,9 +49,15 @@ typedef enum {
 typedef enum {
   KNNJ_PREC_AUTO = 0, /* BF16 filter for d >= 64 real-valued data, FP32 otherwise */
   KNNJ_PREC_FP32 = 1, /* plain FP32 distance tiles */
-  KNNJ_PREC_BF16 = 2, /* BF16 MFMA filter + exact rerank */
+  KNNJ_PREC_BF16 = 2, /* BF16 MFMA filter + exact rerank (implies the MFMA kernel) */
   KNNJ_PREC_FP8 = 3   /* reserved: FP8 filter (not built) */
 } knnj_precision;
+
+typedef enum {
+  KNNJ_KERNEL_AUTO = 0, /* MFMA for 32 <= d <= 256, VALU otherwise */
+  KNNJ_KERNEL_VALU = 1, /* direct-difference FP32 tiles on the vector ALUs (any d) */
+  KNNJ_KERNEL_MFMA = 2  /* GEMM-form tiles on the matrix cores (d <= 256) */
+} knnj_kernel;
 
 typedef struct knnj_ctx knnj_ctx;
 typedef struct knnj_index knnj_index;
@@ -75,8 +81,9 @@ typedef struct {
   int32_t out_aos;     /* 1: out_idx is IndexDist{int32 index; float dist}[nQ][k]
                           (include/common8.h:37-40); out_dist is then ignored */
   int32_t brute_force; /* 1: one cluster per side (no pruning) */
-  int32_t tile_q;      /* queries per wave tile, 0 = auto */
-  int32_t reserved[6];
+  int32_t tile_q;      /* queries per wave tile of the VALU kernel, 0 = auto */
+  int32_t kernel;      /* knnj_kernel: distance-tile implementation, 0 = auto */
+  int32_t reserved[5];
 } knnj_opts;
 
 /* Device-side timings (hipEvents on the context's stream) and counters. */
@@ -96,8 +103,11 @@ typedef struct {
   int32_t tile_q;           /* queries per wave tile used */
   int32_t kcap;             /* candidate capacity per query used */
   int64_t n_tiles;          /* query tiles launched */
-  double filter_max_abs_err;/* largest |filter distance^2 - binary64 distance^2| seen in rerank */
-  int32_t reserved[8];
+  double filter_max_abs_err;/* largest |list key - binary64 distance^2| over re-scored
+                               candidates (VALU: the FP32 filter error; MFMA: the
+                               filter error plus the bound's width) */
+  int32_t kernel_used;      /* knnj_kernel actually run */
+  int32_t reserved[7];
 } knnj_stats;
 
 /* One independent join of a batch (knnj_join_batch). */

@@ -26,6 +26,7 @@ LIB_PATH = os.path.join(HERE, "libknnj.so")
 
 KNNJ_OK, KNNJ_EINVAL, KNNJ_ENOMEM, KNNJ_EDEVICE, KNNJ_EUNSUPPORTED, KNNJ_EOVERFLOW = range(6)
 PREC_AUTO, PREC_FP32, PREC_BF16, PREC_FP8 = range(4)
+KERNEL_AUTO, KERNEL_VALU, KERNEL_MFMA = range(3)
 _STATUS_NAMES = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EUNSUPPORTED", 5: "EOVERFLOW"}
 
 
@@ -45,7 +46,7 @@ class Opts(ctypes.Structure):
                 ("precision", ctypes.c_int32), ("seed", ctypes.c_uint64),
                 ("out_squared", ctypes.c_int32), ("out_aos", ctypes.c_int32),
                 ("brute_force", ctypes.c_int32), ("tile_q", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 6)]
+                ("kernel", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
 
 
 class Stats(ctypes.Structure):
@@ -56,7 +57,8 @@ class Stats(ctypes.Structure):
                 ("n_overflow", ctypes.c_int64), ("exact_mode", ctypes.c_int32),
                 ("precision_used", ctypes.c_int32), ("m_q", ctypes.c_int32), ("m_r", ctypes.c_int32),
                 ("tile_q", ctypes.c_int32), ("kcap", ctypes.c_int32), ("n_tiles", ctypes.c_int64),
-                ("filter_max_abs_err", ctypes.c_double), ("reserved", ctypes.c_int32 * 8)]
+                ("filter_max_abs_err", ctypes.c_double), ("kernel_used", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 7)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
@@ -125,9 +127,9 @@ def _matrix(x):
 
 
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
-              brute_force=False, tile_q=0) -> Opts:
+              brute_force=False, tile_q=0, kernel=KERNEL_AUTO) -> Opts:
     return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
-                tile_q)
+                tile_q, kernel)
 
 
 class Context:

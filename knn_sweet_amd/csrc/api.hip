@@ -360,9 +360,32 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   float maxabs;
   std::memcpy(&maxabs, &hflags[1], 4);
   const double span = 2.0 * (double)maxabs;
-  const bool exact = !hflags[2] && (double)d * span * span < 16777216.0;
+  const bool integral = !hflags[2];
+  const bool exact_valu = integral && (double)d * span * span < 16777216.0;
+  const bool exact_mfma = integral && 4.0 * (double)d * span * span < 16777216.0;
+  int kind = o.kernel;
+  if (o.precision == KNNJ_PREC_BF16) {
+    if (kind == KNNJ_KERNEL_VALU) {
+      set_err("the BF16 filter runs on the MFMA kernel only");
+      return KNNJ_EINVAL;
+    }
+    kind = KNNJ_KERNEL_MFMA;
+  }
+  if (kind == KNNJ_KERNEL_AUTO) kind = (d >= 32 && d <= 256) ? KNNJ_KERNEL_MFMA : KNNJ_KERNEL_VALU;
+  if (kind != KNNJ_KERNEL_VALU && kind != KNNJ_KERNEL_MFMA) {
+    set_err("unknown kernel %d", o.kernel);
+    return KNNJ_EINVAL;
+  }
+  if (kind == KNNJ_KERNEL_MFMA && d > 256) {
+    set_err("the MFMA kernel supports d <= 256 (d = %d)", d);
+    return KNNJ_EUNSUPPORTED;
+  }
+  const bool mfma = kind == KNNJ_KERNEL_MFMA;
+  const bool bf16 = mfma && (o.precision == KNNJ_PREC_BF16 || (o.precision == KNNJ_PREC_AUTO && d >= 64 && !exact_mfma));
+  const bool exact = mfma ? (!bf16 && exact_mfma) : exact_valu;
   st.exact_mode = exact ? 1 : 0;
-  st.precision_used = KNNJ_PREC_FP32;
+  st.precision_used = bf16 ? KNNJ_PREC_BF16 : KNNJ_PREC_FP32;
+  st.kernel_used = kind;
   if (Q.n == 0) {
     if (stats) *stats = st;
     return KNNJ_OK;
@@ -387,6 +410,15 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
     Qi = Ri;
   else
     TRY(build_side(c, 1, dQ, Q.n, dldQ, d, dpad, m_q, trials, seed, &Qi));
+  const int dk = mfma ? (bf16 ? (d + 31) & ~31 : (d + 15) & ~15) : 0;
+  void *rc = nullptr;
+  float *nr = nullptr;
+  if (mfma) {
+    TRY(ws_get(c, "r_rc", (size_t)R.n * dk * (bf16 ? 2 : 4), &rc));
+    TRY(ws(c, "r_nr", (size_t)R.n, &nr));
+    launch_center(Ri, d, dk, bf16, rc, nr, s);
+    HIPCHK(hipGetLastError());
+  }
   HIPCHK(hipEventRecord(c->ev[2], s));
 
   double *theta;
@@ -399,9 +431,13 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   HIPCHK(hipEventRecord(c->ev[3], s));
 
   // capacity, tile size, tiles
-  int kcap = exact ? pow2_at_least(std::max(k, 64)) : pow2_at_least(std::max(k + std::max(8, k / 8), 64));
-  kcap = std::min(kcap, KNNJ_MAX_K);
-  const int tq = o.tile_q ? std::min(o.tile_q, join_tile_q(kcap)) : join_tile_q(kcap);
+  int kcap = exact ? pow2_at_least(std::max(k, 64))
+             : bf16 ? pow2_at_least(std::max(2 * k, 64))
+                    : pow2_at_least(std::max(k + std::max(8, k / 8), 64));
+  kcap = std::min(kcap, mfma ? 512 : KNNJ_MAX_K);
+  if (kcap < k) kcap = KNNJ_MAX_K;  // only the VALU kernel holds k > 512
+  const bool use_mfma = mfma && kcap <= 512;
+  const int tq = use_mfma ? 16 : (o.tile_q ? std::min(o.tile_q, join_tile_q(kcap)) : join_tile_q(kcap));
   const int64_t max_tiles = Q.n / tq + m_q + 1;
   Tile *tiles, *ovf_tiles, *ovf2_tiles;
   int32_t *ntiles, *counters;
@@ -447,7 +483,25 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   a.ovf_tiles = ovf_tiles;
   a.ovf_count = counters + 1;
   a.max_err_bits = (u32 *)(counters + 3);
-  HIPCHK(launch_join(a, tq, max_tiles, s));
+  a.rc = rc;
+  a.nr = nr;
+  a.dk = dk;
+  {
+    const double u = 0x1p-24, uin = bf16 ? 0x1p-8 : 0.0;
+    auto gam = [&](double n) { return n * u / (1.0 - n * u); };
+    a.c1 = (float)((2.0 * (2.0 * uin + uin * uin + gam(d) * (1.0 + uin) * (1.0 + uin)) + 4.0 * u) * 1.01);
+    a.c2 = (float)((gam(d + 3) + 8.0 * u) * 1.01);
+  }
+  if (use_mfma) {
+    a.exact = exact ? 1 : 0;
+    if (bf16)
+      HIPCHK(launch_join_mfma_bf16(a, max_tiles, s));
+    else
+      HIPCHK(launch_join_mfma_f32(a, max_tiles, s));
+  } else {
+    a.exact = (mfma ? exact_valu : exact) ? 1 : 0;
+    HIPCHK(launch_join(a, tq, max_tiles, s));
+  }
   HIPCHK(hipEventRecord(c->ev[4], s));
 
   int32_t hc[4];
@@ -459,6 +513,7 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   if (hc[1] > 0) {
     JoinArgs b = a;
     b.kcap = KNNJ_MAX_K;
+    b.exact = exact_valu ? 1 : 0;
     b.tiles = ovf_tiles;
     b.ntiles = counters + 1;
     b.ovf_tiles = ovf2_tiles;

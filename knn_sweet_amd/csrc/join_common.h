@@ -1,0 +1,90 @@
+// join_common.h -- result emission shared by the join kernels.
+#pragma once
+
+#include "knnj_common.h"
+#include "knnj_kernels.h"
+#include "wave_sort.h"
+
+// Writes row `qo` of the answer from a query's sorted candidate list L.
+//
+// Exact mode: the keys are (exact binary32 squared distance, original row);
+// the first k are the answer.
+// Filter mode: the keys are (upper bound of the squared distance, permuted
+// reference position).  Every entry with key distance <= cutoff is re-scored
+// with the oracle's binary64 sequence and the entries are sorted by
+// (binary64 squared distance, original row).  If the list's last entry is
+// itself within the cutoff, a needed candidate may have been evicted: the
+// query is appended to the overflow list and nothing is written.
+template <int KCAP, bool EXACT>
+__device__ __forceinline__ void emit_query(const JoinArgs &a, const u64 *L, int qpos, int cluster,
+                                           float cutoff, const float *qrow) {
+  const int lane = threadIdx.x & 63;
+  const int k = a.k;
+  const int64_t qo = a.Q.perm[qpos];
+  if (EXACT) {
+    for (int j = lane; j < k; j += 64) {
+      const u64 key = L[j];
+      const float s = key_dist(key);
+      const int32_t row = (int32_t)key_row(key);
+      const float dist = a.out_mode == 1 ? s : (float)sqrt((double)s);
+      if (a.out_mode == 2) {
+        a.out_idx[(qo * k + j) * 2] = row;
+        a.out_idx[(qo * k + j) * 2 + 1] = __float_as_int(dist);
+      } else {
+        a.out_idx[qo * k + j] = row;
+        a.out_dist[qo * k + j] = dist;
+      }
+    }
+    return;
+  }
+  const u64 last = L[KCAP - 1];
+  if (last != KEY_MAX && key_dist(last) <= cutoff) {
+    if (lane == 0) {
+      const int slot = atomicAdd(a.ovf_count, 1);
+      Tile t;
+      t.start = qpos;
+      t.count = 1;
+      t.cluster = cluster;
+      t.pad = 0;
+      a.ovf_tiles[slot] = t;
+    }
+    return;
+  }
+  constexpr int E = KCAP / 64;
+  u64 hs[E];
+  u32 rs[E];
+  float err = 0.f;
+  const int d = a.d, dpR = a.R.dpad;
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const u64 key = L[lane + 64 * u];
+    const bool ok = key != KEY_MAX && key_dist(key) <= cutoff;
+    hs[u] = KEY_MAX;
+    rs[u] = 0xFFFFFFFFu;
+    if (ok) {
+      const int p = (int)key_row(key);
+      const double s64 = sq64_pair(qrow, a.R.px + (int64_t)p * dpR, d);
+      hs[u] = (u64)__double_as_longlong(s64);
+      rs[u] = (u32)a.R.perm[p];
+      err = fmaxf(err, (float)fabs((double)key_dist(key) - s64));
+    }
+  }
+  wave_sort_pair<E>(hs, rs);
+  err = wave_max(err);
+  if (lane == 0 && err > 0.f) atomicMax(a.max_err_bits, __float_as_uint(err));
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int j = lane + 64 * u;
+    if (j < k) {
+      const double s64 = __longlong_as_double((long long)hs[u]);
+      const float dist = a.out_mode == 1 ? (float)s64 : (float)sqrt(s64);
+      if (a.out_mode == 2) {
+        a.out_idx[(qo * k + j) * 2] = (int32_t)rs[u];
+        a.out_idx[(qo * k + j) * 2 + 1] = __float_as_int(dist);
+      } else {
+        a.out_idx[qo * k + j] = (int32_t)rs[u];
+        a.out_dist[qo * k + j] = dist;
+      }
+    }
+  }
+}

@@ -28,6 +28,7 @@
 #include "knnj_common.h"
 #include "knnj_kernels.h"
 #include "wave_sort.h"
+#include "join_common.h"
 
 template <int TQ, int KCAP, bool EXACT>
 __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
@@ -185,77 +186,14 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
   __syncthreads();
 
   // ---- results
+  for (int i = 0; i < nq; ++i) {
+    // the threshold of query i (register array, selected without dynamic indexing)
+    u64 ti = thr[0];
 #pragma unroll
-  for (int i = 0; i < TQ; ++i) {
-    if (i >= nq) break;
-    const int64_t qo = a.Q.perm[tile.start + i];
-    const u64 *L = lists + i * KCAP;
-    if (EXACT) {
-      for (int j = lane; j < k; j += 64) {
-        const u64 key = L[j];
-        const float s = key_dist(key);
-        const int32_t row = (int32_t)key_row(key);
-        const float dist = a.out_mode == 1 ? s : (float)sqrt((double)s);
-        if (a.out_mode == 2) {
-          a.out_idx[(qo * k + j) * 2] = row;
-          a.out_idx[(qo * k + j) * 2 + 1] = __float_as_int(dist);
-        } else {
-          a.out_idx[qo * k + j] = row;
-          a.out_dist[qo * k + j] = dist;
-        }
-      }
-    } else {
-      const float Tf = key_dist(thr[i]);
-      const u64 last = L[KCAP - 1];
-      if (last != KEY_MAX && key_dist(last) <= Tf) {
-        if (lane == 0) {
-          const int slot = atomicAdd(a.ovf_count, 1);
-          Tile t;
-          t.start = tile.start + i;
-          t.count = 1;
-          t.cluster = A;
-          t.pad = 0;
-          a.ovf_tiles[slot] = t;
-        }
-        continue;
-      }
-      constexpr int E = KCAP / 64;
-      u64 hs[E];
-      u32 rs[E];
-      float err = 0.f;
-#pragma unroll
-      for (int u = 0; u < E; ++u) {
-        const u64 key = L[lane + 64 * u];
-        const bool ok = key != KEY_MAX && key_dist(key) <= Tf;
-        hs[u] = KEY_MAX;
-        rs[u] = 0xFFFFFFFFu;
-        if (ok) {
-          const int p = (int)key_row(key);
-          const double s64 = sq64_pair(qrow[i], a.R.px + (int64_t)p * dpR, d);
-          hs[u] = (u64)__double_as_longlong(s64);
-          rs[u] = (u32)a.R.perm[p];
-          err = fmaxf(err, (float)fabs((double)key_dist(key) - s64));
-        }
-      }
-      wave_sort_pair<E>(hs, rs);
-      err = wave_max(err);
-      if (lane == 0 && err > 0.f) atomicMax(a.max_err_bits, __float_as_uint(err));
-#pragma unroll
-      for (int u = 0; u < E; ++u) {
-        const int j = lane + 64 * u;
-        if (j < k) {
-          const double s64 = __longlong_as_double((long long)hs[u]);
-          const float dist = a.out_mode == 1 ? (float)s64 : (float)sqrt(s64);
-          if (a.out_mode == 2) {
-            a.out_idx[(qo * k + j) * 2] = (int32_t)rs[u];
-            a.out_idx[(qo * k + j) * 2 + 1] = __float_as_int(dist);
-          } else {
-            a.out_idx[qo * k + j] = (int32_t)rs[u];
-            a.out_dist[qo * k + j] = dist;
-          }
-        }
-      }
-    }
+    for (int j = 1; j < TQ; ++j)
+      if (j == i) ti = thr[j];
+    emit_query<KCAP, EXACT>(a, lists + i * KCAP, tile.start + i, A, key_dist(ti),
+                            a.Q.px + (int64_t)(tile.start + i) * dpQ);
   }
   if (lane == 0 && evals) atomicAdd(a.n_evals, evals);
 }

@@ -41,10 +41,23 @@ struct JoinArgs {
   u64 *n_evals;
   Tile *ovf_tiles;       // overflowed queries, as one-query tiles
   int32_t *ovf_count;
-  u32 *max_err_bits;     // max |s32 - s64| seen in rerank (float bits)
+  u32 *max_err_bits;     // max |key - s64| seen in rerank (float bits)
+  // MFMA paths: reference rows centred on their cluster representative
+  const void *rc;        // [nR][dk] bf16 or fp32
+  const float *nr;       // [nR] squared norm of the centred row (binary32)
+  int dk;                // padded width of rc
+  float c1, c2;          // error-bound coefficients (see k_join_mfma.h)
 };
 
 // Picks the wave tile size for a candidate capacity; returns the dynamic LDS bytes.
 int join_tile_q(int kcap);
 size_t join_lds_bytes(int tq, int kcap);
 hipError_t launch_join(const JoinArgs &a, int tq, int64_t max_tiles, hipStream_t s);
+
+// k_center.hip
+void launch_center(const SideIndex &Ri, int d, int dk, bool bf16, void *rc, float *nr, hipStream_t s);
+
+// k_join_mfma_*.hip
+hipError_t launch_join_mfma_f32(const JoinArgs &a, int64_t max_tiles, hipStream_t s);
+hipError_t launch_join_mfma_bf16(const JoinArgs &a, int64_t max_tiles, hipStream_t s);
+size_t join_mfma_lds_bytes(int kcap);

@@ -87,29 +87,30 @@ __device__ __forceinline__ int count_below(const u64 *a, int n, int p2, u64 x) {
   return c;
 }
 
-// Merges the unsorted candidate buffer buf[0..cnt) (cnt <= KNNJ_BCAP) into
-// the sorted list[0..KCAP), keeping the KCAP smallest keys.  Keys in list and
+// Merges the unsorted candidate buffer buf[0..cnt) (cnt <= BCAP) into the
+// sorted list[0..KCAP), keeping the KCAP smallest keys.  Keys in list and
 // buf are distinct except for KEY_MAX padding.  tmp: KCAP scratch keys.  The
 // caller's workgroup is this single wave.
-template <int KCAP>
+template <int KCAP, int BCAP = KNNJ_BCAP>
 __device__ __forceinline__ void wave_merge(u64 *list, u64 *buf, int cnt, u64 *tmp) {
+  constexpr int EB = BCAP / 64;
   const int lane = threadIdx.x & 63;
-  u64 v[2];
-  v[0] = lane < cnt ? buf[lane] : KEY_MAX;
-  v[1] = lane + 64 < cnt ? buf[lane + 64] : KEY_MAX;
-  wave_sort_u64<2>(v);
-  buf[lane] = v[0];
-  buf[lane + 64] = v[1];
+  u64 v[EB];
+#pragma unroll
+  for (int u = 0; u < EB; ++u) v[u] = lane + 64 * u < cnt ? buf[lane + 64 * u] : KEY_MAX;
+  wave_sort_u64<EB>(v);
+#pragma unroll
+  for (int u = 0; u < EB; ++u) buf[lane + 64 * u] = v[u];
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < KCAP / 64; ++u) {
     const int j = lane + 64 * u;
     const u64 x = list[j];
-    const int pos = j + count_below<true>(buf, KNNJ_BCAP, KNNJ_BCAP, x);
+    const int pos = j + count_below<true>(buf, BCAP, BCAP, x);
     if (pos < KCAP) tmp[pos] = x;
   }
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
+  for (int u = 0; u < EB; ++u) {
     const int j = lane + 64 * u;
     const u64 x = v[u];
     const int pos = j + count_below<false>(list, KCAP, KCAP, x);

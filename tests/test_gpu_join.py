@@ -136,3 +136,39 @@ def test_integer_selfjoin_50k_sampled_rows(ctx):
     rows = np.random.default_rng(0).choice(X.shape[0], 1500, replace=False)
     assert_matches_oracle(X, X, 200, idx, dist, rows=rows)
     assert st["pruning_ratio"] > 0.5
+
+
+@pytest.mark.parametrize("d,k", [(32, 16), (64, 32), (128, 100), (200, 20)])
+def test_mfma_fp32_real(ctx, d, k):
+    mix = datagen.mixture64_model(seed=d + 5, d=d, comps=30)
+    R = mix.sample(datagen.rng(d * 5 + 1), 7000)
+    Q = mix.sample(datagen.rng(d * 5 + 2), 300)
+    idx, dist, st = ctx.join(R, Q, k, kernel=kj.KERNEL_MFMA, precision=kj.PREC_FP32)
+    assert st["kernel_used"] == kj.KERNEL_MFMA and st["precision_used"] == kj.PREC_FP32
+    assert_matches_oracle(R, Q, k, idx, dist)
+
+
+@pytest.mark.parametrize("d,k", [(64, 10), (96, 50), (128, 100), (128, 10)])
+def test_mfma_bf16_filter(ctx, d, k):
+    R, Q = datagen.mixture128(9000, 300, seed=d + k, d=d)
+    idx, dist, st = ctx.join(R, Q, k, precision=kj.PREC_BF16)
+    assert st["kernel_used"] == kj.KERNEL_MFMA and st["precision_used"] == kj.PREC_BF16
+    assert_matches_oracle(R, Q, k, idx, dist)
+
+
+def test_mfma_exact_integer(ctx):
+    g = np.random.default_rng(12)
+    R = g.integers(0, 16, (5000, 32)).astype(np.float32)
+    Q = g.integers(0, 16, (300, 32)).astype(np.float32)
+    idx, dist, st = ctx.join(R, Q, 40, kernel=kj.KERNEL_MFMA)
+    assert st["exact_mode"] == 1 and st["kernel_used"] == kj.KERNEL_MFMA
+    assert_matches_oracle(R, Q, 40, idx, dist)
+
+
+def test_valu_and_mfma_agree(ctx):
+    R, Q = datagen.mixture64(6000, 400, seed=31)
+    a = ctx.join(R, Q, 32, kernel=kj.KERNEL_VALU)
+    b = ctx.join(R, Q, 32, kernel=kj.KERNEL_MFMA, precision=kj.PREC_FP32)
+    c = ctx.join(R, Q, 32, precision=kj.PREC_BF16)
+    assert (a[0] == b[0]).all() and (a[0] == c[0]).all()
+    assert (a[1] == b[1]).all() and (a[1] == c[1]).all()
