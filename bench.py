@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 METRIC = "exact kNN queries/s/GPU (BF16-filter vs plain FP32, recall=1.0); p99 ms"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def load_config(name: str):
@@ -73,6 +74,8 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--brute-force", action="store_true", help="one cluster per side")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma"])
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -87,7 +90,9 @@ def main():
     Qd = Rd if cfg["self_join"] else torch.from_numpy(Q).cuda()
     torch.cuda.synchronize()
     ctx = kj.Context(0)
-    opts = dict(brute_force=args.brute_force)
+    opts = dict(brute_force=args.brute_force,
+                kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel],
+                precision={"auto": kj.PREC_AUTO, "fp32": kj.PREC_FP32, "bf16": kj.PREC_BF16}[args.precision])
     for _ in range(args.warmup):
         ctx.join(Rd, Qd, k, **opts)
     torch.cuda.synchronize()
@@ -112,17 +117,23 @@ def main():
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None}
     else:
-        flops = evals * 3.0 * d
+        # executed flops per (query, candidate) pair: 3d for the direct form
+        # (d subtractions, d FMAs), 2d for the GEMM form on the matrix cores
+        mfma = st["kernel_used"] == kj.KERNEL_MFMA
+        flops = evals * (2.0 if mfma else 3.0) * d
+        peak = BF16_PEAK_TFLOPS if st["precision_used"] == kj.PREC_BF16 else FP32_PEAK_TFLOPS
         achieved = flops / (t_join * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None}
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                "frac": achieved / peak, "traffic": None}
     out = {
         "metric": METRIC, "value": value, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": total * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded Gaussian mixture, knn_sweet_amd.datagen)",
         "config": {"workload": cfg["workload"], "n_r": nr, "n_q": nq, "d": d, "k": k,
-                   "precision": "fp32", "self_join": cfg["self_join"]},
+                   "precision": {1: "fp32", 2: "bf16-filter+fp64-rerank"}[st["precision_used"]],
+                   "kernel": {1: "valu", 2: "mfma"}[st["kernel_used"]],
+                   "self_join": cfg["self_join"]},
         "p50_ms": float(np.percentile(times, 50)), "p99_ms": float(np.percentile(times, 99)),
         "roofline": roof,
         "stages_ms": {"index": st["t_index_ms"], "bounds": st["t_bounds_ms"], "join": t_join,
@@ -130,7 +141,8 @@ def main():
         "pruning_ratio": st["pruning_ratio"], "dist_evals": int(evals),
         "dist_pairs_per_s": evals / (t_join * 1e-3), "m_q": st["m_q"], "m_r": st["m_r"],
         "kcap": st["kcap"], "tile_q": st["tile_q"], "n_overflow": st["n_overflow"],
-        "exact_mode": st["exact_mode"],
+        "exact_mode": st["exact_mode"], "merges": st["n_merges"], "appended": st["n_appended"],
+        "filter_max_abs_err": st["filter_max_abs_err"],
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(R, Q, k)

@@ -83,7 +83,9 @@ typedef struct {
   int32_t brute_force; /* 1: one cluster per side (no pruning) */
   int32_t tile_q;      /* queries per wave tile of the VALU kernel, 0 = auto */
   int32_t kernel;      /* knnj_kernel: distance-tile implementation, 0 = auto */
-  int32_t reserved[5];
+  int32_t refine_iters;/* Lloyd steps moving each representative to its members' mean
+                          (0 = default 1, negative = keep the sampled points) */
+  int32_t reserved[4];
 } knnj_opts;
 
 /* Device-side timings (hipEvents on the context's stream) and counters. */
@@ -107,7 +109,10 @@ typedef struct {
                                candidates (VALU: the FP32 filter error; MFMA: the
                                filter error plus the bound's width) */
   int32_t kernel_used;      /* knnj_kernel actually run */
-  int32_t reserved[7];
+  int32_t reserved0;
+  int64_t n_merges;         /* k-select buffer merges */
+  int64_t n_appended;       /* candidates that passed a query's threshold */
+  int32_t reserved[4];
 } knnj_stats;
 
 /* One independent join of a batch (knnj_join_batch). */

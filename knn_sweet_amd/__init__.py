@@ -46,7 +46,8 @@ class Opts(ctypes.Structure):
                 ("precision", ctypes.c_int32), ("seed", ctypes.c_uint64),
                 ("out_squared", ctypes.c_int32), ("out_aos", ctypes.c_int32),
                 ("brute_force", ctypes.c_int32), ("tile_q", ctypes.c_int32),
-                ("kernel", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("kernel", ctypes.c_int32), ("refine_iters", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class Stats(ctypes.Structure):
@@ -58,10 +59,12 @@ class Stats(ctypes.Structure):
                 ("precision_used", ctypes.c_int32), ("m_q", ctypes.c_int32), ("m_r", ctypes.c_int32),
                 ("tile_q", ctypes.c_int32), ("kcap", ctypes.c_int32), ("n_tiles", ctypes.c_int64),
                 ("filter_max_abs_err", ctypes.c_double), ("kernel_used", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 7)]
+                ("reserved0", ctypes.c_int32), ("n_merges", ctypes.c_int64),
+                ("n_appended", ctypes.c_int64), ("reserved", ctypes.c_int32 * 4)]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_
+                if not name.startswith("reserved")}
 
 
 class Instance(ctypes.Structure):
@@ -127,9 +130,9 @@ def _matrix(x):
 
 
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
-              brute_force=False, tile_q=0, kernel=KERNEL_AUTO) -> Opts:
+              brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0) -> Opts:
     return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
-                tile_q, kernel)
+                tile_q, kernel, refine_iters)
 
 
 class Context:

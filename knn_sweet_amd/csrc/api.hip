@@ -137,7 +137,7 @@ int default_reps(int64_t n) {
 
 // Builds one side's cluster index from device rows X.
 knnj_status build_side(knnj_ctx *c, int side, const float *X, int64_t n, int64_t ld, int d, int dpad, int m,
-                       int trials, uint64_t seed, SideIndex *out) {
+                       int trials, uint64_t seed, int refine, bool integral, SideIndex *out) {
   const std::string p = side == 0 ? "r_" : "q_";
   hipStream_t s = c->stream;
   float *reps, *d2r, *pdist, *px;
@@ -161,17 +161,35 @@ knnj_status build_side(knnj_ctx *c, int side, const float *X, int64_t n, int64_t
   const int64_t npairs = (int64_t)m * (m - 1) / 2;
   const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(256, (npairs + 255) / 256));
   TRY(ws(c, p + "partial", (size_t)nblk * trials, &partial));
+  __bf16 *rb = nullptr;
+  float *rn = nullptr;
+  const int dk = (d + 31) & ~31;
+  if (d >= 16 && dk <= 256) {
+    TRY(ws(c, p + "repsbf16", (size_t)m * dk, &rb));
+    TRY(ws(c, p + "repsnrm", (size_t)m, &rn));
+  }
 
   std::vector<int32_t> &hc = c->host_cand[side];
   hc.resize((size_t)m * trials);
   uint64_t st = seed;
   for (auto &v : hc) v = (int32_t)(splitmix64(st) % (uint64_t)n);
   HIPCHK(hipMemcpyAsync(cand, hc.data(), hc.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  launch_rep_select(X, ld, d, dpad, cand, m, trials, partial, nblk, reps, rep_rows, s);
+  // Lloyd refinement: members of each sampled representative (ascending row
+  // order), their mean becomes the representative
+  for (int it = 0; it < refine; ++it) {
+    launch_assign(X, n, ld, d, dpad, reps, m, cl, rb, rn, s);
+    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * m, s));
+    HIPCHK(hipMemsetAsync(fill, 0, sizeof(int32_t) * m, s));
+    launch_count(cl, n, cnt, s);
+    launch_bucket_rows(cl, n, m, cnt, off, fill, perm, gkeys, s);
+    launch_centroid(X, ld, d, dpad, m, off, perm, integral ? 1 : 0, reps, s);
+  }
+  launch_assign(X, n, ld, d, dpad, reps, m, cl, rb, rn, s);
   HIPCHK(hipMemsetAsync(radbits, 0, sizeof(u64) * m, s));
   HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * m, s));
   HIPCHK(hipMemsetAsync(fill, 0, sizeof(int32_t) * m, s));
-  launch_rep_select(X, ld, d, dpad, cand, m, trials, partial, nblk, reps, rep_rows, s);
-  launch_assign(X, n, ld, d, dpad, reps, m, cl, d2r, radbits, cnt, s);
+  launch_assign_finish(X, n, ld, d, dpad, reps, cl, d2r, radbits, cnt, s);
   launch_bucket_sort_permute(X, n, ld, d, dpad, m, cl, cnt, off, fill, perm, d2r, pdist, gkeys, px, s);
   HIPCHK(hipGetLastError());
   out->n = n;
@@ -405,11 +423,12 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
 
   HIPCHK(hipEventRecord(c->ev[1], s));
   SideIndex Ri{}, Qi{};
-  TRY(build_side(c, 0, dR, R.n, dldR, d, dpad, m_r, trials, seed, &Ri));
+  const int refine = o.refine_iters < 0 ? 0 : (o.refine_iters > 0 ? o.refine_iters : 1);
+  TRY(build_side(c, 0, dR, R.n, dldR, d, dpad, m_r, trials, seed, refine, integral, &Ri));
   if (self)
     Qi = Ri;
   else
-    TRY(build_side(c, 1, dQ, Q.n, dldQ, d, dpad, m_q, trials, seed, &Qi));
+    TRY(build_side(c, 1, dQ, Q.n, dldQ, d, dpad, m_q, trials, seed, refine, integral, &Qi));
   const int dk = mfma ? (bf16 ? (d + 31) & ~31 : (d + 15) & ~15) : 0;
   void *rc = nullptr;
   float *nr = nullptr;
@@ -446,10 +465,10 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   TRY(ws(c, "ovf_tiles", (size_t)Q.n, &ovf_tiles));
   TRY(ws(c, "ovf2_tiles", (size_t)Q.n, &ovf2_tiles));
   TRY(ws(c, "counters", 8, &counters));  // [0] ntiles, [1] ovf, [2] ovf2, [3] max_err bits
-  TRY(ws(c, "evals", 2, &n_evals));
+  TRY(ws(c, "evals", 4, &n_evals));
   ntiles = counters;
   HIPCHK(hipMemsetAsync(counters, 0, 8 * sizeof(int32_t), s));
-  HIPCHK(hipMemsetAsync(n_evals, 0, 2 * sizeof(u64), s));
+  HIPCHK(hipMemsetAsync(n_evals, 0, 4 * sizeof(u64), s));
   launch_make_tiles(Qi.off, m_q, tq, tiles, ntiles, s);
 
   // outputs
@@ -532,9 +551,9 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
     HIPCHK(hipMemcpyAsync(out_idx, oidx, nout * sizeof(int32_t) * (aos ? 2 : 1), hipMemcpyDeviceToHost, s));
     if (!aos) HIPCHK(hipMemcpyAsync(out_dist, odist, nout * sizeof(float), hipMemcpyDeviceToHost, s));
   }
-  u64 hev = 0;
+  u64 hcnt[4] = {0, 0, 0, 0};
   u32 herr = 0;
-  HIPCHK(hipMemcpyAsync(&hev, n_evals, sizeof(hev), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(hcnt, n_evals, sizeof(hcnt), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(&herr, counters + 3, sizeof(herr), hipMemcpyDeviceToHost, s));
   HIPCHK(hipEventRecord(c->ev[7], s));
   HIPCHK(hipStreamSynchronize(s));
@@ -544,7 +563,10 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   st.t_bounds_ms = elapsed(c->ev[2], c->ev[3]);
   st.t_join_ms = elapsed(c->ev[3], c->ev[4]);
   st.t_fallback_ms = hc[1] > 0 ? elapsed(c->ev[5], c->ev[6]) : 0.0;
+  const u64 hev = hcnt[0];
   st.n_dist_evals = (int64_t)hev;
+  st.n_merges = (int64_t)hcnt[1];
+  st.n_appended = (int64_t)hcnt[2];
   const double tot = (double)R.n * (double)Q.n;
   st.pruning_ratio = tot > 0 ? 1.0 - (double)hev / tot : 0.0;
   st.tile_q = tq;

@@ -128,17 +128,15 @@ void launch_rep_select(const float *X, int64_t ld, int d, int dpad, const int32_
 }
 
 // ---------------------------------------------------------------------------
-// Assignment: nearest representative (FP32, lowest index on ties, as
-// findQCluster's strict compare, knnjoin.cu:64-80), the binary64 distance to
-// it, the binary64 cluster radius (atomic max on the bit pattern of a
-// non-negative double) and the cluster histogram.  Representatives are
-// staged in LDS, 32 at a time, 32 dimensions at a time.
+// Assignment: nearest representative (FP32 direct form, lowest index on
+// ties, as findQCluster's strict compare, knnjoin.cu:64-80) for small d.
+// Representatives are staged in LDS, 32 at a time, up to 32 dimensions at a
+// time.
 #define AS_RB 32
 template <int AS_DC>
 __global__ __launch_bounds__(256) void k_assign(const float *__restrict__ X, int64_t n, int64_t ld, int d,
-                                                int dpad, const float *__restrict__ reps, int m,
-                                                int32_t *cl, float *d2r, u64 *radbits, int32_t *cnt) {
-  __shared__ float sr[AS_RB][AS_DC + 1];
+                                                int dpad, const float *__restrict__ reps, int m, int32_t *cl) {
+  __shared__ float sr[AS_RB][AS_DC];
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const bool live = i < n;
   const float *x = X + (live ? i : 0) * ld;
@@ -178,24 +176,191 @@ __global__ __launch_bounds__(256) void k_assign(const float *__restrict__ X, int
         bi = r0 + r;
       }
   }
-  if (!live) return;
-  const double dd = sqrt(sq64_pair(x, reps + (int64_t)bi * dpad, d));
-  cl[i] = bi;
+  if (live) cl[i] = bi;
+}
+
+// Binary64 distance of every row to its assigned representative, the cluster
+// radius (atomic max on the bit pattern of a non-negative double) and the
+// cluster histogram.
+__global__ void k_assign_finish(const float *__restrict__ X, int64_t n, int64_t ld, int d, int dpad,
+                                const float *__restrict__ reps, const int32_t *__restrict__ cl, float *d2r,
+                                u64 *radbits, int32_t *cnt) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = cl[i];
+  const double dd = sqrt(sq64_pair(X + i * ld, reps + (int64_t)b * dpad, d));
   d2r[i] = (float)dd;
-  atomicMax(&radbits[bi], (u64)__double_as_longlong(dd));
-  atomicAdd(&cnt[bi], 1);
+  atomicMax(&radbits[b], (u64)__double_as_longlong(dd));
+  atomicAdd(&cnt[b], 1);
+}
+
+__global__ void k_count(const int32_t *__restrict__ cl, int64_t n, int32_t *cnt) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[cl[i]], 1);
+}
+
+// Nearest representative on the matrix cores: argmin_c ||c||^2 - 2 x.c with
+// x and c rounded to bf16 and FP32 accumulation (v_mfma_f32_16x16x32_bf16).
+// The argmin only has to be good, not exact: every later bound uses the
+// binary64 distance to whichever representative was chosen.  One wave takes
+// 32 rows (two 16-row A blocks) against all representatives, 16 at a time;
+// ties go to the lower representative index.
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 as_bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float as_f32x4;
+
+template <int DK>
+__global__ __launch_bounds__(256) void k_assign_mfma(const float *__restrict__ X, int64_t n, int64_t ld, int d,
+                                                     const __bf16 *__restrict__ rb, const float *__restrict__ rn,
+                                                     int m, int dk, int32_t *cl) {
+  constexpr int NS = DK / 32;
+  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
+  const int64_t base = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+  if (base >= n) return;
+  const int nst = dk / 32;
+  as_bf16x8 a0[NS], a1[NS];
+  const int64_t p0 = base + c16, p1 = base + 16 + c16;
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int dim = 32 * s + 8 * g + j;
+      const bool okd = s < nst && dim < d;
+      a0[s][j] = (__bf16)((okd && p0 < n) ? X[p0 * ld + dim] : 0.f);
+      a1[s][j] = (__bf16)((okd && p1 < n) ? X[p1 * ld + dim] : 0.f);
+    }
+  float b0[4], b1[4];
+  int i0[4], i1[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    b0[i] = b1[i] = INFINITY;
+    i0[i] = i1[i] = 0;
+  }
+  for (int r0 = 0; r0 < m; r0 += 16) {
+    const int rep = r0 + c16;
+    const bool ok = rep < m;
+    const __bf16 *rr = rb + (int64_t)(ok ? rep : 0) * dk + 8 * g;
+    as_f32x4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (s < nst) {
+        const as_bf16x8 bv = *(const as_bf16x8 *)(rr + 32 * s);
+        c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[s], bv, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[s], bv, c1, 0, 0, 0);
+      }
+    const float nr = ok ? rn[rep] : INFINITY;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float s0 = nr - 2.f * c0[i], s1 = nr - 2.f * c1[i];
+      if (s0 < b0[i]) {
+        b0[i] = s0;
+        i0[i] = rep;
+      }
+      if (s1 < b1[i]) {
+        b1[i] = s1;
+        i1[i] = rep;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int msk = 1; msk < 16; msk <<= 1) {
+      float ov = __shfl_xor(b0[i], msk);
+      int oi = __shfl_xor(i0[i], msk);
+      if (ov < b0[i] || (ov == b0[i] && oi < i0[i])) {
+        b0[i] = ov;
+        i0[i] = oi;
+      }
+      ov = __shfl_xor(b1[i], msk);
+      oi = __shfl_xor(i1[i], msk);
+      if (ov < b1[i] || (ov == b1[i] && oi < i1[i])) {
+        b1[i] = ov;
+        i1[i] = oi;
+      }
+    }
+    if (c16 == 0) {
+      const int64_t r0w = base + 4 * g + i, r1w = base + 16 + 4 * g + i;
+      if (r0w < n) cl[r0w] = i0[i];
+      if (r1w < n) cl[r1w] = i1[i];
+    }
+  }
+}
+
+// bf16 copy of the representatives (row width dk) and their FP32 squared norms.
+__global__ void k_reps_bf16(const float *__restrict__ reps, int m, int d, int dpad, int dk, __bf16 *rb,
+                            float *rn) {
+  const int r = blockIdx.x;
+  if (r >= m) return;
+  const int lane = threadIdx.x;
+  float part = 0.f;
+  for (int t = lane; t < dk; t += 64) {
+    const float v = t < d ? reps[(int64_t)r * dpad + t] : 0.f;
+    rb[(int64_t)r * dk + t] = (__bf16)v;
+    part = __builtin_fmaf(v, v, part);
+  }
+  part = wave_sum(part);
+  if (lane == 0) rn[r] = part;
+}
+
+// Mean of each cluster's members (binary64 sums over members in ascending
+// row order, then rounded to FP32; to the nearest integer for integer
+// inputs, which keeps the GEMM form exact there).  Empty clusters keep their
+// representative.  This is one Lloyd step on the sampled representatives:
+// in high dimension a member of a Gaussian cluster lies about sqrt(2) times
+// farther from another member than from the centre, so centroids shrink the
+// cluster radii that every triangle-inequality bound adds.
+__global__ __launch_bounds__(256) void k_centroid(const float *__restrict__ X, int64_t ld, int d, int dpad,
+                                                  const int32_t *__restrict__ off,
+                                                  const int32_t *__restrict__ perm, int integral, float *reps) {
+  const int b = blockIdx.x;
+  const int s0 = off[b], s1 = off[b + 1];
+  if (s1 <= s0) return;
+  for (int t = threadIdx.x; t < d; t += blockDim.x) {
+    double acc = 0.0;
+    for (int p = s0; p < s1; ++p) acc += (double)X[(int64_t)perm[p] * ld + t];
+    float v = (float)(acc / (double)(s1 - s0));
+    if (integral) v = rintf(v);
+    reps[(int64_t)b * dpad + t] = v;
+  }
 }
 
 void launch_assign(const float *X, int64_t n, int64_t ld, int d, int dpad, const float *reps, int m,
-                   int32_t *cl, float *d2r, u64 *radbits, int32_t *cnt, hipStream_t s) {
+                   int32_t *cl, __bf16 *rb, float *rn, hipStream_t s) {
   if (n <= 0) return;
+  const int dk = (d + 31) & ~31;
+  if (d >= 16 && dk <= 256 && rb && rn) {
+    hipLaunchKernelGGL(k_reps_bf16, dim3(m), dim3(64), 0, s, reps, m, d, dpad, dk, rb, rn);
+    const dim3 g((unsigned)((n + 127) / 128));
+    if (dk <= 128)
+      hipLaunchKernelGGL(k_assign_mfma<128>, g, dim3(256), 0, s, X, n, ld, d, rb, rn, m, dk, cl);
+    else
+      hipLaunchKernelGGL(k_assign_mfma<256>, g, dim3(256), 0, s, X, n, ld, d, rb, rn, m, dk, cl);
+    return;
+  }
   const dim3 g((unsigned)((n + 255) / 256));
   if (dpad <= 4)
-    hipLaunchKernelGGL(k_assign<4>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, m, cl, d2r, radbits, cnt);
+    hipLaunchKernelGGL(k_assign<4>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, m, cl);
   else if (dpad <= 16)
-    hipLaunchKernelGGL(k_assign<16>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, m, cl, d2r, radbits, cnt);
+    hipLaunchKernelGGL(k_assign<16>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, m, cl);
   else
-    hipLaunchKernelGGL(k_assign<32>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, m, cl, d2r, radbits, cnt);
+    hipLaunchKernelGGL(k_assign<32>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, m, cl);
+}
+
+void launch_assign_finish(const float *X, int64_t n, int64_t ld, int d, int dpad, const float *reps,
+                          const int32_t *cl, float *d2r, u64 *radbits, int32_t *cnt, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_assign_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X, n, ld, d, dpad,
+                     reps, cl, d2r, radbits, cnt);
+}
+
+void launch_count(const int32_t *cl, int64_t n, int32_t *cnt, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cl, n, cnt);
+}
+
+void launch_centroid(const float *X, int64_t ld, int d, int dpad, int m, const int32_t *off, const int32_t *perm,
+                     int integral, float *reps, hipStream_t s) {
+  hipLaunchKernelGGL(k_centroid, dim3(m), dim3(256), 0, s, X, ld, d, dpad, off, perm, integral, reps);
 }
 
 // ---------------------------------------------------------------------------
@@ -235,7 +400,7 @@ __global__ void k_scatter(const int32_t *__restrict__ cl, int64_t n, const int32
 }
 
 // Sorts every cluster segment ascending by (distance to rep, original row)
-// and writes pdist.  Segments up to SEG_LDS keys sort in LDS; longer ones
+// and writes pdist; with d2r == NULL, by original row only.  Segments up to SEG_LDS keys sort in LDS; longer ones
 // sort in place in `gkeys` (global, same indexing as perm).
 #define SEG_LDS 8192
 __device__ __forceinline__ void cex(u64 *a, int i, int j) {
@@ -260,7 +425,7 @@ __global__ __launch_bounds__(1024) void k_segsort(const int32_t *__restrict__ of
   for (int j = threadIdx.x; j < (in_lds ? p2 : n); j += blockDim.x) {
     if (j < n) {
       const int row = perm[s0 + j];
-      a[j] = ((u64)__float_as_uint(d2r[row]) << 32) | (u32)row;
+      a[j] = ((u64)(d2r ? __float_as_uint(d2r[row]) : 0u) << 32) | (u32)row;
     } else {
       a[j] = KEY_MAX;
     }
@@ -279,7 +444,7 @@ __global__ __launch_bounds__(1024) void k_segsort(const int32_t *__restrict__ of
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
     const u64 key = a[j];
     perm[s0 + j] = (int32_t)(u32)key;
-    pdist[s0 + j] = __uint_as_float((u32)(key >> 32));
+    if (pdist) pdist[s0 + j] = __uint_as_float((u32)(key >> 32));
   }
 }
 
@@ -345,4 +510,14 @@ __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__
 
 void launch_make_tiles(const int32_t *off, int m, int tq, Tile *tiles, int32_t *ntiles, hipStream_t s) {
   hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, s, off, m, tq, tiles, ntiles);
+}
+
+// Members of every cluster in ascending row order (counts already in cnt).
+void launch_bucket_rows(const int32_t *cl, int64_t n, int m, const int32_t *cnt, int32_t *off, int32_t *fill,
+                        int32_t *perm, u64 *gkeys, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, cnt, m, off);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cl, n, off, fill, perm);
+  hipLaunchKernelGGL(k_segsort, dim3(m), dim3(1024), 0, s, off, perm, (const float *)nullptr, (float *)nullptr,
+                     gkeys);
 }

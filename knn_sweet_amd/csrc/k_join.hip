@@ -61,11 +61,12 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
     cnt[i] = 0;
     qrow[i] = a.Q.px + (int64_t)(tile.start + (i < nq ? i : 0)) * dpQ;
   }
-  u64 evals = 0;
+  u64 evals = 0, nmerge = 0, npass = 0;
 
   auto merge = [&](int i) {
     wave_merge<KCAP>(lists + i * KCAP, bufs + i * KNNJ_BCAP, cnt[i], tmp);
     cnt[i] = 0;
+    ++nmerge;
     const u64 kth = lists[i * KCAP + k - 1];
     if (kth != KEY_MAX) {
       u64 t;
@@ -163,6 +164,7 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
             if (pm) {
               if (pass) bufs[i * KNNJ_BCAP + cnt[i] + (int)lanes_below(pm)] = key;
               cnt[i] += __popcll(pm);
+              npass += __popcll(pm);
             }
             evals += nvalid;
           }
@@ -195,7 +197,11 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
     emit_query<KCAP, EXACT>(a, lists + i * KCAP, tile.start + i, A, key_dist(ti),
                             a.Q.px + (int64_t)(tile.start + i) * dpQ);
   }
-  if (lane == 0 && evals) atomicAdd(a.n_evals, evals);
+  if (lane == 0) {
+    atomicAdd(a.n_evals, evals);
+    atomicAdd(a.n_evals + 1, nmerge);
+    atomicAdd(a.n_evals + 2, npass);
+  }
 }
 
 int join_tile_q(int kcap) {

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
   }
   const bool alive = c16 < nq;  // this lane's A-operand row
   const float *qa = a.Q.px + (int64_t)(tile.start + (alive ? c16 : 0)) * dpQ;
-  u64 evals = 0;
+  u64 evals = 0, nmerge = 0, npass = 0;
 
   // merge of query r (wave-uniform r)
   auto merge = [&](int r) {
@@ -94,6 +94,7 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
       if (j == ii) mine = cnt[j];
     const int cr = __shfl(mine, 16 * gi);
     wave_merge<KCAP, MF_BC>(lists + r * KCAP, bufs + r * MF_BC, cr, tmp);
+    ++nmerge;
     const u64 kth = lists[r * KCAP + k - 1];
     u64 t = KEY_MAX;
     if (kth != KEY_MAX) t = EXACT ? kth - 1 : (((u64)__float_as_uint(key_dist(kth)) << 32) | 0xFFFFFFFFull);
@@ -230,6 +231,7 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
             const u32 gm = (u32)(mk >> (16 * g)) & 0xFFFFu;
             if (pass[i]) bufs[(4 * g + i) * MF_BC + cnt[i] + __popc(gm & below)] = key[i];
             cnt[i] += __popc(gm);
+            npass += __popcll(mk);
           }
           const bool hot = cnt[0] >= MF_WM || cnt[1] >= MF_WM || cnt[2] >= MF_WM || cnt[3] >= MF_WM;
           if (wave_ballot(hot) != 0) {
@@ -274,7 +276,11 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
     emit_query<KCAP, EXACT>(a, lists + r * KCAP, tile.start + r, A, cutoff,
                             a.Q.px + (int64_t)(tile.start + r) * dpQ);
   }
-  if (lane == 0 && evals) atomicAdd(a.n_evals, evals);
+  if (lane == 0) {
+    atomicAdd(a.n_evals, evals);
+    atomicAdd(a.n_evals + 1, nmerge);
+    atomicAdd(a.n_evals + 2, npass);
+  }
 }
 
 inline size_t mfma_lds(int kcap) {

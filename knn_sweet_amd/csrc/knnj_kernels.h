@@ -9,8 +9,16 @@
 void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags, hipStream_t s);
 void launch_rep_select(const float *X, int64_t ld, int d, int dpad, const int32_t *cand, int m, int trials,
                        double *partial, int nblk, float *reps, int32_t *rep_rows, hipStream_t s);
+// nearest representative (MFMA bf16 for 16 <= d <= 256 when rb/rn are given, VALU otherwise)
 void launch_assign(const float *X, int64_t n, int64_t ld, int d, int dpad, const float *reps, int m,
-                   int32_t *cl, float *d2r, u64 *radbits, int32_t *cnt, hipStream_t s);
+                   int32_t *cl, __bf16 *rb, float *rn, hipStream_t s);
+void launch_assign_finish(const float *X, int64_t n, int64_t ld, int d, int dpad, const float *reps,
+                          const int32_t *cl, float *d2r, u64 *radbits, int32_t *cnt, hipStream_t s);
+void launch_count(const int32_t *cl, int64_t n, int32_t *cnt, hipStream_t s);
+void launch_centroid(const float *X, int64_t ld, int d, int dpad, int m, const int32_t *off, const int32_t *perm,
+                     int integral, float *reps, hipStream_t s);
+void launch_bucket_rows(const int32_t *cl, int64_t n, int m, const int32_t *cnt, int32_t *off, int32_t *fill,
+                        int32_t *perm, u64 *gkeys, hipStream_t s);
 void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, int dpad, int m,
                                 const int32_t *cl, const int32_t *cnt, int32_t *off, int32_t *fill,
                                 int32_t *perm, const float *d2r, float *pdist, u64 *gkeys, float *px,
@@ -38,7 +46,7 @@ struct JoinArgs {
   int32_t *out_idx;      // [nQ][k] or IndexDist [nQ][k] when out_mode == 2
   float *out_dist;
   int out_mode;          // 0 Euclidean, 1 squared, 2 AoS IndexDist (Euclidean)
-  u64 *n_evals;
+  u64 *n_evals;          // [0] distance evaluations, [1] merges, [2] candidates appended
   Tile *ovf_tiles;       // overflowed queries, as one-query tiles
   int32_t *ovf_count;
   u32 *max_err_bits;     // max |key - s64| seen in rerank (float bits)

@@ -172,3 +172,13 @@ def test_valu_and_mfma_agree(ctx):
     c = ctx.join(R, Q, 32, precision=kj.PREC_BF16)
     assert (a[0] == b[0]).all() and (a[0] == c[0]).all()
     assert (a[1] == b[1]).all() and (a[1] == c[1]).all()
+
+
+@pytest.mark.parametrize("refine", [-1, 2])
+def test_refinement_options(ctx, refine):
+    R, Q = datagen.mixture128(6000, 200, seed=5, d=96)
+    idx, dist, _ = ctx.join(R, Q, 20, refine_iters=refine)
+    assert_matches_oracle(R, Q, 20, idx, dist)
+    X = datagen.skin_like(3000, seed=4)
+    idx, dist, _ = ctx.join(X, X[:400], 50, refine_iters=refine)
+    assert_matches_oracle(X, X[:400], 50, idx, dist)
