@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--brute-force", action="store_true", help="one cluster per side")
     ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma"])
-    ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16"])
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16", "bf16x1"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -92,7 +92,8 @@ def main():
     ctx = kj.Context(0)
     opts = dict(brute_force=args.brute_force,
                 kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel],
-                precision={"auto": kj.PREC_AUTO, "fp32": kj.PREC_FP32, "bf16": kj.PREC_BF16}[args.precision])
+                precision={"auto": kj.PREC_AUTO, "fp32": kj.PREC_FP32, "bf16": kj.PREC_BF16,
+                           "bf16x1": kj.PREC_BF16X1}[args.precision])
     for _ in range(args.warmup):
         ctx.join(Rd, Qd, k, **opts)
     torch.cuda.synchronize()
@@ -121,7 +122,7 @@ def main():
         # (d subtractions, d FMAs), 2d for the GEMM form on the matrix cores
         mfma = st["kernel_used"] == kj.KERNEL_MFMA
         flops = evals * (2.0 if mfma else 3.0) * d
-        peak = BF16_PEAK_TFLOPS if st["precision_used"] == kj.PREC_BF16 else FP32_PEAK_TFLOPS
+        peak = BF16_PEAK_TFLOPS if st["precision_used"] in (kj.PREC_BF16, kj.PREC_BF16X1) else FP32_PEAK_TFLOPS
         achieved = flops / (t_join * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None}
@@ -131,7 +132,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded Gaussian mixture, knn_sweet_amd.datagen)",
         "config": {"workload": cfg["workload"], "n_r": nr, "n_q": nq, "d": d, "k": k,
-                   "precision": {1: "fp32", 2: "bf16-filter+fp64-rerank"}[st["precision_used"]],
+                   "precision": {1: "fp32", 2: "bf16x3-filter+fp64-rerank",
+                                 4: "bf16x1-filter+fp64-rerank"}[st["precision_used"]],
                    "kernel": {1: "valu", 2: "mfma"}[st["kernel_used"]],
                    "self_join": cfg["self_join"]},
         "p50_ms": float(np.percentile(times, 50)), "p99_ms": float(np.percentile(times, 99)),

@@ -49,8 +49,11 @@ typedef enum {
 typedef enum {
   KNNJ_PREC_AUTO = 0, /* BF16 filter for d >= 64 real-valued data, FP32 otherwise */
   KNNJ_PREC_FP32 = 1, /* plain FP32 distance tiles */
-  KNNJ_PREC_BF16 = 2, /* BF16 MFMA filter + exact rerank (implies the MFMA kernel) */
-  KNNJ_PREC_FP8 = 3   /* reserved: FP8 filter (not built) */
+  KNNJ_PREC_BF16 = 2, /* BF16 MFMA filter, compensated: operands split into bf16 hi + lo,
+                         three products; + exact rerank (implies the MFMA kernel) */
+  KNNJ_PREC_FP8 = 3,  /* reserved: FP8 filter (not built) */
+  KNNJ_PREC_BF16X1 = 4 /* BF16 MFMA filter with a single product per term (wider error
+                          band, larger candidate lists) + exact rerank */
 } knnj_precision;
 
 typedef enum {

@@ -25,7 +25,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libknnj.so")
 
 KNNJ_OK, KNNJ_EINVAL, KNNJ_ENOMEM, KNNJ_EDEVICE, KNNJ_EUNSUPPORTED, KNNJ_EOVERFLOW = range(6)
-PREC_AUTO, PREC_FP32, PREC_BF16, PREC_FP8 = range(4)
+PREC_AUTO, PREC_FP32, PREC_BF16, PREC_FP8, PREC_BF16X1 = range(5)
 KERNEL_AUTO, KERNEL_VALU, KERNEL_MFMA = range(3)
 _STATUS_NAMES = {0: "OK", 1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "EUNSUPPORTED", 5: "EOVERFLOW"}
 

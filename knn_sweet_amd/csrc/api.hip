@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -158,8 +159,7 @@ knnj_status build_side(knnj_ctx *c, int side, const float *X, int64_t n, int64_t
   TRY(ws(c, p + "px", (size_t)nn * dpad, &px));
   TRY(ws(c, p + "cand", (size_t)m * trials, &cand));
   TRY(ws(c, p + "reprows", (size_t)m, &rep_rows));
-  const int64_t npairs = (int64_t)m * (m - 1) / 2;
-  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(256, (npairs + 255) / 256));
+  const int nblk = rep_score_blocks(m);
   TRY(ws(c, p + "partial", (size_t)nblk * trials, &partial));
   __bf16 *rb = nullptr;
   float *rn = nullptr;
@@ -382,7 +382,11 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   const bool exact_valu = integral && (double)d * span * span < 16777216.0;
   const bool exact_mfma = integral && 4.0 * (double)d * span * span < 16777216.0;
   int kind = o.kernel;
-  if (o.precision == KNNJ_PREC_BF16) {
+  if (o.precision < 0 || o.precision > KNNJ_PREC_BF16X1) {
+    set_err("unknown precision %d", o.precision);
+    return KNNJ_EINVAL;
+  }
+  if (o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1) {
     if (kind == KNNJ_KERNEL_VALU) {
       set_err("the BF16 filter runs on the MFMA kernel only");
       return KNNJ_EINVAL;
@@ -399,10 +403,12 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
     return KNNJ_EUNSUPPORTED;
   }
   const bool mfma = kind == KNNJ_KERNEL_MFMA;
-  const bool bf16 = mfma && (o.precision == KNNJ_PREC_BF16 || (o.precision == KNNJ_PREC_AUTO && d >= 64 && !exact_mfma));
+  const bool bf16 = mfma && (o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1 ||
+                             (o.precision == KNNJ_PREC_AUTO && d >= 64 && !exact_mfma));
+  const bool bf16x1 = bf16 && o.precision == KNNJ_PREC_BF16X1;
   const bool exact = mfma ? (!bf16 && exact_mfma) : exact_valu;
   st.exact_mode = exact ? 1 : 0;
-  st.precision_used = bf16 ? KNNJ_PREC_BF16 : KNNJ_PREC_FP32;
+  st.precision_used = bf16 ? (bf16x1 ? KNNJ_PREC_BF16X1 : KNNJ_PREC_BF16) : KNNJ_PREC_FP32;
   st.kernel_used = kind;
   if (Q.n == 0) {
     if (stats) *stats = st;
@@ -416,7 +422,7 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   m_r = std::min(m_r, KNNJ_MAX_REPS);
   m_q = std::min(m_q, KNNJ_MAX_REPS);
   if (self) m_q = m_r;
-  const int trials = o.trials > 0 ? o.trials : 10;
+  const int trials = o.trials > 0 ? std::min(o.trials, 64) : 10;
   const uint64_t seed = o.seed ? o.seed : 2015;
   st.m_q = m_q;
   st.m_r = m_r;
@@ -429,13 +435,15 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
     Qi = Ri;
   else
     TRY(build_side(c, 1, dQ, Q.n, dldQ, d, dpad, m_q, trials, seed, refine, integral, &Qi));
-  const int dk = mfma ? (bf16 ? (d + 31) & ~31 : (d + 15) & ~15) : 0;
+  const int dk = mfma ? (d + 31) & ~31 : 0;
   void *rc = nullptr;
-  float *nr = nullptr;
+  float *nr = nullptr, *reps32 = nullptr, *rn32 = nullptr;
   if (mfma) {
-    TRY(ws_get(c, "r_rc", (size_t)R.n * dk * (bf16 ? 2 : 4), &rc));
+    TRY(ws_get(c, "r_rc", (size_t)R.n * dk * 4, &rc));
     TRY(ws(c, "r_nr", (size_t)R.n, &nr));
-    launch_center(Ri, d, dk, bf16, rc, nr, s);
+    TRY(ws(c, "r_reps32", (size_t)m_r * dk, &reps32));
+    TRY(ws(c, "r_rn32", (size_t)m_r, &rn32));
+    launch_center(Ri, d, dk, bf16 ? (bf16x1 ? 1 : 2) : 0, rc, nr, reps32, rn32, s);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(c->ev[2], s));
@@ -451,7 +459,7 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
 
   // capacity, tile size, tiles
   int kcap = exact ? pow2_at_least(std::max(k, 64))
-             : bf16 ? pow2_at_least(std::max(2 * k, 64))
+             : bf16x1 ? pow2_at_least(std::max(2 * k, 64))
                     : pow2_at_least(std::max(k + std::max(8, k / 8), 64));
   kcap = std::min(kcap, mfma ? 512 : KNNJ_MAX_K);
   if (kcap < k) kcap = KNNJ_MAX_K;  // only the VALU kernel holds k > 512
@@ -464,12 +472,12 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   TRY(ws(c, "tiles", (size_t)max_tiles, &tiles));
   TRY(ws(c, "ovf_tiles", (size_t)Q.n, &ovf_tiles));
   TRY(ws(c, "ovf2_tiles", (size_t)Q.n, &ovf2_tiles));
-  TRY(ws(c, "counters", 8, &counters));  // [0] ntiles, [1] ovf, [2] ovf2, [3] max_err bits
+  TRY(ws(c, "counters", 8, &counters));  // [0] ntiles, [1] ovf, [2] ovf2, [3] max_err bits, [4] ovf3
   TRY(ws(c, "evals", 4, &n_evals));
   ntiles = counters;
   HIPCHK(hipMemsetAsync(counters, 0, 8 * sizeof(int32_t), s));
   HIPCHK(hipMemsetAsync(n_evals, 0, 4 * sizeof(u64), s));
-  launch_make_tiles(Qi.off, m_q, tq, tiles, ntiles, s);
+  launch_make_tiles(Qi.off, m_q, pcnt, tq, tiles, ntiles, s);
 
   // outputs
   int32_t *oidx = out_idx;
@@ -504,17 +512,41 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   a.max_err_bits = (u32 *)(counters + 3);
   a.rc = rc;
   a.nr = nr;
+  // per-entry error codes in the MFMA filter keys: rows must fit 26 bits;
+  // codes span 63 binades below the largest possible squared distance
+  a.ecode = (use_mfma && !exact && R.n < (int64_t)(1 << 26)) ? 1 : 0;
+  {
+    const double smax = std::max(1e-30, (double)d * span * span);
+    a.ebase = std::ilogb(smax) + 1 - 62;
+  }
+  a.reps32 = reps32;
+  a.rn32 = rn32;
+  const bool tile_prof = std::getenv("KNNJ_TILE_PROFILE") != nullptr;
+  if (tile_prof) {
+    u64 *tp;
+    TRY(ws(c, "tile_prof", (size_t)max_tiles * 4, &tp));
+    HIPCHK(hipMemsetAsync(tp, 0, sizeof(u64) * 4 * max_tiles, s));
+    a.tile_prof = tp;
+    float *dg;
+    TRY(ws(c, "ovf_diag", 256, &dg));
+    HIPCHK(hipMemsetAsync(dg, 0, sizeof(float) * 256, s));
+    a.ovf_diag = dg;
+  }
   a.dk = dk;
   {
-    const double u = 0x1p-24, uin = bf16 ? 0x1p-8 : 0.0;
+    const double u = 0x1p-24, ub = 0x1p-8;
     auto gam = [&](double n) { return n * u / (1.0 - n * u); };
-    a.c1 = (float)((2.0 * (2.0 * uin + uin * uin + gam(d) * (1.0 + uin) * (1.0 + uin)) + 4.0 * u) * 1.01);
+    // |computed dot - (q-c).(r-c)| <= cdot ||q-c|| ||r-c||
+    double cdot = gam(d);                                                     // fp32 operands
+    if (bf16x1) cdot = 2.0 * ub + ub * ub + gam(d) * (1.0 + ub) * (1.0 + ub);  // one bf16 product
+    else if (bf16) cdot = 3.1 * ub * ub + 1.02 * gam(3.0 * d);              // hi/lo, three products
+    a.c1 = (float)((2.0 * cdot + 4.0 * u) * 1.01);
     a.c2 = (float)((gam(d + 3) + 8.0 * u) * 1.01);
   }
   if (use_mfma) {
     a.exact = exact ? 1 : 0;
     if (bf16)
-      HIPCHK(launch_join_mfma_bf16(a, max_tiles, s));
+      HIPCHK(launch_join_mfma_bf16(a, bf16x1 ? 1 : 3, max_tiles, s));
     else
       HIPCHK(launch_join_mfma_f32(a, max_tiles, s));
   } else {
@@ -528,22 +560,85 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   HIPCHK(hipStreamSynchronize(s));
   st.n_tiles = hc[0];
   st.n_overflow = hc[1];
+  if (tile_prof && hc[0] > 0) {
+    // diagnostic: distribution of tile durations (wall clock ticks) and cluster counts
+    std::vector<u64> tp((size_t)hc[0] * 4);
+    HIPCHK(hipMemcpy(tp.data(), a.tile_prof, tp.size() * sizeof(u64), hipMemcpyDeviceToHost));
+    std::vector<double> dur(hc[0]);
+    double listed = 0, visited = 0;
+    u64 t0 = ~0ull, t1 = 0;
+    for (int i = 0; i < hc[0]; ++i) {
+      dur[i] = (double)(tp[4 * i + 1] - tp[4 * i]);
+      listed += (double)tp[4 * i + 2];
+      visited += (double)tp[4 * i + 3];
+      t0 = std::min(t0, tp[4 * i]);
+      t1 = std::max(t1, tp[4 * i + 1]);
+    }
+    std::vector<double> sd = dur;
+    std::sort(sd.begin(), sd.end());
+    double sum = 0;
+    for (double v : sd) sum += v;
+    fprintf(stderr,
+            "[knnj tile profile] tiles=%d span=%.0f ticks sum=%.0f mean=%.1f p50=%.0f p99=%.0f max=%.0f "
+            "listed/tile=%.1f visited/tile=%.1f\n",
+            hc[0], (double)(t1 - t0), sum, sum / hc[0], sd[hc[0] / 2], sd[(size_t)(0.99 * (hc[0] - 1))],
+            sd.back(), listed / hc[0], visited / hc[0]);
+    if (hc[1] > 0) {
+      float dg[256];
+      HIPCHK(hipMemcpy(dg, a.ovf_diag, sizeof(dg), hipMemcpyDeviceToHost));
+      for (int i = 0; i < std::min(hc[1], 8); ++i)
+        fprintf(stderr, "[knnj overflow] cutoff=%.6g kth=%.6g last=%.6g first=%.6g\n", dg[4 * i], dg[4 * i + 1],
+                dg[4 * i + 2], dg[4 * i + 3]);
+    }
+  }
   HIPCHK(hipEventRecord(c->ev[5], s));
   if (hc[1] > 0) {
+    // overflowed queries, one per tile, at a larger capacity: the MFMA kernel
+    // at 512 if it ran below that, else the VALU kernel at 1024
     JoinArgs b = a;
-    b.kcap = KNNJ_MAX_K;
-    b.exact = exact_valu ? 1 : 0;
     b.tiles = ovf_tiles;
     b.ntiles = counters + 1;
     b.ovf_tiles = ovf2_tiles;
     b.ovf_count = counters + 2;
-    HIPCHK(launch_join(b, 1, hc[1], s));
+    b.tile_prof = nullptr;
+    b.ovf_diag = nullptr;
+    b.ecode = (b.kcap <= 512 && use_mfma && kcap < 512) ? a.ecode : 0;
+    if (use_mfma && kcap < 512) {
+      b.kcap = 512;
+      if (bf16)
+        HIPCHK(launch_join_mfma_bf16(b, bf16x1 ? 1 : 3, hc[1], s));
+      else
+        HIPCHK(launch_join_mfma_f32(b, hc[1], s));
+    } else {
+      b.kcap = KNNJ_MAX_K;
+      b.exact = exact_valu ? 1 : 0;
+      HIPCHK(launch_join(b, 1, hc[1], s));
+    }
     int32_t h2 = 0;
     HIPCHK(hipMemcpyAsync(&h2, counters + 2, sizeof(h2), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (h2 > 0) {
-      set_err("%d queries overflowed the largest candidate capacity (%d)", h2, KNNJ_MAX_K);
-      return KNNJ_EOVERFLOW;
+      // last resort: the VALU kernel at the largest capacity
+      JoinArgs c2 = b;
+      c2.tiles = ovf2_tiles;
+      c2.ntiles = counters + 2;
+      c2.ovf_tiles = ovf_tiles;
+      c2.ovf_count = counters + 4;
+      c2.kcap = KNNJ_MAX_K;
+      c2.exact = exact_valu ? 1 : 0;
+      c2.ecode = 0;
+      if (!(use_mfma && kcap < 512)) {
+        set_err("%d queries overflowed the largest candidate capacity (%d)", h2, KNNJ_MAX_K);
+        return KNNJ_EOVERFLOW;
+      }
+      HIPCHK(launch_join(c2, 1, h2, s));
+      int32_t h3 = 0;
+      HIPCHK(hipMemcpyAsync(&h3, counters + 4, sizeof(h3), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (h3 > 0) {
+        set_err("%d queries overflowed the largest candidate capacity (%d)", h3, KNNJ_MAX_K);
+        return KNNJ_EOVERFLOW;
+      }
     }
   }
   HIPCHK(hipEventRecord(c->ev[6], s));

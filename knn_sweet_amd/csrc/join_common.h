@@ -10,14 +10,19 @@
 // Exact mode: the keys are (exact binary32 squared distance, original row);
 // the first k are the answer.
 // Filter mode: the keys are (upper bound of the squared distance, permuted
-// reference position).  Every entry with key distance <= cutoff is re-scored
-// with the oracle's binary64 sequence and the entries are sorted by
-// (binary64 squared distance, original row).  If the list's last entry is
-// itself within the cutoff, a needed candidate may have been evicted: the
-// query is appended to the overflow list and nothing is written.
+// reference position).  Entries that may belong to the answer are re-scored
+// with the oracle's binary64 sequence and sorted by (binary64 squared
+// distance, original row):
+//  - with per-entry error codes (a.ecode): entries whose lower bound is
+//    <= theta; an overflow is an evicted entry whose lower bound (evl) is
+//    <= theta;
+//  - otherwise: entries with key distance <= cutoff; an overflow is a last
+//    list entry within the cutoff.
+// An overflowed query is appended to the overflow list and nothing is written.
 template <int KCAP, bool EXACT>
 __device__ __forceinline__ void emit_query(const JoinArgs &a, const u64 *L, int qpos, int cluster,
-                                           float cutoff, const float *qrow) {
+                                           float cutoff, const float *qrow, float theta = 0.f,
+                                           float evl = INFINITY) {
   const int lane = threadIdx.x & 63;
   const int k = a.k;
   const int64_t qo = a.Q.perm[qpos];
@@ -38,7 +43,9 @@ __device__ __forceinline__ void emit_query(const JoinArgs &a, const u64 *L, int 
     return;
   }
   const u64 last = L[KCAP - 1];
-  if (last != KEY_MAX && key_dist(last) <= cutoff) {
+  const float thr_l = theta * (1.f + 0x1p-20f);
+  const bool ovf = a.ecode ? (evl <= thr_l) : (last != KEY_MAX && key_dist(last) <= cutoff);
+  if (ovf) {
     if (lane == 0) {
       const int slot = atomicAdd(a.ovf_count, 1);
       Tile t;
@@ -47,6 +54,14 @@ __device__ __forceinline__ void emit_query(const JoinArgs &a, const u64 *L, int 
       t.cluster = cluster;
       t.pad = 0;
       a.ovf_tiles[slot] = t;
+      if (a.ovf_diag && slot < 64) {
+        // diagnostic record: cutoff, k-th key, last key, first key
+        float *dg = a.ovf_diag + 4 * slot;
+        dg[0] = cutoff;
+        dg[1] = key_dist(L[k - 1]);
+        dg[2] = key_dist(last);
+        dg[3] = key_dist(L[0]);
+      }
     }
     return;
   }
@@ -58,11 +73,11 @@ __device__ __forceinline__ void emit_query(const JoinArgs &a, const u64 *L, int 
 #pragma unroll
   for (int u = 0; u < E; ++u) {
     const u64 key = L[lane + 64 * u];
-    const bool ok = key != KEY_MAX && key_dist(key) <= cutoff;
+    const bool ok = key != KEY_MAX && (a.ecode ? key_lower(key, a.ebase) <= thr_l : key_dist(key) <= cutoff);
     hs[u] = KEY_MAX;
     rs[u] = 0xFFFFFFFFu;
     if (ok) {
-      const int p = (int)key_row(key);
+      const int p = (int)(a.ecode ? (key_row(key) & KEY_ROW_MASK) : key_row(key));
       const double s64 = sq64_pair(qrow, a.R.px + (int64_t)p * dpR, d);
       hs[u] = (u64)__double_as_longlong(s64);
       rs[u] = (u32)a.R.perm[p];

@@ -56,33 +56,67 @@ void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags,
 // Representative sampling score: for trial t, the sum over unordered pairs
 // i < j of the FP32 distance between candidate rows (the reference sums
 // int(distance) over ordered pairs, knnjoin.cu:125-133; the argmax is the
-// same up to that truncation).  Partial sums per block, reduced in a fixed
-// order so the choice is deterministic.
+// same up to that truncation).  The m x m pair matrix of a trial is cut into
+// 64 x 64 blocks (upper triangle); a workgroup stages the two blocks' rows in
+// LDS 32 dimensions at a time and each thread accumulates a 4 x 4 register
+// tile.  Block partial sums are reduced in a fixed order (k_rep_pick), so the
+// choice is deterministic.
+#define RS_B 64
+#define RS_DC 32
 __global__ __launch_bounds__(256) void k_rep_score(const float *__restrict__ X, int64_t ld, int d,
-                                                   const int32_t *__restrict__ cand, int m,
-                                                   double *partial) {
-  const int t = blockIdx.y;
-  const int64_t npairs = (int64_t)m * (m - 1) / 2;
-  const int32_t *c = cand + (int64_t)t * m;
-  double acc = 0.0;
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npairs;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    // p -> (i, j), j < i, p = i(i-1)/2 + j
-    int64_t i = (int64_t)((1.0 + sqrt(1.0 + 8.0 * (double)p)) * 0.5);
-    while (i * (i - 1) / 2 > p) --i;
-    while ((i + 1) * i / 2 <= p) ++i;
-    const int64_t j = p - i * (i - 1) / 2;
-    const float *a = X + (int64_t)c[i] * ld;
-    const float *b = X + (int64_t)c[j] * ld;
-    float s = 0.f;
-    for (int q = 0; q < d; ++q) {
-      const float df = a[q] - b[q];
-      s = __builtin_fmaf(df, df, s);
-    }
-    acc += (double)sqrtf(s);
-  }
+                                                   const int32_t *__restrict__ cand, int m, double *partial) {
+  __shared__ float si[RS_DC][RS_B + 1], sj[RS_DC][RS_B + 1];
   __shared__ double red[256];
-  red[threadIdx.x] = acc;
+  const int t = blockIdx.y;
+  const int nb = (m + RS_B - 1) / RS_B;
+  // block index -> (bi, bj), bi <= bj
+  int rem = blockIdx.x, bi = 0;
+  while (rem >= nb - bi) {
+    rem -= nb - bi;
+    ++bi;
+  }
+  const int bj = bi + rem;
+  const int32_t *c = cand + (int64_t)t * m;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // 16 x 16 threads, 4 x 4 pairs each
+  float acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+  for (int t0 = 0; t0 < d; t0 += RS_DC) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < RS_B * RS_DC; e += blockDim.x) {
+      const int r = e / RS_DC, q = e % RS_DC;
+      const int gi = bi * RS_B + r, gj = bj * RS_B + r;
+      si[q][r] = (gi < m && t0 + q < d) ? X[(int64_t)c[gi] * ld + t0 + q] : 0.f;
+      sj[q][r] = (gj < m && t0 + q < d) ? X[(int64_t)c[gj] * ld + t0 + q] : 0.f;
+    }
+    __syncthreads();
+    for (int q = 0; q < RS_DC; ++q) {
+      float vi[4], vj[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        vi[a] = si[q][ty * 4 + a];
+        vj[a] = sj[q][tx * 4 + a];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const float df = vi[a] - vj[b];
+          acc[a][b] = __builtin_fmaf(df, df, acc[a][b]);
+        }
+    }
+  }
+  double sum = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int gi = bi * RS_B + ty * 4 + a, gj = bj * RS_B + tx * 4 + b;
+      if (gi < m && gj < m && gi < gj) sum += (double)sqrtf(acc[a][b]);
+    }
+  red[threadIdx.x] = sum;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
@@ -96,18 +130,25 @@ __global__ __launch_bounds__(256) void k_rep_score(const float *__restrict__ X, 
 __global__ void k_rep_pick(const double *__restrict__ partial, int nblk, int trials,
                            const int32_t *__restrict__ cand, int m, const float *__restrict__ X,
                            int64_t ld, int d, int dpad, float *reps, int32_t *rep_rows) {
+  __shared__ double tot[64];
   __shared__ int best;
+  // one wave per trial slot: lane-strided sums, then a fixed shuffle tree
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int t = w; t < trials; t += blockDim.x / 64) {
+    double s = 0.0;
+    for (int b = lane; b < nblk; b += 64) s += partial[(int64_t)t * nblk + b];
+    s = wave_sum_d(s);
+    if (lane == 0) tot[t] = s;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     double bs = -1.0;
     int bt = 0;
-    for (int t = 0; t < trials; ++t) {
-      double s = 0.0;
-      for (int b = 0; b < nblk; ++b) s += partial[(int64_t)t * nblk + b];
-      if (s > bs) {
-        bs = s;
+    for (int t = 0; t < trials; ++t)
+      if (tot[t] > bs) {
+        bs = tot[t];
         bt = t;
       }
-    }
     best = bt;
   }
   __syncthreads();
@@ -119,12 +160,17 @@ __global__ void k_rep_pick(const double *__restrict__ partial, int nblk, int tri
   }
 }
 
+int rep_score_blocks(int m) {
+  const int nb = (m + RS_B - 1) / RS_B;
+  return nb * (nb + 1) / 2;
+}
+
 void launch_rep_select(const float *X, int64_t ld, int d, int dpad, const int32_t *cand, int m,
                        int trials, double *partial, int nblk, float *reps, int32_t *rep_rows,
                        hipStream_t s) {
   hipLaunchKernelGGL(k_rep_score, dim3(nblk, trials), dim3(256), 0, s, X, ld, d, cand, m, partial);
-  hipLaunchKernelGGL(k_rep_pick, dim3(1), dim3(256), 0, s, partial, nblk, trials, cand, m, X, ld, d,
-                     dpad, reps, rep_rows);
+  hipLaunchKernelGGL(k_rep_pick, dim3(1), dim3(1024), 0, s, partial, nblk, trials, cand, m, X, ld, d, dpad, reps,
+                     rep_rows);
 }
 
 // ---------------------------------------------------------------------------
@@ -474,15 +520,43 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
 
 // ---------------------------------------------------------------------------
 // Query tiles: cluster A of the query side is cut into ceil(|A| / tq) tiles
-// of consecutive permuted queries.  One workgroup of 1024.
-__global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__ off, int m, int tq,
-                                                     Tile *tiles, int32_t *ntiles) {
+// of consecutive permuted queries.  Clusters are emitted in descending order
+// of their surviving-pair count (the work estimate), so the longest tiles
+// start first and the short ones fill the tail.  One workgroup of 1024.
+__global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__ off, int m,
+                                                     const int32_t *__restrict__ pcnt, int tq, Tile *tiles,
+                                                     int32_t *ntiles) {
+  __shared__ u64 ord[KNNJ_MAX_REPS];
   __shared__ int32_t part[1024];
+  int P = 1;
+  while (P < m) P <<= 1;
+  for (int c = threadIdx.x; c < P; c += blockDim.x)
+    ord[c] = c < m ? (((u64)(0xFFFFFFFFu - (u32)pcnt[c]) << 32) | (u32)c) : KEY_MAX;
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int mask = (stride == (size >> 1)) ? (size - 1) : stride;
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const int j = i ^ mask;
+        if (j > i) {
+          const u64 x = ord[i], y = ord[j];
+          if (y < x) {
+            ord[i] = y;
+            ord[j] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
   const int per = (m + 1023) / 1024;
   const int b = threadIdx.x * per;
   int32_t s = 0;
   for (int j = 0; j < per; ++j)
-    if (b + j < m) s += (off[b + j + 1] - off[b + j] + tq - 1) / tq;
+    if (b + j < m) {
+      const int c = (int)(u32)ord[b + j];
+      s += (off[c + 1] - off[c] + tq - 1) / tq;
+    }
   part[threadIdx.x] = s;
   __syncthreads();
   for (int w = 1; w < 1024; w <<= 1) {
@@ -493,8 +567,8 @@ __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__
   }
   int32_t run = part[threadIdx.x] - s;
   for (int j = 0; j < per; ++j) {
-    const int c = b + j;
-    if (c >= m) break;
+    if (b + j >= m) break;
+    const int c = (int)(u32)ord[b + j];
     const int a0 = off[c], na = off[c + 1] - a0;
     for (int q = 0; q < na; q += tq) {
       Tile t;
@@ -508,8 +582,9 @@ __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__
   if (threadIdx.x == 1023) *ntiles = part[1023];
 }
 
-void launch_make_tiles(const int32_t *off, int m, int tq, Tile *tiles, int32_t *ntiles, hipStream_t s) {
-  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, s, off, m, tq, tiles, ntiles);
+void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, Tile *tiles, int32_t *ntiles,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, s, off, m, pcnt, tq, tiles, ntiles);
 }
 
 // Members of every cluster in ascending row order (counts already in cnt).

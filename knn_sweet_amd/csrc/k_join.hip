@@ -39,6 +39,8 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
   const int lane = threadIdx.x;
   if ((int)blockIdx.x >= *a.ntiles) return;
   const Tile tile = a.tiles[blockIdx.x];
+  const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
+  u64 nvisit = 0;
   const int A = tile.cluster;
   const int nq = tile.count;
   const int d = a.d, dpR = a.R.dpad, dpQ = a.Q.dpad, k = a.k;
@@ -82,11 +84,51 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
 
   const int np = a.pcnt[A];
   const int32_t *pl = a.plist + (int64_t)A * a.R.m;
-  for (int pi = 0; pi < np; ++pi) {
-    const int b = pl[pi];
+  for (int pb = 0; pb < np; pb += 64) {
+    // Screening: lane j takes listed cluster pb + j, computes the distance of
+    // every query of the tile to its representative, and keeps the cluster
+    // if the lower bound d(q,b) - r_b does not exceed some query's threshold.
+    const int pj = pb + lane;
+    const bool lv = pj < np;
+    const int bj = lv ? pl[pj] : pl[pb];
+    const float rbj = (float)a.R.rad[bj] * (1.f + 2.4e-7f);
+    const float *repj = a.R.reps + (int64_t)bj * dpR;
+    float dql[TQ];
+#pragma unroll
+    for (int i = 0; i < TQ; ++i) dql[i] = 0.f;
+    for (int t = 0; t < dpR; t += 4) {
+      const float4 c4 = *(const float4 *)(repj + t);
+#pragma unroll
+      for (int i = 0; i < TQ; ++i) {
+        if (i < nq) {
+          const float4 q4 = *(const float4 *)(qrow[i] + t);
+          float df = q4.x - c4.x;
+          dql[i] = __builtin_fmaf(df, df, dql[i]);
+          df = q4.y - c4.y;
+          dql[i] = __builtin_fmaf(df, df, dql[i]);
+          df = q4.z - c4.z;
+          dql[i] = __builtin_fmaf(df, df, dql[i]);
+          df = q4.w - c4.w;
+          dql[i] = __builtin_fmaf(df, df, dql[i]);
+        }
+      }
+    }
+    bool needj = false;
+#pragma unroll
+    for (int i = 0; i < TQ; ++i) {
+      dql[i] = sqrtf(dql[i]);
+      if (i < nq) {
+        const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
+        needj |= !(dql[i] * (1.f - 2.f * eps) - rbj > thi);
+      }
+    }
+    u64 pending = wave_ballot(lv && needj);
+    while (pending) {
+    const int jl = __builtin_ctzll(pending);
+    pending &= pending - 1;
+    const int b = __builtin_amdgcn_readlane(bj, jl);
     const int s0 = a.R.off[b], s1 = a.R.off[b + 1];
-    const float rb = (float)a.R.rad[b] * (1.f + 2.4e-7f);
-    const float *rep = a.R.reps + (int64_t)b * dpR;
+    const float rb = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, rbj), jl));
     bool use[TQ];
     bool any = false;
     float lo = INFINITY, hi = -INFINITY;
@@ -94,12 +136,7 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
     for (int i = 0; i < TQ; ++i) {
       use[i] = false;
       if (i < nq) {
-        float part = 0.f;
-        for (int j = lane; j < d; j += 64) {
-          const float df = qrow[i][j] - rep[j];
-          part = __builtin_fmaf(df, df, part);
-        }
-        const float dqb = sqrtf(wave_sum(part));
+        const float dqb = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dql[i]), jl));
         const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
         const float lb = dqb * (1.f - 2.f * eps) - rb;
         if (!(lb > thi)) {
@@ -111,6 +148,7 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
       }
     }
     if (!any) continue;
+    ++nvisit;
     lo = lo > 0.f ? lo * (1.f - eps) : -INFINITY;
     hi = hi * (1.f + eps);
 
@@ -181,6 +219,7 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
       }
     }
   }
+  }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < TQ; ++i)
@@ -196,6 +235,13 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
       if (j == i) ti = thr[j];
     emit_query<KCAP, EXACT>(a, lists + i * KCAP, tile.start + i, A, key_dist(ti),
                             a.Q.px + (int64_t)(tile.start + i) * dpQ);
+  }
+  if (lane == 0 && a.tile_prof) {
+    u64 *tp = a.tile_prof + 4 * (size_t)blockIdx.x;
+    tp[0] = clk0;
+    tp[1] = wall_clock64();
+    tp[2] = (u64)np;
+    tp[3] = nvisit;
   }
   if (lane == 0) {
     atomicAdd(a.n_evals, evals);

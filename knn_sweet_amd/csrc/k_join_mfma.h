@@ -9,24 +9,39 @@
 // q-c is formed per (tile, cluster) into the A operand registers, with
 // ||q-c||^2 in binary32; r-c and ||r-c||^2 are precomputed (k_center.hip).
 // The dot runs as v_mfma_f32_16x16x32_bf16 (BF16 filter) or
-// v_mfma_f32_16x16x4_f32 (plain FP32), FP32 accumulation.
+// v_mfma_f32_16x16x4_f32 (plain FP32), FP32 accumulation.  Lane (row c16,
+// group g) holds dimensions 32 s + 8 g + j, j < 8, of every 32-wide step s;
+// for the FP32 instruction the k-slot g of sub-step j stands for dimension
+// 32 s + 8 g + j, so both instructions read the same 16-byte row slices.
+//
+// Before any of that, the reference clusters listed for the tile are screened
+// 16 at a time: one FP32 MFMA block gives ||q - c_b|| for all 16 x 16
+// (query, cluster) pairs, and only clusters whose lower bound
+// d(q, c_b) - r_b can reach some query's threshold are visited.
 //
 // Rigorous error bound (Higham, Accuracy and Stability of Numerical
-// Algorithms, 2nd ed., section 3.1; u = 2^-24, u_in = 2^-8 for bf16 operands
-// and 0 for fp32, gamma_n = n u / (1 - n u)): with a = ||q-c||, R >= ||r-c||
-// (the cluster radius), the computed s_f satisfies
+// Algorithms, 2nd ed., section 3.1; u = 2^-24, u_b = 2^-8, gamma_n =
+// n u / (1 - n u)): with a = ||q-c||, R >= ||r-c|| (the cluster radius), the
+// computed s_f satisfies
 //     |s_f - s| <= E = c1 * a * R + c2 * (a^2 + R^2),
-//     c1 = 2 (2 u_in + u_in^2 + gamma_d (1 + u_in)^2) + 4u,
-//     c2 = gamma_{d+3} + 8u      (norm errors, the final two roundings, the
-//                                  binary32 centring q-c and r-c),
-// both scaled by 1.01 on the host.  Integer inputs with 4 d (2 max|x|)^2 <
+//     c1 = 2 cdot + 4u,  c2 = gamma_{d+3} + 8u  (norm errors, the final two
+//                                  roundings, the binary32 centring),
+// where |computed dot - (q-c).(r-c)| <= cdot ||q-c|| ||r-c|| and
+//     fp32 operands:          cdot = gamma_d
+//     one bf16 product:       cdot = 2 u_b + u_b^2 + gamma_d (1 + u_b)^2
+//     bf16 hi/lo, 3 products: cdot = 3.1 u_b^2 + 1.02 gamma_{3d}
+// (hi = bf16(x), lo = bf16(x - hi) leaves |x - hi - lo| <= u_b^2 |x|; the
+// dropped lo*lo product and the two residuals are <= 3.1 u_b^2 |q_j r_j|;
+// every bf16 x bf16 product is exact in FP32), all scaled by 1.01 on the
+// host.  Integer inputs with 4 d (2 max|x|)^2 <
 // 2^24 make every FP32 step exact (E = 0, exact mode).
 //
 // List keys are (U = s_f + E, row).  theta_q is the k-th smallest U; a
-// candidate passes when L = s_f - E <= theta_q.  E_max(q) is the largest E the
-// query used; at the end every list entry with U <= theta_q + 2 E_max is
-// re-scored in binary64 (emit_query), and a list whose last entry is still
-// within that cutoff reports an overflow.
+// candidate passes when L = s_f - E <= theta_q.  E_max(q) is the largest E of
+// any candidate appended for the query (an entry in the list or evicted from
+// it was appended, so its L >= U - 2 E_max); at the end every list entry with
+// U <= theta_q + 2 E_max is re-scored in binary64 (emit_query), and a list
+// whose last entry is still within that cutoff reports an overflow.
 #pragma once
 
 #include <type_traits>
@@ -36,53 +51,87 @@
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
 
+// lower bound of an evicted filter key (per-entry error code)
+struct DropLower {
+  static constexpr bool tracks = true;
+  int eb;
+  __device__ float operator()(u64 key) const { return key_lower(key, eb); }
+};
+template <class D>
+__device__ __forceinline__ D make_drop(int eb) {
+  if constexpr (D::tracks)
+    return D{eb};
+  else
+    return D{};
+}
+
 #define MF_TQ 16
 #define MF_BC 64  // per-query candidate buffer
 #define MF_WM 48  // merge watermark: a 16-column block adds at most 16
 
-template <bool BF16, int DK>
-struct AOps {
-  static constexpr int KS = BF16 ? 32 : 16;  // dimensions per step
-  static constexpr int NS = DK / KS;
-  typedef typename std::conditional<BF16, bf16x8_t, f32x4_t>::type T;
-  T v[NS];
-};
-
-template <int KCAP, bool EXACT, bool BF16, int DK>
+template <int KCAP, bool EXACT, int PREC, int DK>
 __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
+  // PREC: 0 fp32 operands, 1 bf16, 3 bf16 hi/lo (three products)
+  constexpr bool BF16 = PREC != 0;
+  constexpr bool SPLIT = PREC == 3;
+  constexpr int NS = DK / 32;  // 32-dimension steps
   extern __shared__ u64 smem[];
   u64 *lists = smem;                        // [16][KCAP]
   u64 *bufs = lists + MF_TQ * KCAP;         // [16][MF_BC]
   u64 *tmp = bufs + MF_TQ * MF_BC;          // [KCAP]
-  float *fsh = (float *)(tmp + KCAP);       // [16]
+  float *fsh = (float *)(tmp + KCAP);       // [16] row norms
+  float *dqs = fsh + 16;                    // [16 rows][16 cols] screening distances
   const int lane = threadIdx.x, g = lane >> 4, c16 = lane & 15;
   if ((int)blockIdx.x >= *a.ntiles) return;
   const Tile tile = a.tiles[blockIdx.x];
+  const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
+  u64 nvisit = 0;
   const int A = tile.cluster, nq = tile.count;
-  const int d = a.d, k = a.k, dk = a.dk, dpQ = a.Q.dpad, dpR = a.R.dpad;
+  const int d = a.d, k = a.k, dk = a.dk, dpQ = a.Q.dpad;
   const float eps = dist_relerr(d);
-  typedef AOps<BF16, DK> Ops;
-  const int nst = dk / Ops::KS;
+  const int nst = dk / 32;
 
   for (int e = lane; e < MF_TQ * KCAP; e += 64) lists[e] = KEY_MAX;
-  __syncthreads();
 
   const double th = a.theta[A];
   const float T0 = (float)(th * th * (1.0 + 1e-6));
   // per-lane state of this lane's four C rows r = 4g + i
   u64 thr[4];
   int cnt[4];
-  float emax[4];
+  float emax[4], evl[4];
   bool live[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     thr[i] = ((u64)__float_as_uint(T0) << 32) | 0xFFFFFFFFull;
     cnt[i] = 0;
     emax[i] = 0.f;
+    evl[i] = INFINITY;
     live[i] = (4 * g + i) < nq;
   }
-  const bool alive = c16 < nq;  // this lane's A-operand row
+  const int ebase = a.ebase;
+  typedef typename std::conditional<EXACT, NoDrop, DropLower>::type DropL;
+  // this lane's slice of query row c16: dimensions 32 s + 8 g + j, in FP32
+  const bool alive = c16 < nq;
   const float *qa = a.Q.px + (int64_t)(tile.start + (alive ? c16 : 0)) * dpQ;
+  float qf[NS][8];
+  float qn = 0.f;
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int dim = 32 * s + 8 * g + j;
+      qf[s][j] = (alive && s < nst && dim < d) ? qa[dim] : 0.f;
+      qn = __builtin_fmaf(qf[s][j], qf[s][j], qn);
+    }
+  qn += __shfl_xor(qn, 16);
+  qn += __shfl_xor(qn, 32);
+  if (g == 0) fsh[c16] = qn;
+  __syncthreads();
+  float qnr[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) qnr[i] = fsh[4 * g + i];
+  __syncthreads();
+  const float gE = 1.01f * ((float)(d + 3) * 5.97e-8f + 8.f * 5.97e-8f);  // gamma_{d+3} + 8u, with margin
   u64 evals = 0, nmerge = 0, npass = 0;
 
   // merge of query r (wave-uniform r)
@@ -93,7 +142,7 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
     for (int j = 1; j < 4; ++j)
       if (j == ii) mine = cnt[j];
     const int cr = __shfl(mine, 16 * gi);
-    wave_merge<KCAP, MF_BC>(lists + r * KCAP, bufs + r * MF_BC, cr, tmp);
+    const float dropped = wave_merge<KCAP, MF_BC>(lists + r * KCAP, bufs + r * MF_BC, cr, tmp, make_drop<DropL>(ebase));
     ++nmerge;
     const u64 kth = lists[r * KCAP + k - 1];
     u64 t = KEY_MAX;
@@ -104,145 +153,227 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
         if (j == ii) {
           cnt[j] = 0;
           if (t < thr[j]) thr[j] = t;
+          evl[j] = fminf(evl[j], dropped);
         }
     }
   };
 
   const int np = a.pcnt[A];
   const int32_t *pl = a.plist + (int64_t)A * a.R.m;
-  for (int pi = 0; pi < np; ++pi) {
-    const int b = pl[pi];
-    const int s0 = a.R.off[b], s1 = a.R.off[b + 1];
-    const float Rb = (float)(a.R.rad[b] * (1.0 + 1e-6));
-    const float *rep = a.R.reps + (int64_t)b * dpR;
-
-    // A operand: q - c for row c16, dimensions of k-group g
-    Ops av;
-    float part = 0.f;
+  for (int pb = 0; pb < np; pb += 16) {
+    // ---- screening of 16 listed clusters: ||q - c||^2 = ||q||^2 + ||c||^2 - 2 q.c
+    // on v_mfma_f32_16x16x4_f32 (rows = queries, columns = representatives),
+    // error <= gE (||q|| + ||c||)^2; a cluster is kept if d(q,c) - r_b can be
+    // <= theta_q for some query.
+    const int pj = pb + c16;
+    const bool lv = pj < np;
+    const int bj = lv ? pl[pj] : pl[pb];
+    {
+      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+      const float *cr = a.reps32 + (int64_t)bj * dk + 8 * g;
 #pragma unroll
-    for (int s = 0; s < Ops::NS; ++s) {
-      constexpr int W = BF16 ? 8 : 4;
+      for (int s = 0; s < NS; ++s)
+        if (s < nst) {
+          const f32x4_t c0 = *(const f32x4_t *)(cr + 32 * s);
+          const f32x4_t c1 = *(const f32x4_t *)(cr + 32 * s + 4);
 #pragma unroll
-      for (int j = 0; j < W; ++j) {
-        const int dim = s * Ops::KS + W * g + j;
-        const float v = (alive && s < nst && dim < d) ? qa[dim] - rep[dim] : 0.f;
-        part = __builtin_fmaf(v, v, part);
-        if constexpr (BF16)
-          av.v[s][j] = (__bf16)v;
-        else
-          av.v[s][j] = v;
-      }
-    }
-    part += __shfl_xor(part, 16);
-    part += __shfl_xor(part, 32);
-    if (g == 0) fsh[c16] = part;
-    __syncthreads();
-    float nqr[4];
+          for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qf[s][j], c0[j], acc, 0, 0, 0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) nqr[i] = fsh[4 * g + i];
-    __syncthreads();
-
-    bool use[4];
-    float Ei[4];
-    float lo = INFINITY, hi = -INFINITY;
-    bool anyl = false;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float dqb = sqrtf(nqr[i]);
-      const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
-      const float lb = dqb * (1.f - 2.f * eps) - Rb;
-      use[i] = live[i] && !(lb > thi);
-      Ei[i] = EXACT ? 0.f : a.c1 * dqb * (1.f + eps) * Rb + a.c2 * (nqr[i] + Rb * Rb);
-      if (use[i]) {
-        anyl = true;
-        lo = fminf(lo, dqb * (1.f - 2.f * eps) - thi);
-        hi = fmaxf(hi, dqb * (1.f + 2.f * eps) + thi);
-        emax[i] = fmaxf(emax[i], Ei[i]);
-      }
-    }
-    if (wave_ballot(anyl) == 0) continue;
-    lo = wave_min(lo);
-    hi = wave_max(hi);
-    lo = lo > 0.f ? lo * (1.f - eps) : -INFINITY;
-    hi = hi * (1.f + eps);
-
-    const int nch = (s1 - s0 + 63) >> 6;
-    for (int cg = 0; cg < nch; cg += 64) {
-      const int c = cg + lane;
-      const float cmin = c < nch ? a.R.pdist[s0 + 64 * c] : INFINITY;
-      float nmin = __shfl_down(cmin, 1);
-      if (lane == 63) nmin = (c + 1 < nch) ? a.R.pdist[s0 + 64 * (c + 1)] : INFINITY;
-      const bool need = (c < nch) && (cmin <= hi) && (nmin >= lo);
-      const u64 m = wave_ballot(need);
-      if (m == 0) {
-        if (__shfl(cmin, 0) > hi) break;
-        continue;
-      }
-      const int pbeg = s0 + 64 * (cg + __builtin_ctzll(m));
-      const int pend = min(s1, s0 + 64 * (cg + 64 - __builtin_clzll(m)));
-      for (int p0 = pbeg; p0 < pend; p0 += 16) {
-        const int p = p0 + c16;
-        const bool valid = p < s1;
-        const int pp = valid ? p : p0;
-        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-        if constexpr (BF16) {
-          const __bf16 *rr = (const __bf16 *)a.rc + (int64_t)pp * dk + 8 * g;
-#pragma unroll
-          for (int s = 0; s < Ops::NS; ++s)
-            if (s < nst) {
-              const bf16x8_t bv = *(const bf16x8_t *)(rr + 32 * s);
-              acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av.v[s], bv, acc, 0, 0, 0);
-            }
-        } else {
-          const float *rr = (const float *)a.rc + (int64_t)pp * dk + 4 * g;
-#pragma unroll
-          for (int s = 0; s < Ops::NS; ++s)
-            if (s < nst) {
-              const f32x4_t bv = *(const f32x4_t *)(rr + 16 * s);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.v[s][0], bv[0], acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.v[s][1], bv[1], acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.v[s][2], bv[2], acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.v[s][3], bv[3], acc, 0, 0, 0);
-            }
+          for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qf[s][4 + j], c1[j], acc, 0, 0, 0);
         }
-        const float nrv = a.nr[pp];
-        const u32 row = EXACT ? (u32)a.R.perm[pp] : (u32)pp;
-        bool pass[4];
-        u64 key[4];
-        bool anyp = false;
+      const float cn = a.rn32[bj];
+      const float rbj = (float)(a.R.rad[bj] * (1.0 + 1e-6));
+      bool need = false;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float sqc = (qnr[i] + cn) - 2.f * acc[i];
+        const float nsum = sqrtf(qnr[i]) + sqrtf(cn);
+        const float dlo = sqrtf(fmaxf(sqc - gE * nsum * nsum, 0.f));
+        const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
+        need |= lv && live[i] && !(dlo * (1.f - eps) - rbj > thi);
+        dqs[(4 * g + i) * 16 + c16] = dlo;
+      }
+      need |= __shfl_xor((int)need, 16) != 0;
+      need |= __shfl_xor((int)need, 32) != 0;
+      __syncthreads();
+      u64 pending = wave_ballot(need) & 0xFFFFull;
+      while (pending) {
+        const int jl = __builtin_ctzll(pending);
+        pending &= pending - 1;
+        const int b = __builtin_amdgcn_readlane(bj, jl);
+        const int s0 = a.R.off[b], s1 = a.R.off[b + 1];
+        const float Rb = (float)(a.R.rad[b] * (1.0 + 1e-6));
+        // re-check with the thresholds as they stand now
+        bool use[4];
+        bool anyl = false;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const float sf = fmaxf((nqr[i] + nrv) - 2.f * acc[i], 0.f);
-          if (EXACT) {
-            key[i] = make_key(sf, row);
-            pass[i] = valid && use[i] && key[i] <= thr[i];
-          } else {
-            key[i] = make_key(sf + Ei[i], row);
-            pass[i] = valid && use[i] && (sf - Ei[i]) <= key_dist(thr[i]);
-          }
-          anyp |= pass[i];
-          evals += __popcll(wave_ballot(valid && use[i]));
+          const float dlo = dqs[(4 * g + i) * 16 + jl];
+          const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
+          use[i] = live[i] && !(dlo * (1.f - eps) - Rb > thi);
+          anyl |= use[i];
         }
-        if (wave_ballot(anyp) != 0) {
-          const u32 below = (1u << c16) - 1u;
+        if (wave_ballot(anyl) == 0) continue;
+        ++nvisit;
+        // A operand: q - c for row c16, dimensions of k-group g
+        const float *rep = a.reps32 + (int64_t)b * dk + 8 * g;
+        typename std::conditional<BF16, bf16x8_t, float[8]>::type av[NS];
+        bf16x8_t avl[SPLIT ? NS : 1];
+        float part = 0.f;
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const u64 mk = wave_ballot(pass[i]);
-            const u32 gm = (u32)(mk >> (16 * g)) & 0xFFFFu;
-            if (pass[i]) bufs[(4 * g + i) * MF_BC + cnt[i] + __popc(gm & below)] = key[i];
-            cnt[i] += __popc(gm);
-            npass += __popcll(mk);
+        for (int s = 0; s < NS; ++s) {
+          if (s < nst) {
+            const f32x4_t c0 = *(const f32x4_t *)(rep + 32 * s);
+            const f32x4_t c1 = *(const f32x4_t *)(rep + 32 * s + 4);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              const float v = alive ? qf[s][j] - (j < 4 ? c0[j] : c1[j - 4]) : 0.f;
+              part = __builtin_fmaf(v, v, part);
+              if constexpr (BF16) {
+                const __bf16 h = (__bf16)v;
+                av[s][j] = h;
+                if constexpr (SPLIT) avl[s][j] = (__bf16)(v - (float)h);
+              } else {
+                av[s][j] = v;
+              }
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              if constexpr (BF16) {
+                av[s][j] = (__bf16)0.f;
+                if constexpr (SPLIT) avl[s][j] = (__bf16)0.f;
+              } else {
+                av[s][j] = 0.f;
+              }
+            }
           }
-          const bool hot = cnt[0] >= MF_WM || cnt[1] >= MF_WM || cnt[2] >= MF_WM || cnt[3] >= MF_WM;
-          if (wave_ballot(hot) != 0) {
-            __syncthreads();
-            for (int r = 0; r < MF_TQ; ++r) {
-              const int gi = r >> 2, ii = r & 3;
-              int mine = cnt[0];
+        }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        if (g == 0) fsh[c16] = part;
+        __syncthreads();
+        float nqr[4];
 #pragma unroll
-              for (int j = 1; j < 4; ++j)
-                if (j == ii) mine = cnt[j];
-              if (__shfl(mine, 16 * gi) >= MF_WM) merge(r);
+        for (int i = 0; i < 4; ++i) nqr[i] = fsh[4 * g + i];
+        __syncthreads();
+
+        float Ei[4];
+        u32 ecd[4];
+        float lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float dqb = sqrtf(nqr[i]);
+          const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
+          use[i] = use[i] && !(dqb * (1.f - 2.f * eps) - Rb > thi);
+          Ei[i] = EXACT ? 0.f : a.c1 * dqb * (1.f + eps) * Rb + a.c2 * (nqr[i] + Rb * Rb);
+          ecd[i] = (!EXACT && a.ecode) ? (ecode_encode(Ei[i], ebase) << KEY_ROW_BITS) : 0u;
+          if (use[i]) {
+            lo = fminf(lo, dqb * (1.f - 2.f * eps) - thi);
+            hi = fmaxf(hi, dqb * (1.f + 2.f * eps) + thi);
+          }
+        }
+        lo = wave_min(lo);
+        hi = wave_max(hi);
+        if (!(lo <= hi)) continue;
+        lo = lo > 0.f ? lo * (1.f - eps) : -INFINITY;
+        hi = hi * (1.f + eps);
+
+        const int nch = (s1 - s0 + 63) >> 6;
+        for (int cg = 0; cg < nch; cg += 64) {
+          const int c = cg + lane;
+          const float cmin = c < nch ? a.R.pdist[s0 + 64 * c] : INFINITY;
+          float nmin = __shfl_down(cmin, 1);
+          if (lane == 63) nmin = (c + 1 < nch) ? a.R.pdist[s0 + 64 * (c + 1)] : INFINITY;
+          const bool needc = (c < nch) && (cmin <= hi) && (nmin >= lo);
+          const u64 m = wave_ballot(needc);
+          if (m == 0) {
+            if (__shfl(cmin, 0) > hi) break;
+            continue;
+          }
+          const int pbeg = s0 + 64 * (cg + __builtin_ctzll(m));
+          const int pend = min(s1, s0 + 64 * (cg + 64 - __builtin_clzll(m)));
+          for (int p0 = pbeg; p0 < pend; p0 += 16) {
+            const int p = p0 + c16;
+            const bool valid = p < s1;
+            const int pp = valid ? p : p0;
+            f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+            if constexpr (SPLIT) {
+              const __bf16 *rr = (const __bf16 *)a.rc + (int64_t)pp * 2 * dk + 8 * g;
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                if (s < nst) {
+                  const bf16x8_t bh = *(const bf16x8_t *)(rr + 32 * s);
+                  const bf16x8_t bl = *(const bf16x8_t *)(rr + dk + 32 * s);
+                  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(avl[s], bh, acc, 0, 0, 0);
+                  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], bl, acc, 0, 0, 0);
+                  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], bh, acc, 0, 0, 0);
+                }
+            } else if constexpr (BF16) {
+              const __bf16 *rr = (const __bf16 *)a.rc + (int64_t)pp * dk + 8 * g;
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                if (s < nst) {
+                  const bf16x8_t bv = *(const bf16x8_t *)(rr + 32 * s);
+                  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], bv, acc, 0, 0, 0);
+                }
+            } else {
+              const float *rr = (const float *)a.rc + (int64_t)pp * dk + 8 * g;
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                if (s < nst) {
+                  const f32x4_t b0 = *(const f32x4_t *)(rr + 32 * s);
+                  const f32x4_t b1 = *(const f32x4_t *)(rr + 32 * s + 4);
+#pragma unroll
+                  for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][j], b0[j], acc, 0, 0, 0);
+#pragma unroll
+                  for (int j = 0; j < 4; ++j)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][4 + j], b1[j], acc, 0, 0, 0);
+                }
+            }
+            const float nrv = a.nr[pp];
+            const u32 row = EXACT ? (u32)a.R.perm[pp] : (u32)pp;
+            bool pass[4];
+            u64 key[4];
+            bool anyp = false;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const float sf = fmaxf((nqr[i] + nrv) - 2.f * acc[i], 0.f);
+              if (EXACT) {
+                key[i] = make_key(sf, row);
+                pass[i] = valid && use[i] && key[i] <= thr[i];
+              } else {
+                key[i] = make_key(sf + Ei[i], row | ecd[i]);
+                pass[i] = valid && use[i] && (sf - Ei[i]) <= key_dist(thr[i]);
+              }
+              anyp |= pass[i];
+              evals += __popcll(wave_ballot(valid && use[i]));
+            }
+            if (wave_ballot(anyp) != 0) {
+              const u32 below = (1u << c16) - 1u;
+#pragma unroll
+              for (int i = 0; i < 4; ++i) {
+                const u64 mk = wave_ballot(pass[i]);
+                const u32 gm = (u32)(mk >> (16 * g)) & 0xFFFFu;
+                if (pass[i]) bufs[(4 * g + i) * MF_BC + cnt[i] + __popc(gm & below)] = key[i];
+                cnt[i] += __popc(gm);
+                // every entry ever appended for this row has E <= emax: the
+                // cutoff and the overflow test below rest on that
+                if (gm) emax[i] = fmaxf(emax[i], Ei[i]);
+                npass += __popcll(mk);
+              }
+              const bool hot = cnt[0] >= MF_WM || cnt[1] >= MF_WM || cnt[2] >= MF_WM || cnt[3] >= MF_WM;
+              if (wave_ballot(hot) != 0) {
+                __syncthreads();
+                for (int r = 0; r < MF_TQ; ++r) {
+                  const int gi = r >> 2, ii = r & 3;
+                  int mine = cnt[0];
+#pragma unroll
+                  for (int j = 1; j < 4; ++j)
+                    if (j == ii) mine = cnt[j];
+                  if (__shfl(mine, 16 * gi) >= MF_WM) merge(r);
+                }
+              }
             }
           }
         }
@@ -263,18 +394,27 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
   for (int r = 0; r < nq; ++r) {
     const int gi = r >> 2, ii = r & 3;
     u64 t = thr[0];
-    float em = emax[0];
+    float em = emax[0], ev = evl[0];
 #pragma unroll
     for (int j = 1; j < 4; ++j)
       if (j == ii) {
         t = thr[j];
         em = emax[j];
+        ev = evl[j];
       }
     const float Tr = __shfl(key_dist(t), 16 * gi);
     const float Er = __shfl(em, 16 * gi);
+    const float Vr = __shfl(ev, 16 * gi);
     const float cutoff = EXACT ? Tr : (Tr + 2.f * Er) * (1.f + 2.4e-7f);
     emit_query<KCAP, EXACT>(a, lists + r * KCAP, tile.start + r, A, cutoff,
-                            a.Q.px + (int64_t)(tile.start + r) * dpQ);
+                            a.Q.px + (int64_t)(tile.start + r) * dpQ, Tr, Vr);
+  }
+  if (lane == 0 && a.tile_prof) {
+    u64 *tp = a.tile_prof + 4 * (size_t)blockIdx.x;
+    tp[0] = clk0;
+    tp[1] = wall_clock64();
+    tp[2] = (u64)np;
+    tp[3] = nvisit;
   }
   if (lane == 0) {
     atomicAdd(a.n_evals, evals);
@@ -284,32 +424,31 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
 }
 
 inline size_t mfma_lds(int kcap) {
-  return sizeof(u64) * ((size_t)MF_TQ * kcap + (size_t)MF_TQ * MF_BC + (size_t)kcap) + 16 * sizeof(float);
+  return sizeof(u64) * ((size_t)MF_TQ * kcap + (size_t)MF_TQ * MF_BC + (size_t)kcap) + (16 + 256) * sizeof(float);
 }
 
-template <bool BF16>
+template <int PREC>
 hipError_t launch_mfma_any(const JoinArgs &a, int64_t grid, hipStream_t s) {
   const size_t lds = mfma_lds(a.kcap);
-  const bool big = a.dk > 128;
+  const bool big = a.dk > 128;  // dk = roundup(d, 32)
 #define KJ_LAUNCH(KC, EX, DKV) \
-  hipLaunchKernelGGL((k_join_mfma<KC, EX, BF16, DKV>), dim3((unsigned)grid), dim3(64), lds, s, a)
-#define KJ_DK(KC, EX)     \
-  if (big)                \
+  hipLaunchKernelGGL((k_join_mfma<KC, EX, PREC, DKV>), dim3((unsigned)grid), dim3(64), lds, s, a)
+#define KJ_DK(KC, EX)       \
+  if (big)                  \
     KJ_LAUNCH(KC, EX, 256); \
-  else                    \
+  else                      \
     KJ_LAUNCH(KC, EX, 128);
-#define KJ_EX(KC)                               \
-  if (a.exact && !BF16) {                       \
-    if constexpr (!BF16) { KJ_DK(KC, true) }    \
-  } else {                                      \
-    KJ_DK(KC, false)                            \
+#define KJ_EX(KC)                                  \
+  if (a.exact && PREC == 0) {                      \
+    if constexpr (PREC == 0) { KJ_DK(KC, true) }   \
+  } else {                                         \
+    KJ_DK(KC, false)                               \
   }
   switch (a.kcap) {
     case 64: KJ_EX(64) break;
     case 128: KJ_EX(128) break;
     case 256: KJ_EX(256) break;
     case 512: KJ_EX(512) break;
-    case 1024: KJ_EX(1024) break;
     default: return hipErrorInvalidValue;
   }
 #undef KJ_EX

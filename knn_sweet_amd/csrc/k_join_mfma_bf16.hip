@@ -1,7 +1,8 @@
-// k_join_mfma_bf16.hip -- BF16 MFMA filter join kernels (v_mfma_f32_16x16x32_bf16).
+// k_join_mfma_bf16.hip -- BF16 MFMA filter join kernels (v_mfma_f32_16x16x32_bf16):
+// the compensated three-product form (hi/lo operands) and the single product.
 #include "k_join_mfma.h"
 
-hipError_t launch_join_mfma_bf16(const JoinArgs &a, int64_t max_tiles, hipStream_t s) {
+hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles, hipStream_t s) {
   if (max_tiles <= 0) return hipSuccess;
-  return launch_mfma_any<true>(a, max_tiles, s);
+  return terms == 3 ? launch_mfma_any<3>(a, max_tiles, s) : launch_mfma_any<1>(a, max_tiles, s);
 }

@@ -5,5 +5,5 @@ size_t join_mfma_lds_bytes(int kcap) { return mfma_lds(kcap); }
 
 hipError_t launch_join_mfma_f32(const JoinArgs &a, int64_t max_tiles, hipStream_t s) {
   if (max_tiles <= 0) return hipSuccess;
-  return launch_mfma_any<false>(a, max_tiles, s);
+  return launch_mfma_any<0>(a, max_tiles, s);
 }

@@ -22,6 +22,24 @@ __device__ __forceinline__ u64 make_key(float s, u32 row) {
 __device__ __forceinline__ float key_dist(u64 k) { return __uint_as_float((u32)(k >> 32)); }
 __device__ __forceinline__ u32 key_row(u64 k) { return (u32)k; }
 
+// Filter keys of the MFMA kernel carry, above a 26-bit row field, a 6-bit
+// code c of the entry's error bound: E <= 2^(c + ebase), c = 63 unbounded.
+// A list entry's lower bound is then L = key distance - 2 * decode(c).
+#define KEY_ROW_BITS 26
+#define KEY_ROW_MASK ((1u << KEY_ROW_BITS) - 1u)
+__device__ __forceinline__ u32 ecode_encode(float E, int ebase) {
+  if (!(E > 0.f)) return 0u;
+  int e = ilogbf(E) + 1 - ebase;  // 2^(e + ebase) > E
+  if (e < 0) e = 0;
+  return e > 62 ? 63u : (u32)e;
+}
+__device__ __forceinline__ float ecode_decode(u32 c, int ebase) {
+  return c >= 63u ? INFINITY : ldexpf(1.f, (int)c + ebase);
+}
+__device__ __forceinline__ float key_lower(u64 key, int ebase) {
+  return key_dist(key) - 2.f * ecode_decode((u32)key >> KEY_ROW_BITS, ebase);
+}
+
 __device__ __forceinline__ u32 lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
 // number of set bits of `mask` below this lane

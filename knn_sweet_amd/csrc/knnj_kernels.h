@@ -7,6 +7,7 @@
 
 // k_index.hip
 void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags, hipStream_t s);
+int rep_score_blocks(int m);  // partial sums per trial of launch_rep_select
 void launch_rep_select(const float *X, int64_t ld, int d, int dpad, const int32_t *cand, int m, int trials,
                        double *partial, int nblk, float *reps, int32_t *rep_rows, hipStream_t s);
 // nearest representative (MFMA bf16 for 16 <= d <= 256 when rb/rn are given, VALU otherwise)
@@ -23,7 +24,8 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
                                 const int32_t *cl, const int32_t *cnt, int32_t *off, int32_t *fill,
                                 int32_t *perm, const float *d2r, float *pdist, u64 *gkeys, float *px,
                                 hipStream_t s);
-void launch_make_tiles(const int32_t *off, int m, int tq, Tile *tiles, int32_t *ntiles, hipStream_t s);
+void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, Tile *tiles, int32_t *ntiles,
+                       hipStream_t s);
 
 // k_bounds.hip
 void launch_bounds(const SideIndex &Qi, const SideIndex &Ri, int d, int k, double *theta, int32_t *plist,
@@ -53,8 +55,15 @@ struct JoinArgs {
   // MFMA paths: reference rows centred on their cluster representative
   const void *rc;        // [nR][dk] bf16 or fp32
   const float *nr;       // [nR] squared norm of the centred row (binary32)
-  int dk;                // padded width of rc
+  int dk;                // padded width of rc and reps32: roundup(d, 32)
+  const float *reps32;   // [mR][dk] reference representatives, FP32, zero padded
+  const float *rn32;     // [mR] their squared norms
   float c1, c2;          // error-bound coefficients (see k_join_mfma.h)
+  // optional per-tile record {start clock, end clock, clusters listed, clusters visited}
+  u64 *tile_prof;
+  float *ovf_diag;       // optional: first 64 overflow records {cutoff, k-th key, last key, first key}
+  int ecode;             // MFMA filter keys carry per-entry error codes (rows < 2^26)
+  int ebase;             // exponent base of those codes
 };
 
 // Picks the wave tile size for a candidate capacity; returns the dynamic LDS bytes.
@@ -63,9 +72,12 @@ size_t join_lds_bytes(int tq, int kcap);
 hipError_t launch_join(const JoinArgs &a, int tq, int64_t max_tiles, hipStream_t s);
 
 // k_center.hip
-void launch_center(const SideIndex &Ri, int d, int dk, bool bf16, void *rc, float *nr, hipStream_t s);
+// planes: 0 fp32 rows, 1 bf16 rows, 2 bf16 hi/lo rows
+void launch_center(const SideIndex &Ri, int d, int dk, int planes, void *rc, float *nr, float *reps32, float *rn32,
+                   hipStream_t s);
 
 // k_join_mfma_*.hip
 hipError_t launch_join_mfma_f32(const JoinArgs &a, int64_t max_tiles, hipStream_t s);
-hipError_t launch_join_mfma_bf16(const JoinArgs &a, int64_t max_tiles, hipStream_t s);
+// terms: 1 single bf16 product, 3 compensated hi/lo products
+hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles, hipStream_t s);
 size_t join_mfma_lds_bytes(int kcap);

@@ -87,12 +87,18 @@ __device__ __forceinline__ int count_below(const u64 *a, int n, int p2, u64 x) {
   return c;
 }
 
+struct NoDrop {
+  static constexpr bool tracks = false;
+  __device__ float operator()(u64) const { return INFINITY; }
+};
+
 // Merges the unsorted candidate buffer buf[0..cnt) (cnt <= BCAP) into the
 // sorted list[0..KCAP), keeping the KCAP smallest keys.  Keys in list and
 // buf are distinct except for KEY_MAX padding.  tmp: KCAP scratch keys.  The
-// caller's workgroup is this single wave.
-template <int KCAP, int BCAP = KNNJ_BCAP>
-__device__ __forceinline__ void wave_merge(u64 *list, u64 *buf, int cnt, u64 *tmp) {
+// caller's workgroup is this single wave.  Returns the wave-wide minimum of
+// drop(key) over the real keys that fell off the end (INFINITY if none).
+template <int KCAP, int BCAP = KNNJ_BCAP, class Drop = NoDrop>
+__device__ __forceinline__ float wave_merge(u64 *list, u64 *buf, int cnt, u64 *tmp, Drop drop = Drop()) {
   constexpr int EB = BCAP / 64;
   const int lane = threadIdx.x & 63;
   u64 v[EB];
@@ -102,22 +108,33 @@ __device__ __forceinline__ void wave_merge(u64 *list, u64 *buf, int cnt, u64 *tm
 #pragma unroll
   for (int u = 0; u < EB; ++u) buf[lane + 64 * u] = v[u];
   __syncthreads();
+  float dmin = INFINITY;
 #pragma unroll
   for (int u = 0; u < KCAP / 64; ++u) {
     const int j = lane + 64 * u;
     const u64 x = list[j];
     const int pos = j + count_below<true>(buf, BCAP, BCAP, x);
-    if (pos < KCAP) tmp[pos] = x;
+    if (pos < KCAP)
+      tmp[pos] = x;
+    else if (x != KEY_MAX)
+      dmin = fminf(dmin, drop(x));
   }
 #pragma unroll
   for (int u = 0; u < EB; ++u) {
     const int j = lane + 64 * u;
     const u64 x = v[u];
     const int pos = j + count_below<false>(list, KCAP, KCAP, x);
-    if (pos < KCAP) tmp[pos] = x;
+    if (pos < KCAP)
+      tmp[pos] = x;
+    else if (x != KEY_MAX)
+      dmin = fminf(dmin, drop(x));
   }
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < KCAP / 64; ++u) list[lane + 64 * u] = tmp[lane + 64 * u];
   __syncthreads();
+  if constexpr (Drop::tracks)
+    return wave_min(dmin);
+  else
+    return INFINITY;
 }

@@ -148,11 +148,12 @@ def test_mfma_fp32_real(ctx, d, k):
     assert_matches_oracle(R, Q, k, idx, dist)
 
 
-@pytest.mark.parametrize("d,k", [(64, 10), (96, 50), (128, 100), (128, 10)])
-def test_mfma_bf16_filter(ctx, d, k):
+@pytest.mark.parametrize("prec", [kj.PREC_BF16, kj.PREC_BF16X1])
+@pytest.mark.parametrize("d,k", [(64, 10), (96, 50), (128, 100), (128, 10), (250, 30)])
+def test_mfma_bf16_filter(ctx, d, k, prec):
     R, Q = datagen.mixture128(9000, 300, seed=d + k, d=d)
-    idx, dist, st = ctx.join(R, Q, k, precision=kj.PREC_BF16)
-    assert st["kernel_used"] == kj.KERNEL_MFMA and st["precision_used"] == kj.PREC_BF16
+    idx, dist, st = ctx.join(R, Q, k, precision=prec)
+    assert st["kernel_used"] == kj.KERNEL_MFMA and st["precision_used"] == prec
     assert_matches_oracle(R, Q, k, idx, dist)
 
 
