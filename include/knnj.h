@@ -88,7 +88,8 @@ typedef struct {
   int32_t kernel;      /* knnj_kernel: distance-tile implementation, 0 = auto */
   int32_t refine_iters;/* Lloyd steps moving each representative to its members' mean
                           (0 = default 1, negative = keep the sampled points) */
-  int32_t reserved[4];
+  int32_t xcd_tiles;   /* 1: place all tiles of a query cluster on one XCD (shared L2) */
+  int32_t reserved[3];
 } knnj_opts;
 
 /* Device-side timings (hipEvents on the context's stream) and counters. */

@@ -47,7 +47,7 @@ class Opts(ctypes.Structure):
                 ("out_squared", ctypes.c_int32), ("out_aos", ctypes.c_int32),
                 ("brute_force", ctypes.c_int32), ("tile_q", ctypes.c_int32),
                 ("kernel", ctypes.c_int32), ("refine_iters", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("xcd_tiles", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class Stats(ctypes.Structure):
@@ -130,9 +130,9 @@ def _matrix(x):
 
 
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
-              brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0) -> Opts:
+              brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0, xcd_tiles=False) -> Opts:
     return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
-                tile_q, kernel, refine_iters)
+                tile_q, kernel, refine_iters, int(xcd_tiles))
 
 
 class Context:

@@ -465,7 +465,7 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   if (kcap < k) kcap = KNNJ_MAX_K;  // only the VALU kernel holds k > 512
   const bool use_mfma = mfma && kcap <= 512;
   const int tq = use_mfma ? 16 : (o.tile_q ? std::min(o.tile_q, join_tile_q(kcap)) : join_tile_q(kcap));
-  const int64_t max_tiles = Q.n / tq + m_q + 1;
+  const int64_t max_tiles = 8 * (Q.n / tq + m_q + 1);  // XCD-aware slots, empty ones included
   Tile *tiles, *ovf_tiles, *ovf2_tiles;
   int32_t *ntiles, *counters;
   u64 *n_evals;
@@ -477,7 +477,10 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   ntiles = counters;
   HIPCHK(hipMemsetAsync(counters, 0, 8 * sizeof(int32_t), s));
   HIPCHK(hipMemsetAsync(n_evals, 0, 4 * sizeof(u64), s));
-  launch_make_tiles(Qi.off, m_q, pcnt, tq, tiles, ntiles, s);
+  launch_make_tiles(Qi.off, m_q, pcnt, tq, tiles, ntiles, (int)max_tiles, o.xcd_tiles ? 8 : 1, s);
+  int32_t nslots = 0;
+  HIPCHK(hipMemcpyAsync(&nslots, ntiles, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
 
   // outputs
   int32_t *oidx = out_idx;
@@ -546,12 +549,12 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   if (use_mfma) {
     a.exact = exact ? 1 : 0;
     if (bf16)
-      HIPCHK(launch_join_mfma_bf16(a, bf16x1 ? 1 : 3, max_tiles, s));
+      HIPCHK(launch_join_mfma_bf16(a, bf16x1 ? 1 : 3, nslots, s));
     else
-      HIPCHK(launch_join_mfma_f32(a, max_tiles, s));
+      HIPCHK(launch_join_mfma_f32(a, nslots, s));
   } else {
     a.exact = (mfma ? exact_valu : exact) ? 1 : 0;
-    HIPCHK(launch_join(a, tq, max_tiles, s));
+    HIPCHK(launch_join(a, tq, nslots, s));
   }
   HIPCHK(hipEventRecord(c->ev[4], s));
 
@@ -564,25 +567,34 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
     // diagnostic: distribution of tile durations (wall clock ticks) and cluster counts
     std::vector<u64> tp((size_t)hc[0] * 4);
     HIPCHK(hipMemcpy(tp.data(), a.tile_prof, tp.size() * sizeof(u64), hipMemcpyDeviceToHost));
-    std::vector<double> dur(hc[0]);
+    std::vector<double> dur;
     double listed = 0, visited = 0;
     u64 t0 = ~0ull, t1 = 0;
     for (int i = 0; i < hc[0]; ++i) {
-      dur[i] = (double)(tp[4 * i + 1] - tp[4 * i]);
-      listed += (double)tp[4 * i + 2];
-      visited += (double)tp[4 * i + 3];
+      if (tp[4 * i + 1] == 0) continue;  // empty slot
+      dur.push_back((double)(tp[4 * i + 1] - tp[4 * i]));
+      listed += (double)(tp[4 * i + 3] & 0xFFFFFFFFull);
+      visited += (double)(tp[4 * i + 3] >> 32);
       t0 = std::min(t0, tp[4 * i]);
       t1 = std::max(t1, tp[4 * i + 1]);
+    }
+    if (const char *dump = std::getenv("KNNJ_TILE_DUMP")) {
+      if (FILE *f = fopen(dump, "wb")) {
+        fwrite(tp.data(), sizeof(u64), tp.size(), f);
+        fclose(f);
+      }
     }
     std::vector<double> sd = dur;
     std::sort(sd.begin(), sd.end());
     double sum = 0;
     for (double v : sd) sum += v;
+    const int nt = (int)sd.size();
+    if (nt > 0)
     fprintf(stderr,
             "[knnj tile profile] tiles=%d span=%.0f ticks sum=%.0f mean=%.1f p50=%.0f p99=%.0f max=%.0f "
             "listed/tile=%.1f visited/tile=%.1f\n",
-            hc[0], (double)(t1 - t0), sum, sum / hc[0], sd[hc[0] / 2], sd[(size_t)(0.99 * (hc[0] - 1))],
-            sd.back(), listed / hc[0], visited / hc[0]);
+            nt, (double)(t1 - t0), sum, sum / nt, sd[nt / 2], sd[(size_t)(0.99 * (nt - 1))], sd.back(),
+            listed / nt, visited / nt);
     if (hc[1] > 0) {
       float dg[256];
       HIPCHK(hipMemcpy(dg, a.ovf_diag, sizeof(dg), hipMemcpyDeviceToHost));

@@ -520,14 +520,22 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
 
 // ---------------------------------------------------------------------------
 // Query tiles: cluster A of the query side is cut into ceil(|A| / tq) tiles
-// of consecutive permuted queries.  Clusters are emitted in descending order
-// of their surviving-pair count (the work estimate), so the longest tiles
-// start first and the short ones fill the tail.  One workgroup of 1024.
+// of consecutive permuted queries, clusters taken in descending order of
+// their surviving-pair count (the work estimate) so the longest tiles start
+// first.  With nxcd = 8 the placement is XCD-aware: workgroup b runs on XCD
+// b % 8, so tile slot 8 i + x is the i-th tile of XCD x; clusters are dealt
+// to the XCDs round-robin and all tiles of one cluster go to the same XCD,
+// where they share the reference rows in that XCD's L2 (at the cost of
+// balance when a few clusters are heavy).  nxcd = 1 is the plain order.
+// Slots past the end of a sequence hold empty tiles (count 0); *nslots
+// receives nxcd x the longest sequence.  One workgroup of 1024.
 __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__ off, int m,
                                                      const int32_t *__restrict__ pcnt, int tq, Tile *tiles,
-                                                     int32_t *ntiles) {
+                                                     int32_t *nslots, int max_slots, int nxcd) {
   __shared__ u64 ord[KNNJ_MAX_REPS];
   __shared__ int32_t part[1024];
+  __shared__ int32_t seqlen[8];
+  const int NXCD = nxcd;
   int P = 1;
   while (P < m) P <<= 1;
   for (int c = threadIdx.x; c < P; c += blockDim.x)
@@ -549,13 +557,22 @@ __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__
       __syncthreads();
     }
   }
-  const int per = (m + 1023) / 1024;
+  // regroup: position q of the XCD-major order is sorted cluster
+  // j = x + NXCD * t with x = XCD, t = rank within the XCD
+  const int per_x = (m + NXCD - 1) / NXCD;
+  auto sorted_at = [&](int q) -> int {  // -1 if q maps past m
+    const int x = q / per_x, t = q % per_x;
+    const int j = x + NXCD * t;
+    return j < m ? (int)(u32)ord[j] : -1;
+  };
+  const int Mq = per_x * NXCD;
+  const int per = (Mq + 1023) / 1024;
   const int b = threadIdx.x * per;
   int32_t s = 0;
-  for (int j = 0; j < per; ++j)
-    if (b + j < m) {
-      const int c = (int)(u32)ord[b + j];
-      s += (off[c + 1] - off[c] + tq - 1) / tq;
+  for (int jq = 0; jq < per; ++jq)
+    if (b + jq < Mq) {
+      const int c = sorted_at(b + jq);
+      if (c >= 0) s += (off[c + 1] - off[c] + tq - 1) / tq;
     }
   part[threadIdx.x] = s;
   __syncthreads();
@@ -565,26 +582,66 @@ __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__
     part[threadIdx.x] += v;
     __syncthreads();
   }
+  // exclusive prefix of the whole XCD-major sequence at q = x * per_x, via a
+  // second pass: a thread that owns the first position of XCD x records it
   int32_t run = part[threadIdx.x] - s;
-  for (int j = 0; j < per; ++j) {
-    if (b + j >= m) break;
-    const int c = (int)(u32)ord[b + j];
+  for (int jq = 0; jq < per; ++jq) {
+    const int q = b + jq;
+    if (q >= Mq) break;
+    if (q % per_x == 0) seqlen[q / per_x] = run;  // start of XCD q / per_x
+    const int c = sorted_at(q);
+    if (c >= 0) run += (off[c + 1] - off[c] + tq - 1) / tq;
+  }
+  __syncthreads();
+  const int total = part[1023];
+  __shared__ int32_t seqstart[9];
+  if (threadIdx.x == 0) {
+    for (int x = 0; x < NXCD; ++x) seqstart[x] = seqlen[x];
+    seqstart[NXCD] = total;
+    int L = 0;
+    for (int x = 0; x < NXCD; ++x) L = max(L, seqstart[x + 1] - seqstart[x]);
+    *nslots = min(NXCD * L, max_slots);
+  }
+  __syncthreads();
+  int Lmax = 0;
+  for (int x = 0; x < NXCD; ++x) Lmax = max(Lmax, seqstart[x + 1] - seqstart[x]);
+  // empty slots first, then the tiles
+  for (int e = threadIdx.x; e < NXCD * Lmax && e < max_slots; e += blockDim.x) {
+    Tile t;
+    t.start = 0;
+    t.count = 0;
+    t.cluster = 0;
+    t.pad = 0;
+    tiles[e] = t;
+  }
+  __syncthreads();
+  run = part[threadIdx.x] - s;
+  for (int jq = 0; jq < per; ++jq) {
+    const int q = b + jq;
+    if (q >= Mq) break;
+    const int c = sorted_at(q);
+    if (c < 0) continue;
+    const int x = q / per_x;
     const int a0 = off[c], na = off[c + 1] - a0;
-    for (int q = 0; q < na; q += tq) {
-      Tile t;
-      t.start = a0 + q;
-      t.count = min(tq, na - q);
-      t.cluster = c;
-      t.pad = 0;
-      tiles[run++] = t;
+    for (int qq = 0; qq < na; qq += tq) {
+      const int i = run - seqstart[x];  // rank within XCD x
+      const int slot = NXCD * i + x;
+      if (slot < max_slots) {
+        Tile t;
+        t.start = a0 + qq;
+        t.count = min(tq, na - qq);
+        t.cluster = c;
+        t.pad = 0;
+        tiles[slot] = t;
+      }
+      ++run;
     }
   }
-  if (threadIdx.x == 1023) *ntiles = part[1023];
 }
 
-void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, Tile *tiles, int32_t *ntiles,
-                       hipStream_t s) {
-  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, s, off, m, pcnt, tq, tiles, ntiles);
+void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, Tile *tiles, int32_t *nslots,
+                       int max_slots, int nxcd, hipStream_t s) {
+  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, s, off, m, pcnt, tq, tiles, nslots, max_slots, nxcd);
 }
 
 // Members of every cluster in ascending row order (counts already in cnt).

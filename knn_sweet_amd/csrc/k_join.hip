@@ -39,6 +39,7 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
   const int lane = threadIdx.x;
   if ((int)blockIdx.x >= *a.ntiles) return;
   const Tile tile = a.tiles[blockIdx.x];
+  if (tile.count <= 0) return;
   const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
   u64 nvisit = 0;
   const int A = tile.cluster;
@@ -240,8 +241,8 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
     u64 *tp = a.tile_prof + 4 * (size_t)blockIdx.x;
     tp[0] = clk0;
     tp[1] = wall_clock64();
-    tp[2] = (u64)np;
-    tp[3] = nvisit;
+    tp[2] = 0;
+    tp[3] = (nvisit << 32) | (u64)np;
   }
   if (lane == 0) {
     atomicAdd(a.n_evals, evals);

@@ -67,7 +67,22 @@ __device__ __forceinline__ D make_drop(int eb) {
 
 #define MF_TQ 16
 #define MF_BC 64  // per-query candidate buffer
-#define MF_WM 48  // merge watermark: a 16-column block adds at most 16
+
+// B-operand registers of one 16-column block and 32-dimension step
+template <int PREC>
+struct BOp;
+template <>
+struct BOp<0> {
+  f32x4_t f0, f1;
+};
+template <>
+struct BOp<1> {
+  bf16x8_t h;
+};
+template <>
+struct BOp<3> {
+  bf16x8_t h, l;
+};
 
 template <int KCAP, bool EXACT, int PREC, int DK>
 __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
@@ -75,15 +90,20 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
   constexpr bool BF16 = PREC != 0;
   constexpr bool SPLIT = PREC == 3;
   constexpr int NS = DK / 32;  // 32-dimension steps
+  // 16-column blocks loaded together (B registers: NB * NS * 4 or 8 VGPRs)
+  constexpr int NB = (DK <= 128) ? 4 : 2;
+  typedef BOp<PREC> BT;
   extern __shared__ u64 smem[];
   u64 *lists = smem;                        // [16][KCAP]
   u64 *bufs = lists + MF_TQ * KCAP;         // [16][MF_BC]
   u64 *tmp = bufs + MF_TQ * MF_BC;          // [KCAP]
   float *fsh = (float *)(tmp + KCAP);       // [16] row norms
   float *dqs = fsh + 16;                    // [16 rows][16 cols] screening distances
+  float *qsh = dqs + 256;                   // [16 rows][DK] the tile's query rows
   const int lane = threadIdx.x, g = lane >> 4, c16 = lane & 15;
   if ((int)blockIdx.x >= *a.ntiles) return;
   const Tile tile = a.tiles[blockIdx.x];
+  if (tile.count <= 0) return;
   const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
   u64 nvisit = 0;
   const int A = tile.cluster, nq = tile.count;
@@ -113,16 +133,30 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
   // this lane's slice of query row c16: dimensions 32 s + 8 g + j, in FP32
   const bool alive = c16 < nq;
   const float *qa = a.Q.px + (int64_t)(tile.start + (alive ? c16 : 0)) * dpQ;
-  float qf[NS][8];
-  float qn = 0.f;
+  // q_slice(s): this lane's 8 dimensions 32 s + 8 g + j of query row c16
+  // (re-read from the row when needed rather than held in registers)
+  for (int e = lane; e < MF_TQ * DK; e += 64) {
+    const int r = e / DK, dim = e % DK;
+    qsh[e] = (r < nq && dim < d) ? a.Q.px[(int64_t)(tile.start + r) * dpQ + dim] : 0.f;
+  }
+  __syncthreads();
+  const float *qrow_sh = qsh + c16 * DK + 8 * g;
+  auto q_slice = [&](int s, float (&v)[8]) {
+    const f32x4_t v0 = *(const f32x4_t *)(qrow_sh + 32 * s);
+    const f32x4_t v1 = *(const f32x4_t *)(qrow_sh + 32 * s + 4);
 #pragma unroll
-  for (int s = 0; s < NS; ++s)
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int dim = 32 * s + 8 * g + j;
-      qf[s][j] = (alive && s < nst && dim < d) ? qa[dim] : 0.f;
-      qn = __builtin_fmaf(qf[s][j], qf[s][j], qn);
+    for (int j = 0; j < 4; ++j) {
+      v[j] = v0[j];
+      v[4 + j] = v1[j];
     }
+  };
+  float qn = 0.f;
+  for (int s = 0; s < nst; ++s) {
+    float v[8];
+    q_slice(s, v);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) qn = __builtin_fmaf(v[j], v[j], qn);
+  }
   qn += __shfl_xor(qn, 16);
   qn += __shfl_xor(qn, 32);
   if (g == 0) fsh[c16] = qn;
@@ -168,21 +202,24 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
     const int pj = pb + c16;
     const bool lv = pj < np;
     const int bj = lv ? pl[pj] : pl[pb];
+    // this lane's cluster metadata, read once here and broadcast per visit
+    const int s0j = a.R.off[bj], s1j = a.R.off[bj + 1];
+    const float Rbj = (float)(a.R.rad[bj] * (1.0 + 1e-6));
     {
       f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
       const float *cr = a.reps32 + (int64_t)bj * dk + 8 * g;
+      for (int s = 0; s < nst; ++s) {
+        float qv[8];
+        q_slice(s, qv);
+        const f32x4_t c0 = *(const f32x4_t *)(cr + 32 * s);
+        const f32x4_t c1 = *(const f32x4_t *)(cr + 32 * s + 4);
 #pragma unroll
-      for (int s = 0; s < NS; ++s)
-        if (s < nst) {
-          const f32x4_t c0 = *(const f32x4_t *)(cr + 32 * s);
-          const f32x4_t c1 = *(const f32x4_t *)(cr + 32 * s + 4);
+        for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[j], c0[j], acc, 0, 0, 0);
 #pragma unroll
-          for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qf[s][j], c0[j], acc, 0, 0, 0);
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qf[s][4 + j], c1[j], acc, 0, 0, 0);
-        }
+        for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[4 + j], c1[j], acc, 0, 0, 0);
+      }
       const float cn = a.rn32[bj];
-      const float rbj = (float)(a.R.rad[bj] * (1.0 + 1e-6));
+      const float rbj = Rbj;
       bool need = false;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -201,8 +238,13 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
         const int jl = __builtin_ctzll(pending);
         pending &= pending - 1;
         const int b = __builtin_amdgcn_readlane(bj, jl);
-        const int s0 = a.R.off[b], s1 = a.R.off[b + 1];
-        const float Rb = (float)(a.R.rad[b] * (1.0 + 1e-6));
+        const int s0 = __builtin_amdgcn_readlane(s0j, jl), s1 = __builtin_amdgcn_readlane(s1j, jl);
+        const float Rb = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Rbj), jl));
+        // first 64 chunk-start distances of the cluster, requested now so the
+        // load overlaps the A-operand build below
+        const int nch = (s1 - s0 + 63) >> 6;
+        const float cmin0 = lane < nch ? a.R.pdist[s0 + 64 * lane] : INFINITY;
+        const float cnext0 = (lane == 63 && nch > 64) ? a.R.pdist[s0 + 64 * 64] : INFINITY;
         // re-check with the thresholds as they stand now
         bool use[4];
         bool anyl = false;
@@ -225,9 +267,11 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
           if (s < nst) {
             const f32x4_t c0 = *(const f32x4_t *)(rep + 32 * s);
             const f32x4_t c1 = *(const f32x4_t *)(rep + 32 * s + 4);
+            float qv[8];
+            q_slice(s, qv);
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
-              const float v = alive ? qf[s][j] - (j < 4 ? c0[j] : c1[j - 4]) : 0.f;
+              const float v = alive ? qv[j] - (j < 4 ? c0[j] : c1[j - 4]) : 0.f;
               part = __builtin_fmaf(v, v, part);
               if constexpr (BF16) {
                 const __bf16 h = (__bf16)v;
@@ -279,12 +323,12 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
         lo = lo > 0.f ? lo * (1.f - eps) : -INFINITY;
         hi = hi * (1.f + eps);
 
-        const int nch = (s1 - s0 + 63) >> 6;
         for (int cg = 0; cg < nch; cg += 64) {
           const int c = cg + lane;
-          const float cmin = c < nch ? a.R.pdist[s0 + 64 * c] : INFINITY;
+          const float cmin = cg == 0 ? cmin0 : (c < nch ? a.R.pdist[s0 + 64 * c] : INFINITY);
           float nmin = __shfl_down(cmin, 1);
-          if (lane == 63) nmin = (c + 1 < nch) ? a.R.pdist[s0 + 64 * (c + 1)] : INFINITY;
+          if (lane == 63)
+            nmin = cg == 0 ? cnext0 : ((c + 1 < nch) ? a.R.pdist[s0 + 64 * (c + 1)] : INFINITY);
           const bool needc = (c < nch) && (cmin <= hi) && (nmin >= lo);
           const u64 m = wave_ballot(needc);
           if (m == 0) {
@@ -293,88 +337,116 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
           }
           const int pbeg = s0 + 64 * (cg + __builtin_ctzll(m));
           const int pend = min(s1, s0 + 64 * (cg + 64 - __builtin_clzll(m)));
-          for (int p0 = pbeg; p0 < pend; p0 += 16) {
-            const int p = p0 + c16;
-            const bool valid = p < s1;
-            const int pp = valid ? p : p0;
-            f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-            if constexpr (SPLIT) {
-              const __bf16 *rr = (const __bf16 *)a.rc + (int64_t)pp * 2 * dk + 8 * g;
+          // B operands, norms and rows of NB 16-column blocks, loaded
+          // together so one memory latency covers NB blocks of MFMA work
+          struct Group {
+            BT bo[NB][NS];
+            float nrv[NB];
+            u32 rowv[NB];
+            bool validv[NB];
+          };
+          auto load_group = [&](int g0, Group &G) {
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+              const int p = g0 + 16 * nb + c16;
+              G.validv[nb] = p < pend;
+              const int pp = G.validv[nb] ? p : g0;
 #pragma unroll
               for (int s = 0; s < NS; ++s)
                 if (s < nst) {
-                  const bf16x8_t bh = *(const bf16x8_t *)(rr + 32 * s);
-                  const bf16x8_t bl = *(const bf16x8_t *)(rr + dk + 32 * s);
-                  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(avl[s], bh, acc, 0, 0, 0);
-                  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], bl, acc, 0, 0, 0);
-                  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], bh, acc, 0, 0, 0);
+                  if constexpr (SPLIT) {
+                    const __bf16 *rr = (const __bf16 *)a.rc + (int64_t)pp * 2 * dk + 8 * g + 32 * s;
+                    G.bo[nb][s].h = *(const bf16x8_t *)rr;
+                    G.bo[nb][s].l = *(const bf16x8_t *)(rr + dk);
+                  } else if constexpr (BF16) {
+                    G.bo[nb][s].h = *(const bf16x8_t *)((const __bf16 *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s);
+                  } else {
+                    const float *rr = (const float *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s;
+                    G.bo[nb][s].f0 = *(const f32x4_t *)rr;
+                    G.bo[nb][s].f1 = *(const f32x4_t *)(rr + 4);
+                  }
                 }
-            } else if constexpr (BF16) {
-              const __bf16 *rr = (const __bf16 *)a.rc + (int64_t)pp * dk + 8 * g;
-#pragma unroll
-              for (int s = 0; s < NS; ++s)
-                if (s < nst) {
-                  const bf16x8_t bv = *(const bf16x8_t *)(rr + 32 * s);
-                  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], bv, acc, 0, 0, 0);
-                }
-            } else {
-              const float *rr = (const float *)a.rc + (int64_t)pp * dk + 8 * g;
-#pragma unroll
-              for (int s = 0; s < NS; ++s)
-                if (s < nst) {
-                  const f32x4_t b0 = *(const f32x4_t *)(rr + 32 * s);
-                  const f32x4_t b1 = *(const f32x4_t *)(rr + 32 * s + 4);
-#pragma unroll
-                  for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][j], b0[j], acc, 0, 0, 0);
-#pragma unroll
-                  for (int j = 0; j < 4; ++j)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][4 + j], b1[j], acc, 0, 0, 0);
-                }
+              G.nrv[nb] = a.nr[pp];
+              G.rowv[nb] = EXACT ? (u32)a.R.perm[pp] : (u32)pp;
             }
-            const float nrv = a.nr[pp];
-            const u32 row = EXACT ? (u32)a.R.perm[pp] : (u32)pp;
-            bool pass[4];
-            u64 key[4];
-            bool anyp = false;
+          };
+          auto compute_group = [&](int g0, Group &G) {
+            // compute stage
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const float sf = fmaxf((nqr[i] + nrv) - 2.f * acc[i], 0.f);
-              if (EXACT) {
-                key[i] = make_key(sf, row);
-                pass[i] = valid && use[i] && key[i] <= thr[i];
-              } else {
-                key[i] = make_key(sf + Ei[i], row | ecd[i]);
-                pass[i] = valid && use[i] && (sf - Ei[i]) <= key_dist(thr[i]);
-              }
-              anyp |= pass[i];
-              evals += __popcll(wave_ballot(valid && use[i]));
-            }
-            if (wave_ballot(anyp) != 0) {
-              const u32 below = (1u << c16) - 1u;
+            for (int nb = 0; nb < NB; ++nb) {
+              if (g0 + 16 * nb >= pend) break;
+              const bool valid = G.validv[nb];
+              f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                if (s < nst) {
+                  if constexpr (SPLIT) {
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(avl[s], G.bo[nb][s].h, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].l, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].h, acc, 0, 0, 0);
+                  } else if constexpr (BF16) {
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].h, acc, 0, 0, 0);
+                  } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][j], G.bo[nb][s].f0[j], acc, 0, 0, 0);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][4 + j], G.bo[nb][s].f1[j], acc, 0, 0, 0);
+                  }
+                }
+              const float nrb = G.nrv[nb];
+              const u32 row = G.rowv[nb];
+              bool pass[4];
+              u64 key[4];
+              bool anyp = false;
 #pragma unroll
               for (int i = 0; i < 4; ++i) {
-                const u64 mk = wave_ballot(pass[i]);
-                const u32 gm = (u32)(mk >> (16 * g)) & 0xFFFFu;
-                if (pass[i]) bufs[(4 * g + i) * MF_BC + cnt[i] + __popc(gm & below)] = key[i];
-                cnt[i] += __popc(gm);
-                // every entry ever appended for this row has E <= emax: the
-                // cutoff and the overflow test below rest on that
-                if (gm) emax[i] = fmaxf(emax[i], Ei[i]);
-                npass += __popcll(mk);
+                const float sf = fmaxf((nqr[i] + nrb) - 2.f * acc[i], 0.f);
+                if (EXACT) {
+                  key[i] = make_key(sf, row);
+                  pass[i] = valid && use[i] && key[i] <= thr[i];
+                } else {
+                  key[i] = make_key(sf + Ei[i], row | ecd[i]);
+                  pass[i] = valid && use[i] && (sf - Ei[i]) <= key_dist(thr[i]);
+                }
+                anyp |= pass[i];
+                evals += __popcll(wave_ballot(valid && use[i]));
               }
-              const bool hot = cnt[0] >= MF_WM || cnt[1] >= MF_WM || cnt[2] >= MF_WM || cnt[3] >= MF_WM;
-              if (wave_ballot(hot) != 0) {
-                __syncthreads();
-                for (int r = 0; r < MF_TQ; ++r) {
-                  const int gi = r >> 2, ii = r & 3;
-                  int mine = cnt[0];
+              if (wave_ballot(anyp) != 0) {
+                const u32 below = (1u << c16) - 1u;
 #pragma unroll
-                  for (int j = 1; j < 4; ++j)
-                    if (j == ii) mine = cnt[j];
-                  if (__shfl(mine, 16 * gi) >= MF_WM) merge(r);
+                for (int i = 0; i < 4; ++i) {
+                  const u64 mk = wave_ballot(pass[i]);
+                  const u32 gm = (u32)(mk >> (16 * g)) & 0xFFFFu;
+                  if (pass[i]) bufs[(4 * g + i) * MF_BC + cnt[i] + __popc(gm & below)] = key[i];
+                  cnt[i] += __popc(gm);
+                  npass += __popcll(mk);
+                  // every entry ever appended for this row has E <= emax
+                  if (gm) emax[i] = fmaxf(emax[i], Ei[i]);
+                }
+                // a 16-column block appends at most 16 keys per query, so a
+                // buffer below MF_BC - 16 has room for the next block
+                constexpr int WM = MF_BC - 16;
+                const bool hot = cnt[0] >= WM || cnt[1] >= WM || cnt[2] >= WM || cnt[3] >= WM;
+                if (wave_ballot(hot) != 0) {
+                  __syncthreads();
+                  for (int r = 0; r < MF_TQ; ++r) {
+                    const int gi = r >> 2, ii = r & 3;
+                    int mine = cnt[0];
+#pragma unroll
+                    for (int j = 1; j < 4; ++j)
+                      if (j == ii) mine = cnt[j];
+                    if (__shfl(mine, 16 * gi) >= WM) merge(r);
+                  }
                 }
               }
             }
+          };
+          Group G0;
+          for (int g0 = pbeg; g0 < pend; g0 += 16 * NB) {
+            load_group(g0, G0);
+            compute_group(g0, G0);
           }
         }
       }
@@ -409,12 +481,24 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
     emit_query<KCAP, EXACT>(a, lists + r * KCAP, tile.start + r, A, cutoff,
                             a.Q.px + (int64_t)(tile.start + r) * dpQ, Tr, Vr);
   }
-  if (lane == 0 && a.tile_prof) {
-    u64 *tp = a.tile_prof + 4 * (size_t)blockIdx.x;
-    tp[0] = clk0;
-    tp[1] = wall_clock64();
-    tp[2] = (u64)np;
-    tp[3] = nvisit;
+  if (a.tile_prof) {
+    // diagnostic: the largest and smallest final threshold of the tile's rows
+    float tmax = 0.f, tmin = INFINITY;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (live[i]) {
+        tmax = fmaxf(tmax, key_dist(thr[i]));
+        tmin = fminf(tmin, key_dist(thr[i]));
+      }
+    tmax = wave_max(tmax);
+    tmin = wave_min(tmin);
+    if (lane == 0) {
+      u64 *tp = a.tile_prof + 4 * (size_t)blockIdx.x;
+      tp[0] = clk0;
+      tp[1] = wall_clock64();
+      tp[2] = ((u64)__float_as_uint(tmax) << 32) | (u64)__float_as_uint(tmin);
+      tp[3] = (nvisit << 32) | (u64)np;
+    }
   }
   if (lane == 0) {
     atomicAdd(a.n_evals, evals);
@@ -423,14 +507,15 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
   }
 }
 
-inline size_t mfma_lds(int kcap) {
-  return sizeof(u64) * ((size_t)MF_TQ * kcap + (size_t)MF_TQ * MF_BC + (size_t)kcap) + (16 + 256) * sizeof(float);
+inline size_t mfma_lds(int kcap, int dkmax) {
+  return sizeof(u64) * ((size_t)MF_TQ * kcap + (size_t)MF_TQ * MF_BC + (size_t)kcap) +
+         (16 + 256 + (size_t)MF_TQ * dkmax) * sizeof(float);
 }
 
 template <int PREC>
 hipError_t launch_mfma_any(const JoinArgs &a, int64_t grid, hipStream_t s) {
-  const size_t lds = mfma_lds(a.kcap);
   const bool big = a.dk > 128;  // dk = roundup(d, 32)
+  const size_t lds = mfma_lds(a.kcap, big ? 256 : 128);
 #define KJ_LAUNCH(KC, EX, DKV) \
   hipLaunchKernelGGL((k_join_mfma<KC, EX, PREC, DKV>), dim3((unsigned)grid), dim3(64), lds, s, a)
 #define KJ_DK(KC, EX)       \

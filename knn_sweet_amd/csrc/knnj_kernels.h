@@ -24,8 +24,9 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
                                 const int32_t *cl, const int32_t *cnt, int32_t *off, int32_t *fill,
                                 int32_t *perm, const float *d2r, float *pdist, u64 *gkeys, float *px,
                                 hipStream_t s);
-void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, Tile *tiles, int32_t *ntiles,
-                       hipStream_t s);
+// XCD-aware tile slots; *nslots = slots used (empty tiles have count 0)
+void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, Tile *tiles, int32_t *nslots,
+                       int max_slots, int nxcd, hipStream_t s);
 
 // k_bounds.hip
 void launch_bounds(const SideIndex &Qi, const SideIndex &Ri, int d, int k, double *theta, int32_t *plist,

@@ -183,3 +183,11 @@ def test_refinement_options(ctx, refine):
     X = datagen.skin_like(3000, seed=4)
     idx, dist, _ = ctx.join(X, X[:400], 50, refine_iters=refine)
     assert_matches_oracle(X, X[:400], 50, idx, dist)
+
+
+def test_xcd_tile_placement(ctx):
+    R, Q = datagen.mixture128(8000, 700, seed=41, d=64)
+    a = ctx.join(R, Q, 16)
+    b = ctx.join(R, Q, 16, xcd_tiles=True)
+    assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
+    assert_matches_oracle(R, Q, 16, b[0], b[1])
