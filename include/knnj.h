@@ -113,7 +113,7 @@ typedef struct {
                                candidates (VALU: the FP32 filter error; MFMA: the
                                filter error plus the bound's width) */
   int32_t kernel_used;      /* knnj_kernel actually run */
-  int32_t reserved0;
+  int32_t n_launches;       /* join kernel launches (knnj_join_batch: fused launches) */
   int64_t n_merges;         /* k-select buffer merges */
   int64_t n_appended;       /* candidates that passed a query's threshold */
   int32_t reserved[4];
@@ -150,6 +150,19 @@ knnj_status knnj_ctx_set_budget(knnj_ctx *ctx, int64_t bytes);
 knnj_status knnj_join(knnj_ctx *ctx, knnj_matrix R, knnj_matrix Q, int32_t k,
                       const knnj_opts *opts, int32_t *out_idx, float *out_dist,
                       knnj_stats *stats);
+
+/* Many independent joins (sizes, d and k may differ; one opts for all) in
+ * one call.  The reference has no batched form: each instance runs the
+ * whole program of knnj_join (knnjoin.cu:983-1172), with every instance's
+ * index build queued back to back on the context's stream and the join
+ * tiles of all instances that share a kernel instantiation (kernel,
+ * precision, candidate capacity, exact mode, width class) executed by one
+ * launch, each tile carrying its instance number.  Results are those of
+ * knnj_join on each instance alone.  stats: the plan of the first
+ * non-empty instance, counters summed over all, t_index_ms covering index
+ * builds, bounds and tile lists. */
+knnj_status knnj_join_batch(knnj_ctx *ctx, int32_t n_inst, const knnj_instance *inst,
+                            const knnj_opts *opts, knnj_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -59,7 +59,7 @@ class Stats(ctypes.Structure):
                 ("precision_used", ctypes.c_int32), ("m_q", ctypes.c_int32), ("m_r", ctypes.c_int32),
                 ("tile_q", ctypes.c_int32), ("kcap", ctypes.c_int32), ("n_tiles", ctypes.c_int64),
                 ("filter_max_abs_err", ctypes.c_double), ("kernel_used", ctypes.c_int32),
-                ("reserved0", ctypes.c_int32), ("n_merges", ctypes.c_int64),
+                ("n_launches", ctypes.c_int32), ("n_merges", ctypes.c_int64),
                 ("n_appended", ctypes.c_int64), ("reserved", ctypes.c_int32 * 4)]
 
     def as_dict(self) -> dict:
@@ -73,7 +73,7 @@ class Instance(ctypes.Structure):
 
 
 EXPORTS = ("knnj_version", "knnj_last_error", "knnj_ctx_create", "knnj_ctx_destroy",
-           "knnj_ctx_set_budget", "knnj_join")
+           "knnj_ctx_set_budget", "knnj_join", "knnj_join_batch")
 
 _lib = None
 
@@ -97,6 +97,9 @@ def load_library(path: str = LIB_PATH):
     lib.knnj_join.argtypes = [ctypes.c_void_p, Matrix, Matrix, ctypes.c_int32, ctypes.POINTER(Opts),
                               ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Stats)]
     lib.knnj_join.restype = ctypes.c_int
+    lib.knnj_join_batch.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Instance),
+                                    ctypes.POINTER(Opts), ctypes.POINTER(Stats)]
+    lib.knnj_join_batch.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -127,6 +130,18 @@ def _matrix(x):
         raise ValueError("expected a 2-D array")
     a = np.ascontiguousarray(a, dtype=np.float32)
     return Matrix(a.ctypes.data, a.shape[0], a.shape[1], a.shape[1], 0), a, False
+
+
+def _outputs(nq: int, k: int, aos: bool, qdev):
+    """Output arrays (idx, dist) and their pointers: torch on qdev's device, else numpy."""
+    if qdev is not None:
+        import torch
+        idx = torch.empty((nq, k, 2) if aos else (nq, k), dtype=torch.int32, device=qdev.device)
+        dist = None if aos else torch.empty((nq, k), dtype=torch.float32, device=qdev.device)
+        return idx, dist, idx.data_ptr(), (dist.data_ptr() if dist is not None else None)
+    idx = np.empty((nq, k, 2) if aos else (nq, k), np.int32)
+    dist = None if aos else np.empty((nq, k), np.float32)
+    return idx, dist, idx.ctypes.data, (dist.ctypes.data if dist is not None else None)
 
 
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
@@ -166,22 +181,34 @@ class Context:
         if R is Q:
             mQ = mR
         o = make_opts(**opts)
-        aos = bool(o.out_aos)
-        nq = mQ.n
-        if qdev:
-            import torch
-            dev = keepQ.device
-            idx = torch.empty((nq, k, 2) if aos else (nq, k), dtype=torch.int32, device=dev)
-            dist = None if aos else torch.empty((nq, k), dtype=torch.float32, device=dev)
-            pi, pd = idx.data_ptr(), (dist.data_ptr() if dist is not None else None)
-        else:
-            idx = np.empty((nq, k, 2) if aos else (nq, k), np.int32)
-            dist = None if aos else np.empty((nq, k), np.float32)
-            pi, pd = idx.ctypes.data, (dist.ctypes.data if dist is not None else None)
+        idx, dist, pi, pd = _outputs(mQ.n, k, bool(o.out_aos), keepQ if qdev else None)
         st = Stats()
         _check(_lib.knnj_join(self._h, mR, mQ, k, ctypes.byref(o), pi, pd, ctypes.byref(st)))
         del keepR, keepQ
         return idx, dist, st.as_dict()
+
+    def join_batch(self, instances, **opts):
+        """Many independent joins in one call.
+
+        instances: sequence of (R, Q, k).  Returns ([(idx, dist), ...], stats-dict)
+        with each result equal to ``join(R, Q, k)`` of that instance alone.
+        """
+        o = make_opts(**opts)
+        arr = (Instance * max(len(instances), 1))()
+        keep, results = [], []
+        for i, (R, Q, k) in enumerate(instances):
+            mR, keepR, _ = _matrix(R)
+            mQ, keepQ, qdev = _matrix(Q)
+            if R is Q:
+                mQ = mR
+            idx, dist, pi, pd = _outputs(mQ.n, k, bool(o.out_aos), keepQ if qdev else None)
+            arr[i] = Instance(mR, mQ, k, pi, pd)
+            keep += [keepR, keepQ]
+            results.append((idx, dist))
+        st = Stats()
+        _check(_lib.knnj_join_batch(self._h, len(instances), arr, ctypes.byref(o), ctypes.byref(st)))
+        del keep
+        return results, st.as_dict()
 
 
 _default_ctx = {}

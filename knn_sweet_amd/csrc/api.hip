@@ -7,6 +7,12 @@
 // stage timers instead of CLOCK_REALTIME around cudaDeviceSynchronize
 // (common8.h:83-90), and device-side sorts instead of the OpenMP
 // cudaMemcpy round trips (knnjoin.cu:380-400, 1076-1094).
+//
+// A join is planned as an `Inst`: decide() picks the kernel, precision and
+// capacities from the input scan; alloc_inst() lays out its device buffers
+// through an Alloc (named context workspace for knnj_join, one carved arena
+// for knnj_join_batch); enqueue_index() queues the index build, centring,
+// bounds and tile list; fill_args() fills the join kernel arguments.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,6 +23,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/knnj.h"
@@ -36,6 +43,13 @@ void set_err(const char *fmt, ...) {
   g_err = buf;
 }
 
+// Prefixes the current error message.
+void prefix_err(const char *fmt, int v) {
+  char buf[64];
+  snprintf(buf, sizeof(buf), fmt, v);
+  g_err = std::string(buf) + g_err;
+}
+
 struct DevBuf {
   void *p = nullptr;
   size_t n = 0;
@@ -48,11 +62,6 @@ uint64_t splitmix64(uint64_t &s) {
   return z ^ (z >> 31);
 }
 
-struct Status {
-  knnj_status code = KNNJ_OK;
-  bool ok() const { return code == KNNJ_OK; }
-};
-
 }  // namespace
 
 struct knnj_ctx {
@@ -60,17 +69,24 @@ struct knnj_ctx {
   hipStream_t stream = nullptr;
   std::map<std::string, DevBuf> ws;
   int64_t budget = 0;
-  std::vector<int32_t> host_cand[2];
-  hipEvent_t ev[8] = {};
+  // representative draws of the current call, kept alive for the async H2D
+  std::vector<std::vector<int32_t>> host_cand;
+  hipEvent_t ev[10] = {};
 };
 
-#define HIPCHK(expr)                                                             \
-  do {                                                                           \
-    hipError_t e_ = (expr);                                                      \
-    if (e_ != hipSuccess) {                                                      \
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
       set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
-      return KNNJ_EDEVICE;                                                       \
-    }                                                                            \
+      return KNNJ_EDEVICE;                                                                \
+    }                                                                                     \
+  } while (0)
+
+#define TRY(expr)                 \
+  do {                            \
+    knnj_status s_ = (expr);      \
+    if (s_ != KNNJ_OK) return s_; \
   } while (0)
 
 namespace {
@@ -91,8 +107,8 @@ knnj_status ws_get(knnj_ctx *c, const std::string &name, size_t bytes, void **ou
   }
   const size_t cur = ws_total(c) - b.n;
   if (c->budget > 0 && (int64_t)(cur + bytes) > c->budget) {
-    set_err("workspace budget exceeded: %s needs %zu bytes, %zu in use, budget %lld", name.c_str(), bytes,
-            cur, (long long)c->budget);
+    set_err("workspace budget exceeded: %s needs %zu bytes, %zu in use, budget %lld", name.c_str(), bytes, cur,
+            (long long)c->budget);
     return KNNJ_ENOMEM;
   }
   if (b.p) {
@@ -119,11 +135,23 @@ knnj_status ws(knnj_ctx *c, const std::string &name, size_t count, T **out) {
   return s;
 }
 
-#define TRY(expr)                      \
-  do {                                 \
-    knnj_status s_ = (expr);           \
-    if (s_ != KNNJ_OK) return s_;      \
-  } while (0)
+// Buffer source: the named context workspace, or consecutive 256-byte
+// aligned carves of one arena.  An arena Alloc with a null base is a sizing
+// pass: it hands out null pointers and only advances `off`.
+struct Alloc {
+  knnj_ctx *c = nullptr;
+  bool arena = false;
+  char *base = nullptr;
+  size_t off = 0;
+  template <class T>
+  knnj_status get(const std::string &name, size_t count, T **out) {
+    if (!arena) return ws(c, name, count, out);
+    const size_t bytes = (std::max<size_t>(count * sizeof(T), 1) + 255) & ~(size_t)255;
+    *out = base ? (T *)(base + off) : nullptr;
+    off += bytes;
+    return KNNJ_OK;
+  }
+};
 
 int pow2_at_least(int v) {
   int p = 1;
@@ -134,74 +162,6 @@ int pow2_at_least(int v) {
 int default_reps(int64_t n) {
   int64_t m = (int64_t)std::llround(2.0 * std::sqrt((double)n));
   return (int)std::max<int64_t>(1, std::min<int64_t>({m, (int64_t)KNNJ_MAX_REPS, n}));
-}
-
-// Builds one side's cluster index from device rows X.
-knnj_status build_side(knnj_ctx *c, int side, const float *X, int64_t n, int64_t ld, int d, int dpad, int m,
-                       int trials, uint64_t seed, int refine, bool integral, SideIndex *out) {
-  const std::string p = side == 0 ? "r_" : "q_";
-  hipStream_t s = c->stream;
-  float *reps, *d2r, *pdist, *px;
-  u64 *radbits, *gkeys;
-  int32_t *off, *perm, *cl, *cnt, *fill, *cand, *rep_rows;
-  double *partial;
-  const int64_t nn = std::max<int64_t>(n, 1);
-  TRY(ws(c, p + "reps", (size_t)m * dpad, &reps));
-  TRY(ws(c, p + "rad", (size_t)m, &radbits));
-  TRY(ws(c, p + "off", (size_t)m + 1, &off));
-  TRY(ws(c, p + "cnt", (size_t)m, &cnt));
-  TRY(ws(c, p + "fill", (size_t)m, &fill));
-  TRY(ws(c, p + "perm", (size_t)nn, &perm));
-  TRY(ws(c, p + "cl", (size_t)nn, &cl));
-  TRY(ws(c, p + "d2r", (size_t)nn, &d2r));
-  TRY(ws(c, p + "pdist", (size_t)nn, &pdist));
-  TRY(ws(c, p + "gkeys", (size_t)nn, &gkeys));
-  TRY(ws(c, p + "px", (size_t)nn * dpad, &px));
-  TRY(ws(c, p + "cand", (size_t)m * trials, &cand));
-  TRY(ws(c, p + "reprows", (size_t)m, &rep_rows));
-  const int nblk = rep_score_blocks(m);
-  TRY(ws(c, p + "partial", (size_t)nblk * trials, &partial));
-  __bf16 *rb = nullptr;
-  float *rn = nullptr;
-  const int dk = (d + 31) & ~31;
-  if (d >= 16 && dk <= 256) {
-    TRY(ws(c, p + "repsbf16", (size_t)m * dk, &rb));
-    TRY(ws(c, p + "repsnrm", (size_t)m, &rn));
-  }
-
-  std::vector<int32_t> &hc = c->host_cand[side];
-  hc.resize((size_t)m * trials);
-  uint64_t st = seed;
-  for (auto &v : hc) v = (int32_t)(splitmix64(st) % (uint64_t)n);
-  HIPCHK(hipMemcpyAsync(cand, hc.data(), hc.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  launch_rep_select(X, ld, d, dpad, cand, m, trials, partial, nblk, reps, rep_rows, s);
-  // Lloyd refinement: members of each sampled representative (ascending row
-  // order), their mean becomes the representative
-  for (int it = 0; it < refine; ++it) {
-    launch_assign(X, n, ld, d, dpad, reps, m, cl, rb, rn, s);
-    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * m, s));
-    HIPCHK(hipMemsetAsync(fill, 0, sizeof(int32_t) * m, s));
-    launch_count(cl, n, cnt, s);
-    launch_bucket_rows(cl, n, m, cnt, off, fill, perm, gkeys, s);
-    launch_centroid(X, ld, d, dpad, m, off, perm, integral ? 1 : 0, reps, s);
-  }
-  launch_assign(X, n, ld, d, dpad, reps, m, cl, rb, rn, s);
-  HIPCHK(hipMemsetAsync(radbits, 0, sizeof(u64) * m, s));
-  HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int32_t) * m, s));
-  HIPCHK(hipMemsetAsync(fill, 0, sizeof(int32_t) * m, s));
-  launch_assign_finish(X, n, ld, d, dpad, reps, cl, d2r, radbits, cnt, s);
-  launch_bucket_sort_permute(X, n, ld, d, dpad, m, cl, cnt, off, fill, perm, d2r, pdist, gkeys, px, s);
-  HIPCHK(hipGetLastError());
-  out->n = n;
-  out->m = m;
-  out->dpad = dpad;
-  out->reps = reps;
-  out->rad = (const double *)radbits;
-  out->off = off;
-  out->perm = perm;
-  out->pdist = pdist;
-  out->px = px;
-  return KNNJ_OK;
 }
 
 knnj_status check_matrix(const knnj_matrix &M, const char *name) {
@@ -234,11 +194,503 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
   return ms;
 }
 
+knnj_status check_join_args(const knnj_matrix &R, const knnj_matrix &Q, int32_t k, const knnj_opts &o,
+                            const void *out_idx, const void *out_dist) {
+  TRY(check_matrix(R, "R"));
+  TRY(check_matrix(Q, "Q"));
+  if (R.d != Q.d) {
+    set_err("R.d (%d) != Q.d (%d)", R.d, Q.d);
+    return KNNJ_EINVAL;
+  }
+  if (k < 1 || k > R.n) {
+    set_err("k = %d outside [1, |R| = %lld]", k, (long long)R.n);
+    return KNNJ_EINVAL;
+  }
+  if (k > KNNJ_MAX_K) {
+    set_err("k = %d above the supported maximum %d", k, KNNJ_MAX_K);
+    return KNNJ_EUNSUPPORTED;
+  }
+  if (Q.n > 0 && (!out_idx || (!o.out_aos && !out_dist))) {
+    set_err("output pointer is NULL");
+    return KNNJ_EINVAL;
+  }
+  if (o.precision < 0 || o.precision > KNNJ_PREC_BF16X1) {
+    set_err("unknown precision %d", o.precision);
+    return KNNJ_EINVAL;
+  }
+  if (o.precision == KNNJ_PREC_FP8) {
+    set_err("FP8 filter is not built");
+    return KNNJ_EUNSUPPORTED;
+  }
+  if (o.tile_q != 0 && o.tile_q != 1 && o.tile_q != 2 && o.tile_q != 4 && o.tile_q != 8) {
+    set_err("tile_q must be 0, 1, 2, 4 or 8");
+    return KNNJ_EINVAL;
+  }
+  if (o.kernel < 0 || o.kernel > KNNJ_KERNEL_MFMA) {
+    set_err("unknown kernel %d", o.kernel);
+    return KNNJ_EINVAL;
+  }
+  if ((o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1) && o.kernel == KNNJ_KERNEL_VALU) {
+    set_err("the BF16 filter runs on the MFMA kernel only");
+    return KNNJ_EINVAL;
+  }
+  return KNNJ_OK;
+}
+
+// Buffers of one side's cluster index.
+struct SideBufs {
+  float *reps = nullptr, *d2r = nullptr, *pdist = nullptr, *px = nullptr, *rn = nullptr;
+  u64 *radbits = nullptr, *gkeys = nullptr;
+  int32_t *off = nullptr, *perm = nullptr, *cl = nullptr, *cnt = nullptr, *fill = nullptr, *cand = nullptr,
+          *rep_rows = nullptr;
+  double *partial = nullptr;
+  __bf16 *rb = nullptr;
+  int nblk = 0;
+};
+
+// One planned join.
+struct Inst {
+  int id = 0;  // position in a batch (the tiles' pad field)
+  // inputs: device rows, or host rows still to be copied
+  const float *dR = nullptr, *dQ = nullptr;
+  const float *hR = nullptr, *hQ = nullptr;
+  int64_t nR = 0, nQ = 0, ldR = 0, ldQ = 0;
+  int d = 0, k = 0;
+  bool self = false;
+  knnj_opts o{};
+  // outputs: the caller's pointers, and device staging when they are host
+  int32_t *user_idx = nullptr;
+  float *user_dist = nullptr;
+  int32_t *out_idx = nullptr;
+  float *out_dist = nullptr;
+  bool out_host = false, aos = false;
+  // decisions
+  int kind = 0, kcap = 0, tq = 0, dpad = 0, dk = 0, m_r = 0, m_q = 0, trials = 10, refine = 1;
+  bool mfma = false, bf16 = false, bf16x1 = false, exact = false, exact_valu = false, integral = false;
+  bool use_mfma = false;
+  double span = 0.0;
+  uint64_t seed = 2015;
+  int64_t max_slots = 0;
+  // device state
+  SideBufs rb, qb;
+  SideIndex Ri{}, Qi{};
+  void *rc = nullptr;
+  float *nr = nullptr, *reps32 = nullptr, *rn32 = nullptr;
+  double *theta = nullptr;
+  int32_t *plist = nullptr, *pcnt = nullptr;
+  Tile *tiles = nullptr, *ovf_tiles = nullptr, *ovf2_tiles = nullptr;
+  int32_t *counters = nullptr;  // [0] slots, [1] overflow, [2] overflow 2, [3] max err bits, [4] overflow 3
+  u64 *evals = nullptr;         // [0] evaluations, [1] merges, [2] appended
+  JoinArgs args{};
+
+  // exact flag of the kernel that runs the tiles
+  int kernel_exact() const { return use_mfma ? (exact ? 1 : 0) : ((mfma ? exact_valu : exact) ? 1 : 0); }
+  // kernel instantiation: joins with equal keys can share one launch
+  std::tuple<int, int, int, int, int> launch_key() const {
+    return std::make_tuple(use_mfma ? (bf16 ? (bf16x1 ? 3 : 2) : 1) : 0, kcap, kernel_exact(), use_mfma ? 0 : tq,
+                           use_mfma && dk > 128 ? 1 : 0);
+  }
+};
+
+// Input-side fields of a join (no device work).
+void describe(Inst &I, const knnj_matrix &R, const knnj_matrix &Q, int32_t k, const knnj_opts &o,
+              int32_t *out_idx, float *out_dist) {
+  I.d = R.d;
+  I.k = k;
+  I.o = o;
+  I.nR = R.n;
+  I.nQ = Q.n;
+  const int64_t ldR = R.ld ? R.ld : R.d, ldQ = Q.ld ? Q.ld : Q.d;
+  I.self = R.data == Q.data && R.n == Q.n && ldR == ldQ && R.on_device == Q.on_device;
+  (R.on_device ? I.dR : I.hR) = R.data;
+  I.ldR = ldR;
+  (Q.on_device ? I.dQ : I.hQ) = Q.data;
+  I.ldQ = ldQ;
+  I.aos = o.out_aos != 0;
+  I.out_host = !Q.on_device;
+  I.user_idx = out_idx;
+  I.user_dist = out_dist;
+}
+
+// Device copies of host inputs (rows packed to ld = d), then the H2D copies.
+knnj_status stage_inputs(Alloc &A, Inst &I, hipStream_t s) {
+  if (I.hR) {
+    float *b;
+    TRY(A.get("in_r", (size_t)std::max<int64_t>(I.nR, 1) * I.d, &b));
+    if (b && I.nR > 0)
+      HIPCHK(hipMemcpy2DAsync(b, sizeof(float) * I.d, I.hR, sizeof(float) * I.ldR, sizeof(float) * I.d, I.nR,
+                              hipMemcpyHostToDevice, s));
+    I.dR = b;
+    I.ldR = I.d;
+  }
+  if (I.self) {
+    I.dQ = I.dR;
+    I.ldQ = I.ldR;
+  } else if (I.hQ) {
+    float *b;
+    TRY(A.get("in_q", (size_t)std::max<int64_t>(I.nQ, 1) * I.d, &b));
+    if (b && I.nQ > 0)
+      HIPCHK(hipMemcpy2DAsync(b, sizeof(float) * I.d, I.hQ, sizeof(float) * I.ldQ, sizeof(float) * I.d, I.nQ,
+                              hipMemcpyHostToDevice, s));
+    I.dQ = b;
+    I.ldQ = I.d;
+  }
+  return KNNJ_OK;
+}
+
+// Kernel, precision and capacity choices from the input scan flags
+// {non-finite, max |x| bits, non-integral}.
+knnj_status decide(Inst &I, const u32 *flags) {
+  if (flags[0]) {
+    set_err("non-finite value in R or Q");
+    return KNNJ_EINVAL;
+  }
+  const int d = I.d, k = I.k;
+  const knnj_opts &o = I.o;
+  float maxabs;
+  std::memcpy(&maxabs, &flags[1], 4);
+  I.span = 2.0 * (double)maxabs;
+  I.integral = !flags[2];
+  I.exact_valu = I.integral && (double)d * I.span * I.span < 16777216.0;
+  const bool exact_mfma = I.integral && 4.0 * (double)d * I.span * I.span < 16777216.0;
+  int kind = o.kernel;
+  if (o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1) kind = KNNJ_KERNEL_MFMA;
+  if (kind == KNNJ_KERNEL_AUTO) kind = (d >= 32 && d <= 256) ? KNNJ_KERNEL_MFMA : KNNJ_KERNEL_VALU;
+  if (kind == KNNJ_KERNEL_MFMA && d > 256) {
+    set_err("the MFMA kernel supports d <= 256 (d = %d)", d);
+    return KNNJ_EUNSUPPORTED;
+  }
+  I.kind = kind;
+  I.mfma = kind == KNNJ_KERNEL_MFMA;
+  I.bf16 = I.mfma && (o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1 ||
+                      (o.precision == KNNJ_PREC_AUTO && d >= 64 && !exact_mfma));
+  I.bf16x1 = I.bf16 && o.precision == KNNJ_PREC_BF16X1;
+  I.exact = I.mfma ? (!I.bf16 && exact_mfma) : I.exact_valu;
+  I.dpad = (d + 3) & ~3;
+  I.dk = I.mfma ? (d + 31) & ~31 : 0;
+  const bool brute = o.brute_force != 0;
+  int m_r = brute ? 1 : (o.m_r > 0 ? (int)std::min<int64_t>(o.m_r, I.nR) : default_reps(I.nR));
+  int m_q = brute ? 1 : (o.m_q > 0 ? (int)std::min<int64_t>(o.m_q, I.nQ) : default_reps(I.nQ));
+  I.m_r = std::max(1, std::min(m_r, KNNJ_MAX_REPS));
+  I.m_q = std::max(1, std::min(m_q, KNNJ_MAX_REPS));
+  if (I.self) I.m_q = I.m_r;
+  I.trials = o.trials > 0 ? std::min(o.trials, 64) : 10;
+  I.seed = o.seed ? o.seed : 2015;
+  I.refine = o.refine_iters < 0 ? 0 : (o.refine_iters > 0 ? o.refine_iters : 1);
+  int kcap = I.exact    ? pow2_at_least(std::max(k, 64))
+             : I.bf16x1 ? pow2_at_least(std::max(2 * k, 64))
+                        : pow2_at_least(std::max(k + std::max(8, k / 8), 64));
+  kcap = std::min(kcap, I.mfma ? 512 : KNNJ_MAX_K);
+  if (kcap < k) kcap = KNNJ_MAX_K;  // only the VALU kernel holds k > 512
+  I.kcap = kcap;
+  I.use_mfma = I.mfma && kcap <= 512;
+  I.tq = I.use_mfma ? 16 : (o.tile_q ? std::min(o.tile_q, join_tile_q(kcap)) : join_tile_q(kcap));
+  // XCD-aware slots, empty ones included
+  I.max_slots = (int64_t)(o.xcd_tiles ? 8 : 1) * (I.nQ / I.tq + I.m_q + 1);
+  return KNNJ_OK;
+}
+
+knnj_status alloc_side(Alloc &A, const std::string &p, int64_t n, int m, int dpad, int d, int trials,
+                       SideBufs &b) {
+  const int64_t nn = std::max<int64_t>(n, 1);
+  TRY(A.get(p + "reps", (size_t)m * dpad, &b.reps));
+  TRY(A.get(p + "rad", (size_t)m, &b.radbits));
+  TRY(A.get(p + "off", (size_t)m + 1, &b.off));
+  TRY(A.get(p + "cnt", (size_t)m, &b.cnt));
+  TRY(A.get(p + "fill", (size_t)m, &b.fill));
+  TRY(A.get(p + "perm", (size_t)nn, &b.perm));
+  TRY(A.get(p + "cl", (size_t)nn, &b.cl));
+  TRY(A.get(p + "d2r", (size_t)nn, &b.d2r));
+  TRY(A.get(p + "pdist", (size_t)nn, &b.pdist));
+  TRY(A.get(p + "gkeys", (size_t)nn, &b.gkeys));
+  TRY(A.get(p + "px", (size_t)nn * dpad, &b.px));
+  TRY(A.get(p + "cand", (size_t)m * trials, &b.cand));
+  TRY(A.get(p + "reprows", (size_t)m, &b.rep_rows));
+  b.nblk = rep_score_blocks(m);
+  TRY(A.get(p + "partial", (size_t)b.nblk * trials, &b.partial));
+  const int dk = (d + 31) & ~31;
+  if (d >= 16 && dk <= 256) {  // the MFMA assignment's bf16 representatives
+    TRY(A.get(p + "repsbf16", (size_t)m * dk, &b.rb));
+    TRY(A.get(p + "repsnrm", (size_t)m, &b.rn));
+  }
+  return KNNJ_OK;
+}
+
+// Every device buffer of one join after decide(); `tiles` given: the join's
+// slot range of a batch's shared tile array.
+knnj_status alloc_inst(Alloc &A, Inst &I, Tile *tiles) {
+  TRY(alloc_side(A, "r_", I.nR, I.m_r, I.dpad, I.d, I.trials, I.rb));
+  if (!I.self) TRY(alloc_side(A, "q_", I.nQ, I.m_q, I.dpad, I.d, I.trials, I.qb));
+  if (I.mfma) {
+    char *rc;
+    TRY(A.get("r_rc", (size_t)I.nR * I.dk * 4, &rc));
+    I.rc = rc;
+    TRY(A.get("r_nr", (size_t)I.nR, &I.nr));
+    TRY(A.get("r_reps32", (size_t)I.m_r * I.dk, &I.reps32));
+    TRY(A.get("r_rn32", (size_t)I.m_r, &I.rn32));
+  }
+  TRY(A.get("theta", (size_t)I.m_q, &I.theta));
+  TRY(A.get("plist", (size_t)I.m_q * I.m_r, &I.plist));
+  TRY(A.get("pcnt", (size_t)I.m_q, &I.pcnt));
+  if (tiles)
+    I.tiles = tiles;
+  else
+    TRY(A.get("tiles", (size_t)I.max_slots, &I.tiles));
+  TRY(A.get("ovf_tiles", (size_t)I.nQ, &I.ovf_tiles));
+  TRY(A.get("ovf2_tiles", (size_t)I.nQ, &I.ovf2_tiles));
+  TRY(A.get("counters", 8, &I.counters));
+  TRY(A.get("evals", 4, &I.evals));
+  const size_t nout = (size_t)I.nQ * I.k;
+  if (I.out_host) {
+    TRY(A.get("out_idx", nout * (I.aos ? 2 : 1), &I.out_idx));
+    if (!I.aos) TRY(A.get("out_dist", nout, &I.out_dist));
+  } else {
+    I.out_idx = I.user_idx;
+    I.out_dist = I.user_dist;
+  }
+  return KNNJ_OK;
+}
+
+// One side's cluster index: sampled representatives, Lloyd refinement,
+// assignment, radii, and the cluster-sorted permuted copy.
+knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
+  SideBufs &b = ref_side ? I.rb : I.qb;
+  const float *X = ref_side ? I.dR : I.dQ;
+  const int64_t n = ref_side ? I.nR : I.nQ, ld = ref_side ? I.ldR : I.ldQ;
+  const int m = ref_side ? I.m_r : I.m_q, d = I.d, dpad = I.dpad;
+  c->host_cand.emplace_back((size_t)m * I.trials);
+  std::vector<int32_t> &hc = c->host_cand.back();
+  uint64_t st = I.seed;
+  for (auto &v : hc) v = (int32_t)(splitmix64(st) % (uint64_t)n);
+  HIPCHK(hipMemcpyAsync(b.cand, hc.data(), hc.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  launch_rep_select(X, ld, d, dpad, b.cand, m, I.trials, b.partial, b.nblk, b.reps, b.rep_rows, s);
+  // Lloyd refinement: members of each sampled representative (ascending row
+  // order), their mean becomes the representative
+  for (int it = 0; it < I.refine; ++it) {
+    launch_assign(X, n, ld, d, dpad, b.reps, m, b.cl, b.rb, b.rn, s);
+    HIPCHK(hipMemsetAsync(b.cnt, 0, sizeof(int32_t) * m, s));
+    HIPCHK(hipMemsetAsync(b.fill, 0, sizeof(int32_t) * m, s));
+    launch_count(b.cl, n, b.cnt, s);
+    launch_bucket_rows(b.cl, n, m, b.cnt, b.off, b.fill, b.perm, b.gkeys, s);
+    launch_centroid(X, ld, d, dpad, m, b.off, b.perm, I.integral ? 1 : 0, b.reps, s);
+  }
+  launch_assign(X, n, ld, d, dpad, b.reps, m, b.cl, b.rb, b.rn, s);
+  HIPCHK(hipMemsetAsync(b.radbits, 0, sizeof(u64) * m, s));
+  HIPCHK(hipMemsetAsync(b.cnt, 0, sizeof(int32_t) * m, s));
+  HIPCHK(hipMemsetAsync(b.fill, 0, sizeof(int32_t) * m, s));
+  launch_assign_finish(X, n, ld, d, dpad, b.reps, b.cl, b.d2r, b.radbits, b.cnt, s);
+  launch_bucket_sort_permute(X, n, ld, d, dpad, m, b.cl, b.cnt, b.off, b.fill, b.perm, b.d2r, b.pdist, b.gkeys,
+                             b.px, s);
+  HIPCHK(hipGetLastError());
+  SideIndex &out = ref_side ? I.Ri : I.Qi;
+  out.n = n;
+  out.m = m;
+  out.dpad = dpad;
+  out.reps = b.reps;
+  out.rad = (const double *)b.radbits;
+  out.off = b.off;
+  out.perm = b.perm;
+  out.pdist = b.pdist;
+  out.px = b.px;
+  return KNNJ_OK;
+}
+
+// Index build, centring, bounds and the tile list of one join.  Optional
+// events are recorded after the index (with centring) and after the bounds.
+knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_index, hipEvent_t ev_bounds) {
+  TRY(enqueue_side(c, I, true, s));
+  if (I.self)
+    I.Qi = I.Ri;
+  else
+    TRY(enqueue_side(c, I, false, s));
+  if (I.mfma) {
+    launch_center(I.Ri, I.d, I.dk, I.bf16 ? (I.bf16x1 ? 1 : 2) : 0, I.rc, I.nr, I.reps32, I.rn32, s);
+    HIPCHK(hipGetLastError());
+  }
+  if (ev_index) HIPCHK(hipEventRecord(ev_index, s));
+  launch_bounds(I.Qi, I.Ri, I.d, I.k, I.theta, I.plist, I.pcnt, s);
+  HIPCHK(hipGetLastError());
+  if (ev_bounds) HIPCHK(hipEventRecord(ev_bounds, s));
+  HIPCHK(hipMemsetAsync(I.counters, 0, 8 * sizeof(int32_t), s));
+  HIPCHK(hipMemsetAsync(I.evals, 0, 4 * sizeof(u64), s));
+  launch_make_tiles(I.Qi.off, I.m_q, I.pcnt, I.tq, I.id, I.tiles, I.counters, (int)I.max_slots,
+                    I.o.xcd_tiles ? 8 : 1, s);
+  HIPCHK(hipGetLastError());
+  return KNNJ_OK;
+}
+
+void fill_args(Inst &I) {
+  JoinArgs &a = I.args;
+  a = JoinArgs{};
+  const int d = I.d;
+  a.Q = I.Qi;
+  a.R = I.Ri;
+  a.d = d;
+  a.k = I.k;
+  a.kcap = I.kcap;
+  a.exact = I.kernel_exact();
+  const double delta = I.exact ? 0.0 : (double)(d + 4) * 0x1p-23 + 0x1p-40;
+  a.delta = delta;
+  a.fac = (1.0 + delta) / (1.0 - delta) * (1.0 + 0x1p-20);
+  a.tiles = I.tiles;
+  a.ntiles = I.counters;
+  a.theta = I.theta;
+  a.plist = I.plist;
+  a.pcnt = I.pcnt;
+  a.out_idx = I.out_idx;
+  a.out_dist = I.out_dist;
+  a.out_mode = I.aos ? 2 : (I.o.out_squared ? 1 : 0);
+  a.n_evals = I.evals;
+  a.ovf_tiles = I.ovf_tiles;
+  a.ovf_count = I.counters + 1;
+  a.max_err_bits = (u32 *)(I.counters + 3);
+  a.rc = I.rc;
+  a.nr = I.nr;
+  a.reps32 = I.reps32;
+  a.rn32 = I.rn32;
+  a.dk = I.dk;
+  // per-entry error codes in the MFMA filter keys: rows must fit 26 bits;
+  // codes span 63 binades below the largest possible squared distance
+  a.ecode = (I.use_mfma && !I.exact && I.nR < (int64_t)(1 << 26)) ? 1 : 0;
+  const double smax = std::max(1e-30, (double)d * I.span * I.span);
+  a.ebase = std::ilogb(smax) + 1 - 62;
+  const double u = 0x1p-24, ub = 0x1p-8;
+  auto gam = [&](double n) { return n * u / (1.0 - n * u); };
+  // |computed dot - (q-c).(r-c)| <= cdot ||q-c|| ||r-c||
+  double cdot = gam(d);  // fp32 operands
+  if (I.bf16x1)
+    cdot = 2.0 * ub + ub * ub + gam(d) * (1.0 + ub) * (1.0 + ub);  // one bf16 product
+  else if (I.bf16)
+    cdot = 3.1 * ub * ub + 1.02 * gam(3.0 * d);  // hi/lo, three products
+  a.c1 = (float)((2.0 * cdot + 4.0 * u) * 1.01);
+  a.c2 = (float)((gam(d + 3) + 8.0 * u) * 1.01);
+}
+
+hipError_t launch_main(const Inst &I, int64_t grid, hipStream_t s) {
+  if (I.use_mfma) {
+    if (I.bf16) return launch_join_mfma_bf16(I.args, I.bf16x1 ? 1 : 3, grid, s);
+    return launch_join_mfma_f32(I.args, grid, s);
+  }
+  return launch_join(I.args, I.tq, grid, s);
+}
+
+// Overflowed queries of one join, one per tile, at a larger capacity: the
+// MFMA kernel at 512 if it ran below that, else (and as the last resort)
+// the VALU kernel at 1024.
+knnj_status rerun_overflow(Inst &I, int32_t novf, hipStream_t s) {
+  JoinArgs b = I.args;
+  b.tiles = I.ovf_tiles;
+  b.ntiles = I.counters + 1;
+  b.ovf_tiles = I.ovf2_tiles;
+  b.ovf_count = I.counters + 2;
+  b.tile_prof = nullptr;
+  b.ovf_diag = nullptr;
+  const bool mfma_up = I.use_mfma && I.kcap < 512;
+  if (mfma_up) {
+    b.kcap = 512;
+    if (I.bf16)
+      HIPCHK(launch_join_mfma_bf16(b, I.bf16x1 ? 1 : 3, novf, s));
+    else
+      HIPCHK(launch_join_mfma_f32(b, novf, s));
+  } else {
+    b.kcap = KNNJ_MAX_K;
+    b.exact = I.exact_valu ? 1 : 0;
+    b.ecode = 0;
+    HIPCHK(launch_join(b, 1, novf, s));
+  }
+  int32_t h2 = 0;
+  HIPCHK(hipMemcpyAsync(&h2, I.counters + 2, sizeof(h2), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (h2 == 0) return KNNJ_OK;
+  if (!mfma_up) {
+    set_err("%d queries overflowed the largest candidate capacity (%d)", h2, KNNJ_MAX_K);
+    return KNNJ_EOVERFLOW;
+  }
+  JoinArgs c2 = b;
+  c2.tiles = I.ovf2_tiles;
+  c2.ntiles = I.counters + 2;
+  c2.ovf_tiles = I.ovf_tiles;
+  c2.ovf_count = I.counters + 4;
+  c2.kcap = KNNJ_MAX_K;
+  c2.exact = I.exact_valu ? 1 : 0;
+  c2.ecode = 0;
+  HIPCHK(launch_join(c2, 1, h2, s));
+  int32_t h3 = 0;
+  HIPCHK(hipMemcpyAsync(&h3, I.counters + 4, sizeof(h3), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (h3 > 0) {
+    set_err("%d queries overflowed the largest candidate capacity (%d)", h3, KNNJ_MAX_K);
+    return KNNJ_EOVERFLOW;
+  }
+  return KNNJ_OK;
+}
+
+knnj_status copy_outputs(const Inst &I, hipStream_t s) {
+  if (!I.out_host || I.nQ == 0) return KNNJ_OK;
+  const size_t nout = (size_t)I.nQ * I.k;
+  HIPCHK(hipMemcpyAsync(I.user_idx, I.out_idx, nout * sizeof(int32_t) * (I.aos ? 2 : 1), hipMemcpyDeviceToHost,
+                        s));
+  if (!I.aos) HIPCHK(hipMemcpyAsync(I.user_dist, I.out_dist, nout * sizeof(float), hipMemcpyDeviceToHost, s));
+  return KNNJ_OK;
+}
+
+void plan_stats(const Inst &I, knnj_stats &st) {
+  st.exact_mode = I.exact ? 1 : 0;
+  st.precision_used = I.bf16 ? (I.bf16x1 ? KNNJ_PREC_BF16X1 : KNNJ_PREC_BF16) : KNNJ_PREC_FP32;
+  st.kernel_used = I.kind;
+  st.m_q = I.m_q;
+  st.m_r = I.m_r;
+  st.tile_q = I.tq;
+  st.kcap = I.kcap;
+}
+
+// Diagnostic print of the per-tile record (KNNJ_TILE_PROFILE set).
+knnj_status print_tile_profile(const Inst &I, int32_t nslots, int32_t novf) {
+  if (nslots <= 0) return KNNJ_OK;
+  std::vector<u64> tp((size_t)nslots * 4);
+  HIPCHK(hipMemcpy(tp.data(), I.args.tile_prof, tp.size() * sizeof(u64), hipMemcpyDeviceToHost));
+  std::vector<double> dur;
+  double listed = 0, visited = 0;
+  u64 t0 = ~0ull, t1 = 0;
+  for (int i = 0; i < nslots; ++i) {
+    if (tp[4 * i + 1] == 0) continue;  // empty slot
+    dur.push_back((double)(tp[4 * i + 1] - tp[4 * i]));
+    listed += (double)(tp[4 * i + 3] & 0xFFFFFFFFull);
+    visited += (double)(tp[4 * i + 3] >> 32);
+    t0 = std::min(t0, tp[4 * i]);
+    t1 = std::max(t1, tp[4 * i + 1]);
+  }
+  if (const char *dump = std::getenv("KNNJ_TILE_DUMP")) {
+    if (FILE *f = fopen(dump, "wb")) {
+      fwrite(tp.data(), sizeof(u64), tp.size(), f);
+      fclose(f);
+    }
+  }
+  std::sort(dur.begin(), dur.end());
+  double sum = 0;
+  for (double v : dur) sum += v;
+  const int nt = (int)dur.size();
+  if (nt > 0)
+    fprintf(stderr,
+            "[knnj tile profile] tiles=%d span=%.0f ticks sum=%.0f mean=%.1f p50=%.0f p99=%.0f max=%.0f "
+            "listed/tile=%.1f visited/tile=%.1f\n",
+            nt, (double)(t1 - t0), sum, sum / nt, dur[nt / 2], dur[(size_t)(0.99 * (nt - 1))], dur.back(),
+            listed / nt, visited / nt);
+  if (novf > 0 && I.args.ovf_diag) {
+    float dg[256];
+    HIPCHK(hipMemcpy(dg, I.args.ovf_diag, sizeof(dg), hipMemcpyDeviceToHost));
+    for (int i = 0; i < std::min(novf, 8); ++i)
+      fprintf(stderr, "[knnj overflow] cutoff=%.6g kth=%.6g last=%.6g first=%.6g\n", dg[4 * i], dg[4 * i + 1],
+              dg[4 * i + 2], dg[4 * i + 3]);
+  }
+  return KNNJ_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
-const char *knnj_version(void) { return "knnj 0.1.0 (gfx950)"; }
+const char *knnj_version(void) { return "knnj 0.2.0 (gfx950)"; }
 
 const char *knnj_last_error(void) { return g_err.c_str(); }
 
@@ -302,393 +754,287 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
     set_err("context is NULL");
     return KNNJ_EINVAL;
   }
-  TRY(check_matrix(R, "R"));
-  TRY(check_matrix(Q, "Q"));
-  if (R.d != Q.d) {
-    set_err("R.d (%d) != Q.d (%d)", R.d, Q.d);
-    return KNNJ_EINVAL;
-  }
-  if (k < 1 || k > R.n) {
-    set_err("k = %d outside [1, |R| = %lld]", k, (long long)R.n);
-    return KNNJ_EINVAL;
-  }
-  if (k > KNNJ_MAX_K) {
-    set_err("k = %d above the supported maximum %d", k, KNNJ_MAX_K);
-    return KNNJ_EUNSUPPORTED;
-  }
   knnj_opts o{};
   if (opts) o = *opts;
-  const bool aos = o.out_aos != 0;
-  if (Q.n > 0 && (!out_idx || (!aos && !out_dist))) {
-    set_err("output pointer is NULL");
-    return KNNJ_EINVAL;
-  }
-  if (o.precision == KNNJ_PREC_FP8) {
-    set_err("FP8 filter is not built");
-    return KNNJ_EUNSUPPORTED;
-  }
-  if (o.tile_q != 0 && o.tile_q != 1 && o.tile_q != 2 && o.tile_q != 4 && o.tile_q != 8) {
-    set_err("tile_q must be 0, 1, 2, 4 or 8");
-    return KNNJ_EINVAL;
-  }
-  knnj_stats st{};
+  TRY(check_join_args(R, Q, k, o, out_idx, out_dist));
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
-  const int d = R.d;
-  const int64_t ldR = R.ld ? R.ld : d, ldQ = Q.ld ? Q.ld : d;
-  const bool self = R.data == Q.data && R.n == Q.n && ldR == ldQ && R.on_device == Q.on_device;
-  HIPCHK(hipEventRecord(c->ev[0], s));
+  hipEvent_t *ev = c->ev;
+  c->host_cand.clear();
+  knnj_stats st{};
+  Inst I;
+  describe(I, R, Q, k, o, out_idx, out_dist);
+  HIPCHK(hipEventRecord(ev[0], s));
 
-  // device-resident inputs
-  const float *dR = R.data, *dQ = Q.data;
-  int64_t dldR = ldR, dldQ = ldQ;
-  if (!R.on_device) {
-    float *b;
-    TRY(ws(c, "in_r", (size_t)std::max<int64_t>(R.n, 1) * d, &b));
-    HIPCHK(hipMemcpy2DAsync(b, sizeof(float) * d, R.data, sizeof(float) * ldR, sizeof(float) * d, R.n,
-                            hipMemcpyHostToDevice, s));
-    dR = b;
-    dldR = d;
-  }
-  if (self) {
-    dQ = dR;
-    dldQ = dldR;
-  } else if (!Q.on_device && Q.n > 0) {
-    float *b;
-    TRY(ws(c, "in_q", (size_t)Q.n * d, &b));
-    HIPCHK(hipMemcpy2DAsync(b, sizeof(float) * d, Q.data, sizeof(float) * ldQ, sizeof(float) * d, Q.n,
-                            hipMemcpyHostToDevice, s));
-    dQ = b;
-    dldQ = d;
-  }
-
-  // input scan: finiteness, integrality, magnitude
+  // device-resident inputs, then the input scan: finiteness, integrality, magnitude
+  Alloc A;
+  A.c = c;
+  TRY(stage_inputs(A, I, s));
   u32 *flags;
   TRY(ws(c, "flags", 8, &flags));
   HIPCHK(hipMemsetAsync(flags, 0, 8 * sizeof(u32), s));
-  launch_scan_input(dR, R.n, d, dldR, flags, s);
-  if (!self && Q.n > 0) launch_scan_input(dQ, Q.n, d, dldQ, flags, s);
+  launch_scan_input(I.dR, I.nR, I.d, I.ldR, flags, s);
+  if (!I.self && I.nQ > 0) launch_scan_input(I.dQ, I.nQ, I.d, I.ldQ, flags, s);
   u32 hflags[3];
   HIPCHK(hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  if (hflags[0]) {
-    set_err("non-finite value in R or Q");
-    return KNNJ_EINVAL;
-  }
-  float maxabs;
-  std::memcpy(&maxabs, &hflags[1], 4);
-  const double span = 2.0 * (double)maxabs;
-  const bool integral = !hflags[2];
-  const bool exact_valu = integral && (double)d * span * span < 16777216.0;
-  const bool exact_mfma = integral && 4.0 * (double)d * span * span < 16777216.0;
-  int kind = o.kernel;
-  if (o.precision < 0 || o.precision > KNNJ_PREC_BF16X1) {
-    set_err("unknown precision %d", o.precision);
-    return KNNJ_EINVAL;
-  }
-  if (o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1) {
-    if (kind == KNNJ_KERNEL_VALU) {
-      set_err("the BF16 filter runs on the MFMA kernel only");
-      return KNNJ_EINVAL;
-    }
-    kind = KNNJ_KERNEL_MFMA;
-  }
-  if (kind == KNNJ_KERNEL_AUTO) kind = (d >= 32 && d <= 256) ? KNNJ_KERNEL_MFMA : KNNJ_KERNEL_VALU;
-  if (kind != KNNJ_KERNEL_VALU && kind != KNNJ_KERNEL_MFMA) {
-    set_err("unknown kernel %d", o.kernel);
-    return KNNJ_EINVAL;
-  }
-  if (kind == KNNJ_KERNEL_MFMA && d > 256) {
-    set_err("the MFMA kernel supports d <= 256 (d = %d)", d);
-    return KNNJ_EUNSUPPORTED;
-  }
-  const bool mfma = kind == KNNJ_KERNEL_MFMA;
-  const bool bf16 = mfma && (o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1 ||
-                             (o.precision == KNNJ_PREC_AUTO && d >= 64 && !exact_mfma));
-  const bool bf16x1 = bf16 && o.precision == KNNJ_PREC_BF16X1;
-  const bool exact = mfma ? (!bf16 && exact_mfma) : exact_valu;
-  st.exact_mode = exact ? 1 : 0;
-  st.precision_used = bf16 ? (bf16x1 ? KNNJ_PREC_BF16X1 : KNNJ_PREC_BF16) : KNNJ_PREC_FP32;
-  st.kernel_used = kind;
-  if (Q.n == 0) {
+  TRY(decide(I, hflags));
+  plan_stats(I, st);
+  if (I.nQ == 0) {
     if (stats) *stats = st;
     return KNNJ_OK;
   }
+  TRY(alloc_inst(A, I, nullptr));
 
-  const int dpad = (d + 3) & ~3;
-  const bool brute = o.brute_force != 0;
-  int m_r = brute ? 1 : (o.m_r > 0 ? std::min<int64_t>(o.m_r, R.n) : default_reps(R.n));
-  int m_q = brute ? 1 : (o.m_q > 0 ? std::min<int64_t>(o.m_q, Q.n) : default_reps(Q.n));
-  m_r = std::min(m_r, KNNJ_MAX_REPS);
-  m_q = std::min(m_q, KNNJ_MAX_REPS);
-  if (self) m_q = m_r;
-  const int trials = o.trials > 0 ? std::min(o.trials, 64) : 10;
-  const uint64_t seed = o.seed ? o.seed : 2015;
-  st.m_q = m_q;
-  st.m_r = m_r;
-
-  HIPCHK(hipEventRecord(c->ev[1], s));
-  SideIndex Ri{}, Qi{};
-  const int refine = o.refine_iters < 0 ? 0 : (o.refine_iters > 0 ? o.refine_iters : 1);
-  TRY(build_side(c, 0, dR, R.n, dldR, d, dpad, m_r, trials, seed, refine, integral, &Ri));
-  if (self)
-    Qi = Ri;
-  else
-    TRY(build_side(c, 1, dQ, Q.n, dldQ, d, dpad, m_q, trials, seed, refine, integral, &Qi));
-  const int dk = mfma ? (d + 31) & ~31 : 0;
-  void *rc = nullptr;
-  float *nr = nullptr, *reps32 = nullptr, *rn32 = nullptr;
-  if (mfma) {
-    TRY(ws_get(c, "r_rc", (size_t)R.n * dk * 4, &rc));
-    TRY(ws(c, "r_nr", (size_t)R.n, &nr));
-    TRY(ws(c, "r_reps32", (size_t)m_r * dk, &reps32));
-    TRY(ws(c, "r_rn32", (size_t)m_r, &rn32));
-    launch_center(Ri, d, dk, bf16 ? (bf16x1 ? 1 : 2) : 0, rc, nr, reps32, rn32, s);
-    HIPCHK(hipGetLastError());
-  }
-  HIPCHK(hipEventRecord(c->ev[2], s));
-
-  double *theta;
-  int32_t *plist, *pcnt;
-  TRY(ws(c, "theta", (size_t)m_q, &theta));
-  TRY(ws(c, "plist", (size_t)m_q * m_r, &plist));
-  TRY(ws(c, "pcnt", (size_t)m_q, &pcnt));
-  launch_bounds(Qi, Ri, d, k, theta, plist, pcnt, s);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c->ev[3], s));
-
-  // capacity, tile size, tiles
-  int kcap = exact ? pow2_at_least(std::max(k, 64))
-             : bf16x1 ? pow2_at_least(std::max(2 * k, 64))
-                    : pow2_at_least(std::max(k + std::max(8, k / 8), 64));
-  kcap = std::min(kcap, mfma ? 512 : KNNJ_MAX_K);
-  if (kcap < k) kcap = KNNJ_MAX_K;  // only the VALU kernel holds k > 512
-  const bool use_mfma = mfma && kcap <= 512;
-  const int tq = use_mfma ? 16 : (o.tile_q ? std::min(o.tile_q, join_tile_q(kcap)) : join_tile_q(kcap));
-  const int64_t max_tiles = 8 * (Q.n / tq + m_q + 1);  // XCD-aware slots, empty ones included
-  Tile *tiles, *ovf_tiles, *ovf2_tiles;
-  int32_t *ntiles, *counters;
-  u64 *n_evals;
-  TRY(ws(c, "tiles", (size_t)max_tiles, &tiles));
-  TRY(ws(c, "ovf_tiles", (size_t)Q.n, &ovf_tiles));
-  TRY(ws(c, "ovf2_tiles", (size_t)Q.n, &ovf2_tiles));
-  TRY(ws(c, "counters", 8, &counters));  // [0] ntiles, [1] ovf, [2] ovf2, [3] max_err bits, [4] ovf3
-  TRY(ws(c, "evals", 4, &n_evals));
-  ntiles = counters;
-  HIPCHK(hipMemsetAsync(counters, 0, 8 * sizeof(int32_t), s));
-  HIPCHK(hipMemsetAsync(n_evals, 0, 4 * sizeof(u64), s));
-  launch_make_tiles(Qi.off, m_q, pcnt, tq, tiles, ntiles, (int)max_tiles, o.xcd_tiles ? 8 : 1, s);
+  HIPCHK(hipEventRecord(ev[1], s));
+  TRY(enqueue_index(c, I, s, ev[2], ev[3]));
   int32_t nslots = 0;
-  HIPCHK(hipMemcpyAsync(&nslots, ntiles, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&nslots, I.counters, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
 
-  // outputs
-  int32_t *oidx = out_idx;
-  float *odist = out_dist;
-  const size_t nout = (size_t)Q.n * k;
-  if (!Q.on_device) {
-    TRY(ws(c, "out_idx", nout * (aos ? 2 : 1), &oidx));
-    if (!aos) TRY(ws(c, "out_dist", nout, &odist));
-  }
-
-  JoinArgs a{};
-  a.Q = Qi;
-  a.R = Ri;
-  a.d = d;
-  a.k = k;
-  a.kcap = kcap;
-  a.exact = exact ? 1 : 0;
-  const double delta = exact ? 0.0 : (double)(d + 4) * 0x1p-23 + 0x1p-40;
-  a.delta = delta;
-  a.fac = (1.0 + delta) / (1.0 - delta) * (1.0 + 0x1p-20);
-  a.tiles = tiles;
-  a.ntiles = ntiles;
-  a.theta = theta;
-  a.plist = plist;
-  a.pcnt = pcnt;
-  a.out_idx = oidx;
-  a.out_dist = odist;
-  a.out_mode = aos ? 2 : (o.out_squared ? 1 : 0);
-  a.n_evals = n_evals;
-  a.ovf_tiles = ovf_tiles;
-  a.ovf_count = counters + 1;
-  a.max_err_bits = (u32 *)(counters + 3);
-  a.rc = rc;
-  a.nr = nr;
-  // per-entry error codes in the MFMA filter keys: rows must fit 26 bits;
-  // codes span 63 binades below the largest possible squared distance
-  a.ecode = (use_mfma && !exact && R.n < (int64_t)(1 << 26)) ? 1 : 0;
-  {
-    const double smax = std::max(1e-30, (double)d * span * span);
-    a.ebase = std::ilogb(smax) + 1 - 62;
-  }
-  a.reps32 = reps32;
-  a.rn32 = rn32;
+  fill_args(I);
   const bool tile_prof = std::getenv("KNNJ_TILE_PROFILE") != nullptr;
   if (tile_prof) {
     u64 *tp;
-    TRY(ws(c, "tile_prof", (size_t)max_tiles * 4, &tp));
-    HIPCHK(hipMemsetAsync(tp, 0, sizeof(u64) * 4 * max_tiles, s));
-    a.tile_prof = tp;
     float *dg;
+    TRY(ws(c, "tile_prof", (size_t)I.max_slots * 4, &tp));
     TRY(ws(c, "ovf_diag", 256, &dg));
+    HIPCHK(hipMemsetAsync(tp, 0, sizeof(u64) * 4 * I.max_slots, s));
     HIPCHK(hipMemsetAsync(dg, 0, sizeof(float) * 256, s));
-    a.ovf_diag = dg;
+    I.args.tile_prof = tp;
+    I.args.ovf_diag = dg;
   }
-  a.dk = dk;
-  {
-    const double u = 0x1p-24, ub = 0x1p-8;
-    auto gam = [&](double n) { return n * u / (1.0 - n * u); };
-    // |computed dot - (q-c).(r-c)| <= cdot ||q-c|| ||r-c||
-    double cdot = gam(d);                                                     // fp32 operands
-    if (bf16x1) cdot = 2.0 * ub + ub * ub + gam(d) * (1.0 + ub) * (1.0 + ub);  // one bf16 product
-    else if (bf16) cdot = 3.1 * ub * ub + 1.02 * gam(3.0 * d);              // hi/lo, three products
-    a.c1 = (float)((2.0 * cdot + 4.0 * u) * 1.01);
-    a.c2 = (float)((gam(d + 3) + 8.0 * u) * 1.01);
-  }
-  if (use_mfma) {
-    a.exact = exact ? 1 : 0;
-    if (bf16)
-      HIPCHK(launch_join_mfma_bf16(a, bf16x1 ? 1 : 3, nslots, s));
-    else
-      HIPCHK(launch_join_mfma_f32(a, nslots, s));
-  } else {
-    a.exact = (mfma ? exact_valu : exact) ? 1 : 0;
-    HIPCHK(launch_join(a, tq, nslots, s));
-  }
-  HIPCHK(hipEventRecord(c->ev[4], s));
+  HIPCHK(hipEventRecord(ev[8], s));
+  HIPCHK(launch_main(I, nslots, s));
+  HIPCHK(hipEventRecord(ev[4], s));
 
   int32_t hc[4];
-  HIPCHK(hipMemcpyAsync(hc, counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(hc, I.counters, sizeof(hc), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   st.n_tiles = hc[0];
   st.n_overflow = hc[1];
-  if (tile_prof && hc[0] > 0) {
-    // diagnostic: distribution of tile durations (wall clock ticks) and cluster counts
-    std::vector<u64> tp((size_t)hc[0] * 4);
-    HIPCHK(hipMemcpy(tp.data(), a.tile_prof, tp.size() * sizeof(u64), hipMemcpyDeviceToHost));
-    std::vector<double> dur;
-    double listed = 0, visited = 0;
-    u64 t0 = ~0ull, t1 = 0;
-    for (int i = 0; i < hc[0]; ++i) {
-      if (tp[4 * i + 1] == 0) continue;  // empty slot
-      dur.push_back((double)(tp[4 * i + 1] - tp[4 * i]));
-      listed += (double)(tp[4 * i + 3] & 0xFFFFFFFFull);
-      visited += (double)(tp[4 * i + 3] >> 32);
-      t0 = std::min(t0, tp[4 * i]);
-      t1 = std::max(t1, tp[4 * i + 1]);
-    }
-    if (const char *dump = std::getenv("KNNJ_TILE_DUMP")) {
-      if (FILE *f = fopen(dump, "wb")) {
-        fwrite(tp.data(), sizeof(u64), tp.size(), f);
-        fclose(f);
-      }
-    }
-    std::vector<double> sd = dur;
-    std::sort(sd.begin(), sd.end());
-    double sum = 0;
-    for (double v : sd) sum += v;
-    const int nt = (int)sd.size();
-    if (nt > 0)
-    fprintf(stderr,
-            "[knnj tile profile] tiles=%d span=%.0f ticks sum=%.0f mean=%.1f p50=%.0f p99=%.0f max=%.0f "
-            "listed/tile=%.1f visited/tile=%.1f\n",
-            nt, (double)(t1 - t0), sum, sum / nt, sd[nt / 2], sd[(size_t)(0.99 * (nt - 1))], sd.back(),
-            listed / nt, visited / nt);
-    if (hc[1] > 0) {
-      float dg[256];
-      HIPCHK(hipMemcpy(dg, a.ovf_diag, sizeof(dg), hipMemcpyDeviceToHost));
-      for (int i = 0; i < std::min(hc[1], 8); ++i)
-        fprintf(stderr, "[knnj overflow] cutoff=%.6g kth=%.6g last=%.6g first=%.6g\n", dg[4 * i], dg[4 * i + 1],
-                dg[4 * i + 2], dg[4 * i + 3]);
-    }
-  }
-  HIPCHK(hipEventRecord(c->ev[5], s));
-  if (hc[1] > 0) {
-    // overflowed queries, one per tile, at a larger capacity: the MFMA kernel
-    // at 512 if it ran below that, else the VALU kernel at 1024
-    JoinArgs b = a;
-    b.tiles = ovf_tiles;
-    b.ntiles = counters + 1;
-    b.ovf_tiles = ovf2_tiles;
-    b.ovf_count = counters + 2;
-    b.tile_prof = nullptr;
-    b.ovf_diag = nullptr;
-    b.ecode = (b.kcap <= 512 && use_mfma && kcap < 512) ? a.ecode : 0;
-    if (use_mfma && kcap < 512) {
-      b.kcap = 512;
-      if (bf16)
-        HIPCHK(launch_join_mfma_bf16(b, bf16x1 ? 1 : 3, hc[1], s));
-      else
-        HIPCHK(launch_join_mfma_f32(b, hc[1], s));
-    } else {
-      b.kcap = KNNJ_MAX_K;
-      b.exact = exact_valu ? 1 : 0;
-      HIPCHK(launch_join(b, 1, hc[1], s));
-    }
-    int32_t h2 = 0;
-    HIPCHK(hipMemcpyAsync(&h2, counters + 2, sizeof(h2), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (h2 > 0) {
-      // last resort: the VALU kernel at the largest capacity
-      JoinArgs c2 = b;
-      c2.tiles = ovf2_tiles;
-      c2.ntiles = counters + 2;
-      c2.ovf_tiles = ovf_tiles;
-      c2.ovf_count = counters + 4;
-      c2.kcap = KNNJ_MAX_K;
-      c2.exact = exact_valu ? 1 : 0;
-      c2.ecode = 0;
-      if (!(use_mfma && kcap < 512)) {
-        set_err("%d queries overflowed the largest candidate capacity (%d)", h2, KNNJ_MAX_K);
-        return KNNJ_EOVERFLOW;
-      }
-      HIPCHK(launch_join(c2, 1, h2, s));
-      int32_t h3 = 0;
-      HIPCHK(hipMemcpyAsync(&h3, counters + 4, sizeof(h3), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      if (h3 > 0) {
-        set_err("%d queries overflowed the largest candidate capacity (%d)", h3, KNNJ_MAX_K);
-        return KNNJ_EOVERFLOW;
-      }
-    }
-  }
-  HIPCHK(hipEventRecord(c->ev[6], s));
-  if (!Q.on_device) {
-    HIPCHK(hipMemcpyAsync(out_idx, oidx, nout * sizeof(int32_t) * (aos ? 2 : 1), hipMemcpyDeviceToHost, s));
-    if (!aos) HIPCHK(hipMemcpyAsync(out_dist, odist, nout * sizeof(float), hipMemcpyDeviceToHost, s));
-  }
+  st.n_launches = 1;
+  if (tile_prof) TRY(print_tile_profile(I, nslots, hc[1]));
+  HIPCHK(hipEventRecord(ev[5], s));
+  if (hc[1] > 0) TRY(rerun_overflow(I, hc[1], s));
+  HIPCHK(hipEventRecord(ev[6], s));
+  TRY(copy_outputs(I, s));
   u64 hcnt[4] = {0, 0, 0, 0};
   u32 herr = 0;
-  HIPCHK(hipMemcpyAsync(hcnt, n_evals, sizeof(hcnt), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(&herr, counters + 3, sizeof(herr), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipEventRecord(c->ev[7], s));
+  HIPCHK(hipMemcpyAsync(hcnt, I.evals, sizeof(hcnt), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&herr, I.counters + 3, sizeof(herr), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(ev[7], s));
   HIPCHK(hipStreamSynchronize(s));
 
-  st.t_total_ms = elapsed(c->ev[0], c->ev[7]);
-  st.t_index_ms = elapsed(c->ev[1], c->ev[2]);
-  st.t_bounds_ms = elapsed(c->ev[2], c->ev[3]);
-  st.t_join_ms = elapsed(c->ev[3], c->ev[4]);
-  st.t_fallback_ms = hc[1] > 0 ? elapsed(c->ev[5], c->ev[6]) : 0.0;
-  const u64 hev = hcnt[0];
-  st.n_dist_evals = (int64_t)hev;
+  st.t_total_ms = elapsed(ev[0], ev[7]);
+  st.t_index_ms = elapsed(ev[1], ev[2]);
+  st.t_bounds_ms = elapsed(ev[2], ev[3]);
+  st.t_join_ms = elapsed(ev[8], ev[4]);
+  st.t_fallback_ms = hc[1] > 0 ? elapsed(ev[5], ev[6]) : 0.0;
+  st.n_dist_evals = (int64_t)hcnt[0];
   st.n_merges = (int64_t)hcnt[1];
   st.n_appended = (int64_t)hcnt[2];
-  const double tot = (double)R.n * (double)Q.n;
-  st.pruning_ratio = tot > 0 ? 1.0 - (double)hev / tot : 0.0;
-  st.tile_q = tq;
-  st.kcap = kcap;
+  const double tot = (double)I.nR * (double)I.nQ;
+  st.pruning_ratio = tot > 0 ? 1.0 - (double)hcnt[0] / tot : 0.0;
   float ferr;
   std::memcpy(&ferr, &herr, 4);
   st.filter_max_abs_err = ferr;
-  // surviving cluster pairs
   {
-    std::vector<int32_t> hp(m_q);
-    HIPCHK(hipMemcpy(hp.data(), pcnt, sizeof(int32_t) * m_q, hipMemcpyDeviceToHost));
+    std::vector<int32_t> hp(I.m_q);
+    HIPCHK(hipMemcpy(hp.data(), I.pcnt, sizeof(int32_t) * I.m_q, hipMemcpyDeviceToHost));
     int64_t t = 0;
     for (int32_t v : hp) t += v;
     st.n_cluster_pairs = t;
   }
+  if (stats) *stats = st;
+  return KNNJ_OK;
+}
+
+knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *inst, const knnj_opts *opts,
+                            knnj_stats *stats) {
+  g_err.clear();
+  if (!c) {
+    set_err("context is NULL");
+    return KNNJ_EINVAL;
+  }
+  if (n_inst < 0 || (n_inst > 0 && !inst)) {
+    set_err("bad instance array");
+    return KNNJ_EINVAL;
+  }
+  knnj_opts o{};
+  if (opts) o = *opts;
+  knnj_stats st{};
+  if (n_inst == 0) {
+    if (stats) *stats = st;
+    return KNNJ_OK;
+  }
+  for (int i = 0; i < n_inst; ++i) {
+    const knnj_status e = check_join_args(inst[i].R, inst[i].Q, inst[i].k, o, inst[i].out_idx, inst[i].out_dist);
+    if (e != KNNJ_OK) {
+      prefix_err("instance %d: ", i);
+      return e;
+    }
+  }
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  hipEvent_t *ev = c->ev;
+  c->host_cand.clear();
+  HIPCHK(hipEventRecord(ev[0], s));
+  std::vector<Inst> I(n_inst);
+  for (int i = 0; i < n_inst; ++i) {
+    describe(I[i], inst[i].R, inst[i].Q, inst[i].k, o, inst[i].out_idx, inst[i].out_dist);
+    I[i].id = i;
+  }
+
+  // inputs: one arena for the host copies, one flag quadruple per join, one sync
+  {
+    Alloc sz;
+    sz.c = c;
+    sz.arena = true;
+    for (Inst x : I) TRY(stage_inputs(sz, x, s));  // sizing pass on copies, no device work
+    char *base = nullptr;
+    if (sz.off > 0) TRY(ws(c, "batch_inputs", sz.off, &base));
+    Alloc A;
+    A.c = c;
+    A.arena = true;
+    A.base = base;
+    for (Inst &x : I) TRY(stage_inputs(A, x, s));
+  }
+  u32 *flags;
+  TRY(ws(c, "batch_flags", (size_t)4 * n_inst, &flags));
+  HIPCHK(hipMemsetAsync(flags, 0, sizeof(u32) * 4 * n_inst, s));
+  for (const Inst &x : I) {
+    launch_scan_input(x.dR, x.nR, x.d, x.ldR, flags + 4 * x.id, s);
+    if (!x.self && x.nQ > 0) launch_scan_input(x.dQ, x.nQ, x.d, x.ldQ, flags + 4 * x.id, s);
+  }
+  std::vector<u32> hflags((size_t)4 * n_inst);
+  HIPCHK(hipMemcpyAsync(hflags.data(), flags, sizeof(u32) * hflags.size(), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (Inst &x : I) {
+    const knnj_status e = decide(x, &hflags[4 * x.id]);
+    if (e != KNNJ_OK) {
+      prefix_err("instance %d: ", x.id);
+      return e;
+    }
+  }
+
+  // launch groups: joins sharing a kernel instantiation run in one launch,
+  // their tile slots contiguous in one array
+  std::vector<int> order;
+  for (const Inst &x : I)
+    if (x.nQ > 0) order.push_back(x.id);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return I[a].launch_key() < I[b].launch_key(); });
+  int64_t total_slots = 0;
+  std::vector<int64_t> slot0(n_inst, 0);
+  for (int id : order) {
+    slot0[id] = total_slots;
+    total_slots += I[id].max_slots;
+  }
+  if (total_slots > (int64_t)INT32_MAX) {
+    set_err("batch needs %lld tile slots, above 2^31-1", (long long)total_slots);
+    return KNNJ_EUNSUPPORTED;
+  }
+  Tile *all_tiles;
+  TRY(ws(c, "batch_tiles", (size_t)std::max<int64_t>(total_slots, 1), &all_tiles));
+  {
+    // one arena for every join's buffers: a sizing pass, then the carve
+    Alloc sz;
+    sz.c = c;
+    sz.arena = true;
+    for (int id : order) {
+      Inst tmp = I[id];
+      TRY(alloc_inst(sz, tmp, all_tiles + slot0[id]));
+    }
+    char *arena = nullptr;
+    TRY(ws(c, "batch_arena", sz.off, &arena));
+    Alloc A;
+    A.c = c;
+    A.arena = true;
+    A.base = arena;
+    for (int id : order) TRY(alloc_inst(A, I[id], all_tiles + slot0[id]));
+  }
+
+  HIPCHK(hipEventRecord(ev[1], s));
+  for (int id : order) TRY(enqueue_index(c, I[id], s, nullptr, nullptr));
+  HIPCHK(hipEventRecord(ev[2], s));
+  JoinArgs *dargs;
+  TRY(ws(c, "batch_args", (size_t)n_inst, &dargs));
+  std::vector<JoinArgs> hargs(n_inst);
+  for (int id : order) {
+    fill_args(I[id]);
+    hargs[id] = I[id].args;
+  }
+  HIPCHK(hipMemcpyAsync(dargs, hargs.data(), sizeof(JoinArgs) * n_inst, hipMemcpyHostToDevice, s));
+  HIPCHK(hipEventRecord(ev[8], s));
+  int n_launch = 0;
+  for (size_t g = 0; g < order.size();) {
+    size_t h = g;
+    while (h < order.size() && I[order[h]].launch_key() == I[order[g]].launch_key()) ++h;
+    const Inst &P = I[order[g]];
+    const int64_t first = slot0[order[g]];
+    const int64_t last = slot0[order[h - 1]] + I[order[h - 1]].max_slots;
+    const int nslots = (int)(last - first);
+    const Tile *t0 = all_tiles + first;
+    if (P.use_mfma) {
+      if (P.bf16)
+        HIPCHK(launch_join_mfma_bf16_multi(P.args, P.bf16x1 ? 1 : 3, dargs, t0, nslots, s));
+      else
+        HIPCHK(launch_join_mfma_f32_multi(P.args, dargs, t0, nslots, s));
+    } else {
+      HIPCHK(launch_join_multi(P.args, P.tq, dargs, t0, nslots, s));
+    }
+    ++n_launch;
+    g = h;
+  }
+  HIPCHK(hipEventRecord(ev[4], s));
+  // counters of every join in one copy: the counter and evaluation blocks
+  // sit in the arena, gathered here per join
+  std::vector<int32_t> hc((size_t)4 * n_inst, 0);
+  std::vector<u64> hcnt((size_t)4 * n_inst, 0);
+  for (int id : order) {
+    HIPCHK(hipMemcpyAsync(&hc[4 * id], I[id].counters, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&hcnt[4 * id], I[id].evals, 4 * sizeof(u64), hipMemcpyDeviceToHost, s));
+  }
+  HIPCHK(hipStreamSynchronize(s));
+
+  HIPCHK(hipEventRecord(ev[5], s));
+  int64_t novf = 0, evals = 0, merges = 0, appended = 0;
+  double tot = 0.0, ferr = 0.0;
+  for (int id : order) {
+    Inst &x = I[id];
+    if (hc[4 * id + 1] > 0) {
+      const knnj_status e = rerun_overflow(x, hc[4 * id + 1], s);
+      if (e != KNNJ_OK) {
+        prefix_err("instance %d: ", id);
+        return e;
+      }
+    }
+    novf += hc[4 * id + 1];
+    TRY(copy_outputs(x, s));
+    evals += (int64_t)hcnt[4 * id];
+    merges += (int64_t)hcnt[4 * id + 1];
+    appended += (int64_t)hcnt[4 * id + 2];
+    tot += (double)x.nR * (double)x.nQ;
+    float fe;
+    std::memcpy(&fe, &hc[4 * id + 3], 4);
+    ferr = std::max(ferr, (double)fe);
+  }
+  HIPCHK(hipEventRecord(ev[6], s));
+  HIPCHK(hipStreamSynchronize(s));
+  st.t_total_ms = elapsed(ev[0], ev[6]);
+  st.t_index_ms = elapsed(ev[1], ev[2]);  // index builds, centring, bounds and tile lists of all joins
+  st.t_join_ms = elapsed(ev[8], ev[4]);
+  st.t_fallback_ms = novf > 0 ? elapsed(ev[5], ev[6]) : 0.0;
+  st.n_dist_evals = evals;
+  st.n_merges = merges;
+  st.n_appended = appended;
+  st.n_overflow = novf;
+  st.n_tiles = total_slots;
+  st.pruning_ratio = tot > 0 ? 1.0 - (double)evals / tot : 0.0;
+  st.filter_max_abs_err = ferr;
+  if (!order.empty()) plan_stats(I[order[0]], st);
+  st.n_launches = n_launch;
   if (stats) *stats = st;
   return KNNJ_OK;
 }

@@ -527,11 +527,13 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
 // to the XCDs round-robin and all tiles of one cluster go to the same XCD,
 // where they share the reference rows in that XCD's L2 (at the cost of
 // balance when a few clusters are heavy).  nxcd = 1 is the plain order.
-// Slots past the end of a sequence hold empty tiles (count 0); *nslots
-// receives nxcd x the longest sequence.  One workgroup of 1024.
+// Every slot not holding a tile, up to max_slots, holds an empty tile
+// (count 0); *nslots receives nxcd x the longest sequence.  Every tile
+// carries `inst` (the join it belongs to in a batched launch) in its pad
+// field.  One workgroup of 1024.
 __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__ off, int m,
-                                                     const int32_t *__restrict__ pcnt, int tq, Tile *tiles,
-                                                     int32_t *nslots, int max_slots, int nxcd) {
+                                                     const int32_t *__restrict__ pcnt, int tq, int inst,
+                                                     Tile *tiles, int32_t *nslots, int max_slots, int nxcd) {
   __shared__ u64 ord[KNNJ_MAX_REPS];
   __shared__ int32_t part[1024];
   __shared__ int32_t seqlen[8];
@@ -606,12 +608,12 @@ __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__
   int Lmax = 0;
   for (int x = 0; x < NXCD; ++x) Lmax = max(Lmax, seqstart[x + 1] - seqstart[x]);
   // empty slots first, then the tiles
-  for (int e = threadIdx.x; e < NXCD * Lmax && e < max_slots; e += blockDim.x) {
+  for (int e = threadIdx.x; e < max_slots; e += blockDim.x) {
     Tile t;
     t.start = 0;
     t.count = 0;
     t.cluster = 0;
-    t.pad = 0;
+    t.pad = inst;
     tiles[e] = t;
   }
   __syncthreads();
@@ -631,7 +633,7 @@ __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__
         t.start = a0 + qq;
         t.count = min(tq, na - qq);
         t.cluster = c;
-        t.pad = 0;
+        t.pad = inst;
         tiles[slot] = t;
       }
       ++run;
@@ -639,9 +641,10 @@ __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__
   }
 }
 
-void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, Tile *tiles, int32_t *nslots,
-                       int max_slots, int nxcd, hipStream_t s) {
-  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, s, off, m, pcnt, tq, tiles, nslots, max_slots, nxcd);
+void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, int inst, Tile *tiles,
+                       int32_t *nslots, int max_slots, int nxcd, hipStream_t s) {
+  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, s, off, m, pcnt, tq, inst, tiles, nslots, max_slots,
+                     nxcd);
 }
 
 // Members of every cluster in ascending row order (counts already in cnt).

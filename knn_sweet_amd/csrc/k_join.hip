@@ -31,15 +31,12 @@
 #include "join_common.h"
 
 template <int TQ, int KCAP, bool EXACT>
-__global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
+__device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
   extern __shared__ u64 smem[];
   u64 *lists = smem;                       // [TQ][KCAP]
   u64 *bufs = smem + TQ * KCAP;            // [TQ][BCAP]
   u64 *tmp = bufs + TQ * KNNJ_BCAP;        // [KCAP]
   const int lane = threadIdx.x;
-  if ((int)blockIdx.x >= *a.ntiles) return;
-  const Tile tile = a.tiles[blockIdx.x];
-  if (tile.count <= 0) return;
   const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
   u64 nvisit = 0;
   const int A = tile.cluster;
@@ -251,6 +248,24 @@ __global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
   }
 }
 
+template <int TQ, int KCAP, bool EXACT>
+__global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
+  if ((int)blockIdx.x >= *a.ntiles) return;
+  const Tile tile = a.tiles[blockIdx.x];
+  if (tile.count <= 0) return;
+  valu_tile<TQ, KCAP, EXACT>(a, tile);
+}
+
+// Many joins in one launch (see k_join_mfma_multi).
+template <int TQ, int KCAP, bool EXACT>
+__global__ __launch_bounds__(64) void k_join_valu_multi(const JoinArgs *__restrict__ argv,
+                                                        const Tile *__restrict__ tiles, int nslots) {
+  if ((int)blockIdx.x >= nslots) return;
+  const Tile tile = tiles[blockIdx.x];
+  if (tile.count <= 0) return;
+  valu_tile<TQ, KCAP, EXACT>(argv[tile.pad], tile);
+}
+
 int join_tile_q(int kcap) {
   if (kcap <= 64) return 8;
   if (kcap <= 256) return 4;
@@ -263,41 +278,61 @@ size_t join_lds_bytes(int tq, int kcap) {
 }
 
 template <int TQ, int KCAP>
-static hipError_t launch_tq_kcap(const JoinArgs &a, int64_t grid, hipStream_t s) {
+static hipError_t launch_tq_kcap(const JoinArgs &a, int64_t grid, hipStream_t s, const JoinArgs *argv,
+                                 const Tile *tiles) {
   const size_t lds = join_lds_bytes(TQ, KCAP);
-  if (a.exact)
+  if (argv) {
+    if (a.exact)
+      hipLaunchKernelGGL((k_join_valu_multi<TQ, KCAP, true>), dim3((unsigned)grid), dim3(64), lds, s, argv, tiles,
+                         (int)grid);
+    else
+      hipLaunchKernelGGL((k_join_valu_multi<TQ, KCAP, false>), dim3((unsigned)grid), dim3(64), lds, s, argv, tiles,
+                         (int)grid);
+  } else if (a.exact) {
     hipLaunchKernelGGL((k_join_valu<TQ, KCAP, true>), dim3((unsigned)grid), dim3(64), lds, s, a);
-  else
+  } else {
     hipLaunchKernelGGL((k_join_valu<TQ, KCAP, false>), dim3((unsigned)grid), dim3(64), lds, s, a);
+  }
   return hipGetLastError();
 }
 
 template <int KCAP>
-static hipError_t launch_kcap(const JoinArgs &a, int tq, int64_t grid, hipStream_t s) {
+static hipError_t launch_kcap(const JoinArgs &a, int tq, int64_t grid, hipStream_t s, const JoinArgs *argv,
+                              const Tile *tiles) {
   switch (tq) {
     case 8:
-      if constexpr (KCAP <= 256) return launch_tq_kcap<8, KCAP>(a, grid, s);
+      if constexpr (KCAP <= 256) return launch_tq_kcap<8, KCAP>(a, grid, s, argv, tiles);
       break;
     case 4:
-      if constexpr (KCAP <= 256) return launch_tq_kcap<4, KCAP>(a, grid, s);
+      if constexpr (KCAP <= 256) return launch_tq_kcap<4, KCAP>(a, grid, s, argv, tiles);
       break;
     case 2:
-      if constexpr (KCAP <= 512) return launch_tq_kcap<2, KCAP>(a, grid, s);
+      if constexpr (KCAP <= 512) return launch_tq_kcap<2, KCAP>(a, grid, s, argv, tiles);
       break;
     case 1:
-      return launch_tq_kcap<1, KCAP>(a, grid, s);
+      return launch_tq_kcap<1, KCAP>(a, grid, s, argv, tiles);
+  }
+  return hipErrorInvalidValue;
+}
+
+static hipError_t launch_join_any(const JoinArgs &a, int tq, int64_t grid, hipStream_t s, const JoinArgs *argv,
+                                  const Tile *tiles) {
+  if (grid <= 0) return hipSuccess;
+  switch (a.kcap) {
+    case 64: return launch_kcap<64>(a, tq, grid, s, argv, tiles);
+    case 128: return launch_kcap<128>(a, tq, grid, s, argv, tiles);
+    case 256: return launch_kcap<256>(a, tq, grid, s, argv, tiles);
+    case 512: return launch_kcap<512>(a, tq, grid, s, argv, tiles);
+    case 1024: return launch_kcap<1024>(a, tq, grid, s, argv, tiles);
   }
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_join(const JoinArgs &a, int tq, int64_t max_tiles, hipStream_t s) {
-  if (max_tiles <= 0) return hipSuccess;
-  switch (a.kcap) {
-    case 64: return launch_kcap<64>(a, tq, max_tiles, s);
-    case 128: return launch_kcap<128>(a, tq, max_tiles, s);
-    case 256: return launch_kcap<256>(a, tq, max_tiles, s);
-    case 512: return launch_kcap<512>(a, tq, max_tiles, s);
-    case 1024: return launch_kcap<1024>(a, tq, max_tiles, s);
-  }
-  return hipErrorInvalidValue;
+  return launch_join_any(a, tq, max_tiles, s, nullptr, nullptr);
+}
+
+hipError_t launch_join_multi(const JoinArgs &proto, int tq, const JoinArgs *argv, const Tile *tiles, int nslots,
+                             hipStream_t s) {
+  return launch_join_any(proto, tq, nslots, s, argv, tiles);
 }

@@ -85,7 +85,7 @@ struct BOp<3> {
 };
 
 template <int KCAP, bool EXACT, int PREC, int DK>
-__global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
+__device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   // PREC: 0 fp32 operands, 1 bf16, 3 bf16 hi/lo (three products)
   constexpr bool BF16 = PREC != 0;
   constexpr bool SPLIT = PREC == 3;
@@ -101,9 +101,6 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
   float *dqs = fsh + 16;                    // [16 rows][16 cols] screening distances
   float *qsh = dqs + 256;                   // [16 rows][DK] the tile's query rows
   const int lane = threadIdx.x, g = lane >> 4, c16 = lane & 15;
-  if ((int)blockIdx.x >= *a.ntiles) return;
-  const Tile tile = a.tiles[blockIdx.x];
-  if (tile.count <= 0) return;
   const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
   u64 nvisit = 0;
   const int A = tile.cluster, nq = tile.count;
@@ -507,17 +504,43 @@ __global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
   }
 }
 
+template <int KCAP, bool EXACT, int PREC, int DK>
+__global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
+  if ((int)blockIdx.x >= *a.ntiles) return;
+  const Tile tile = a.tiles[blockIdx.x];
+  if (tile.count <= 0) return;
+  mfma_tile<KCAP, EXACT, PREC, DK>(a, tile);
+}
+
+// Many joins in one launch: slot b holds a tile of instance tiles[b].pad,
+// whose arguments are argv[tiles[b].pad].
+template <int KCAP, bool EXACT, int PREC, int DK>
+__global__ __launch_bounds__(64) void k_join_mfma_multi(const JoinArgs *__restrict__ argv,
+                                                        const Tile *__restrict__ tiles, int nslots) {
+  if ((int)blockIdx.x >= nslots) return;
+  const Tile tile = tiles[blockIdx.x];
+  if (tile.count <= 0) return;
+  mfma_tile<KCAP, EXACT, PREC, DK>(argv[tile.pad], tile);
+}
+
 inline size_t mfma_lds(int kcap, int dkmax) {
   return sizeof(u64) * ((size_t)MF_TQ * kcap + (size_t)MF_TQ * MF_BC + (size_t)kcap) +
          (16 + 256 + (size_t)MF_TQ * dkmax) * sizeof(float);
 }
 
 template <int PREC>
-hipError_t launch_mfma_any(const JoinArgs &a, int64_t grid, hipStream_t s) {
+hipError_t launch_mfma_any(const JoinArgs &a, int64_t grid, hipStream_t s, const JoinArgs *argv = nullptr,
+                           const Tile *tiles = nullptr) {
   const bool big = a.dk > 128;  // dk = roundup(d, 32)
   const size_t lds = mfma_lds(a.kcap, big ? 256 : 128);
-#define KJ_LAUNCH(KC, EX, DKV) \
-  hipLaunchKernelGGL((k_join_mfma<KC, EX, PREC, DKV>), dim3((unsigned)grid), dim3(64), lds, s, a)
+#define KJ_LAUNCH(KC, EX, DKV)                                                                               \
+  do {                                                                                                       \
+    if (argv)                                                                                                \
+      hipLaunchKernelGGL((k_join_mfma_multi<KC, EX, PREC, DKV>), dim3((unsigned)grid), dim3(64), lds, s, argv, \
+                         tiles, (int)grid);                                                                  \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_join_mfma<KC, EX, PREC, DKV>), dim3((unsigned)grid), dim3(64), lds, s, a);       \
+  } while (0)
 #define KJ_DK(KC, EX)       \
   if (big)                  \
     KJ_LAUNCH(KC, EX, 256); \

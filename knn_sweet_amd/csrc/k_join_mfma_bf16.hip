@@ -6,3 +6,10 @@ hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles
   if (max_tiles <= 0) return hipSuccess;
   return terms == 3 ? launch_mfma_any<3>(a, max_tiles, s) : launch_mfma_any<1>(a, max_tiles, s);
 }
+
+hipError_t launch_join_mfma_bf16_multi(const JoinArgs &proto, int terms, const JoinArgs *argv, const Tile *tiles,
+                                       int nslots, hipStream_t s) {
+  if (nslots <= 0) return hipSuccess;
+  return terms == 3 ? launch_mfma_any<3>(proto, nslots, s, argv, tiles)
+                    : launch_mfma_any<1>(proto, nslots, s, argv, tiles);
+}

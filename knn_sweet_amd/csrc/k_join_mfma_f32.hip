@@ -7,3 +7,9 @@ hipError_t launch_join_mfma_f32(const JoinArgs &a, int64_t max_tiles, hipStream_
   if (max_tiles <= 0) return hipSuccess;
   return launch_mfma_any<0>(a, max_tiles, s);
 }
+
+hipError_t launch_join_mfma_f32_multi(const JoinArgs &proto, const JoinArgs *argv, const Tile *tiles, int nslots,
+                                      hipStream_t s) {
+  if (nslots <= 0) return hipSuccess;
+  return launch_mfma_any<0>(proto, nslots, s, argv, tiles);
+}

@@ -24,9 +24,10 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
                                 const int32_t *cl, const int32_t *cnt, int32_t *off, int32_t *fill,
                                 int32_t *perm, const float *d2r, float *pdist, u64 *gkeys, float *px,
                                 hipStream_t s);
-// XCD-aware tile slots; *nslots = slots used (empty tiles have count 0)
-void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, Tile *tiles, int32_t *nslots,
-                       int max_slots, int nxcd, hipStream_t s);
+// XCD-aware tile slots; *nslots = slots used (empty tiles have count 0);
+// every slot's pad field = inst
+void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, int inst, Tile *tiles,
+                       int32_t *nslots, int max_slots, int nxcd, hipStream_t s);
 
 // k_bounds.hip
 void launch_bounds(const SideIndex &Qi, const SideIndex &Ri, int d, int k, double *theta, int32_t *plist,
@@ -82,3 +83,13 @@ hipError_t launch_join_mfma_f32(const JoinArgs &a, int64_t max_tiles, hipStream_
 // terms: 1 single bf16 product, 3 compensated hi/lo products
 hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles, hipStream_t s);
 size_t join_mfma_lds_bytes(int kcap);
+
+// Fused launches over the tiles of many instances: slot b runs tile tiles[b]
+// of instance tiles[b].pad with arguments argv[tiles[b].pad]; proto carries
+// the template selectors (kcap, exact, dk) shared by all those instances.
+hipError_t launch_join_multi(const JoinArgs &proto, int tq, const JoinArgs *argv, const Tile *tiles, int nslots,
+                             hipStream_t s);
+hipError_t launch_join_mfma_f32_multi(const JoinArgs &proto, const JoinArgs *argv, const Tile *tiles, int nslots,
+                                      hipStream_t s);
+hipError_t launch_join_mfma_bf16_multi(const JoinArgs &proto, int terms, const JoinArgs *argv, const Tile *tiles,
+                                       int nslots, hipStream_t s);

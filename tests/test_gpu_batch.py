@@ -1,0 +1,105 @@
+"""GPU parity of the batched join (knnj_join_batch): every instance's result
+is bit-identical to the same join run alone (knnj_join) and to the FP64
+oracle, whatever the mix of sizes, dimensions, k, kernels and precisions
+that share the fused launches."""
+import numpy as np
+import pytest
+
+import knn_sweet_amd as kj
+from knn_sweet_amd import datagen
+from helpers import assert_matches_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = kj.Context(0)
+    yield c
+    c.close()
+
+
+def _dev(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+def _host(t):
+    return t.cpu().numpy() if hasattr(t, "cpu") else np.asarray(t)
+
+
+def _assert_same(a, b, what):
+    a, b = _host(a), _host(b)
+    assert a.shape == b.shape, what
+    if a.size == 0:
+        return
+    bad = np.nonzero((a != b).reshape(a.shape[0], -1).any(axis=1))[0]
+    assert bad.size == 0, f"{what}: {bad.size} rows differ, first {bad[0]}"
+
+
+def test_batch_uneven_mixture_matches_alone_and_oracle(ctx):
+    """Small version of the batched workload: uneven |R_i|, |Q_i| = |R_i|/4,
+    d_i in {16, 32, 96}, 8-64 Gaussians each, k = 16."""
+    inst = datagen.batch_instances(n_inst=10, seed=91, r_lo=300, r_hi=12000)
+    dev = [(_dev(R), _dev(Q), 16) for R, Q in inst]
+    res, st = ctx.join_batch(dev)
+    assert st["n_launches"] <= 3  # one per kernel instantiation in the mix
+    for i, ((R, Q), (idx, dist), (Rd, Qd, k)) in enumerate(zip(inst, res, dev)):
+        a_idx, a_dist, _ = ctx.join(Rd, Qd, k)
+        _assert_same(idx, a_idx, f"instance {i} idx (d={R.shape[1]})")
+        _assert_same(dist, a_dist, f"instance {i} dist")
+        assert_matches_oracle(R, Q, k, _host(idx), _host(dist))
+
+
+def test_batch_mixed_k_kernels_precisions_and_hosts(ctx):
+    """Different k, integer and real inputs, VALU / MFMA-FP32 / BF16 filter,
+    a self-join, host and device buffers, an empty query set."""
+    X = datagen.skin_like(3000, seed=92)
+    m96 = datagen.mixture64_model(seed=93, d=96, comps=20)
+    R96, Q96 = m96.sample(datagen.rng(1), 5000), m96.sample(datagen.rng(2), 700)
+    m32 = datagen.mixture64_model(seed=94, d=32, comps=12)
+    R32, Q32 = m32.sample(datagen.rng(3), 4000), m32.sample(datagen.rng(4), 333)
+    m5 = datagen.mixture64_model(seed=95, d=5, comps=9)
+    R5, Q5 = m5.sample(datagen.rng(5), 2000), m5.sample(datagen.rng(6), 100)
+    Xd = _dev(X)
+    cases = [
+        (Xd, Xd, 200),                      # integer self-join, k = 200
+        (R96, Q96, 100),                    # host, BF16 filter, k = 100
+        (_dev(R96), _dev(Q96), 1),          # device, k = 1
+        (_dev(R32), _dev(Q32), 16),         # MFMA FP32
+        (R5, Q5, 37),                       # VALU, real
+        (_dev(R5), _dev(Q5[:0]), 3),        # empty Q
+        (_dev(R32), _dev(Q32), 600),        # k above the MFMA capacity: VALU at 1024
+    ]
+    res, st = ctx.join_batch(cases)
+    oracle_inputs = [(X, X), (R96, Q96), (R96, Q96), (R32, Q32), (R5, Q5), (R5, Q5[:0]), (R32, Q32)]
+    for i, (case, (idx, dist), (R, Q)) in enumerate(zip(cases, res, oracle_inputs)):
+        a_idx, a_dist, _ = ctx.join(*case)
+        _assert_same(idx, a_idx, f"case {i} idx")
+        _assert_same(dist, a_dist, f"case {i} dist")
+        if Q.shape[0]:
+            assert_matches_oracle(R, Q, case[2], _host(idx), _host(dist))
+
+
+def test_batch_repeated_calls_reuse_workspace(ctx):
+    """Two batches in a row, the second smaller: no state leaks between calls."""
+    big = datagen.batch_instances(n_inst=6, seed=96, r_lo=2000, r_hi=9000)
+    small = datagen.batch_instances(n_inst=3, seed=97, r_lo=300, r_hi=1500)
+    for inst in (big, small):
+        res, _ = ctx.join_batch([(_dev(R), _dev(Q), 16) for R, Q in inst])
+        for (R, Q), (idx, dist) in zip(inst, res):
+            assert_matches_oracle(R, Q, 16, _host(idx), _host(dist))
+
+
+def test_batch_errors_name_the_instance(ctx):
+    R = np.zeros((10, 4), np.float32)
+    with pytest.raises(kj.KnnjError) as e:
+        ctx.join_batch([(R, R, 3), (R, R, 11)])
+    assert e.value.status == kj.KNNJ_EINVAL and "instance 1" in str(e.value)
+    bad = R.copy()
+    bad[3, 2] = np.nan
+    with pytest.raises(kj.KnnjError) as e:
+        ctx.join_batch([(R, R, 3), (bad, R, 2)])
+    assert "instance 1" in str(e.value) and "non-finite" in str(e.value)
+    res, st = ctx.join_batch([])
+    assert res == [] and st["n_dist_evals"] == 0
