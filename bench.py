@@ -1,18 +1,25 @@
 """Benchmark of the exact kNN join on one MI355X.
 
-    python bench.py [--config c2] [--steps K] [--warmup W] [--gpus 1]
+    python bench.py [--config c3] [--k 100] [--steps K] [--warmup W] [--gpus 1]
 
-A step is one whole exact join (index build + bounds + join + rerank) of
-the configuration's query set against its reference set, inputs already
+A step is one whole exact join (index build + bounds + join + rerank) of the
+configuration's query set against its reference set, inputs already
 resident in HBM.  `value` is queries per second over the K timed steps.
 Workloads (seeded synthetic data, knn_sweet_amd.datagen):
-  c1  self-join of 245,057 integer points, d=3, k=200
+  c1  self-join of 245,057 integer points, d=3, k=200 (plain FP32, exact keys)
   c2  |R|=1,000,000, |Q|=100,000, d=64, k=32, FP32 Gaussian mixture
-  c3  |R|=2,000,000, |Q|=200,000, d=128, k=100, FP32 Gaussian mixture
+  c3  |R|=2,000,000, |Q|=200,000, d=128, k=100 (or 10), FP32 Gaussian mixture:
+      the BF16-filter join timed as the headline, the plain FP32 join of the
+      same inputs timed beside it (`plain_fp32`, `bf16_vs_fp32`)
+  c4  512 independent joins, |R_i| log-uniform in [1e3, 2e5], |Q_i| = |R_i|/4,
+      d_i in {16, 32, 96}, k=16: one batched call per step, the per-instance
+      loop timed beside it
 The roofline object describes the join kernel (the dominant one); its
-duration is the hipEvent time the library records around it on its own
-stream.  cpu_baseline times the FP64 brute-force oracle on a bounded sample
-of the same queries on this host's cores.
+duration is the hipEvent time the library records around its launch on its
+own stream.  `traffic` is the HBM bytes per launch recorded from rocprofv3
+FETCH_SIZE + WRITE_SIZE runs in profiles/hbm_traffic.json (null if that
+workload has no record).  cpu_baseline times the FP64 brute-force oracle on
+a bounded sample of the same queries on this host's cores.
 """
 from __future__ import annotations
 
@@ -31,19 +38,26 @@ METRIC = "exact kNN queries/s/GPU (BF16-filter vs plain FP32, recall=1.0); p99 m
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
+PREC_NAMES = {1: "fp32", 2: "bf16x3-filter+fp64-rerank", 4: "bf16x1-filter+fp64-rerank"}
+KERNEL_NAMES = {1: "valu", 2: "mfma"}
 
 
-def load_config(name: str):
+def load_config(name: str, k: int | None):
     from knn_sweet_amd import datagen
     if name == "c1":
         X = datagen.skin_like(245_057, seed=2016)
-        return dict(workload="selfjoin-int-d3-245057-k200", R=X, Q=X, k=200, self_join=True)
+        return dict(workload="selfjoin-int-d3-245057-k200", R=X, Q=X, k=k or 200, self_join=True)
     if name == "c2":
         R, Q = datagen.mixture64(1_000_000, 100_000, seed=2017)
-        return dict(workload="mixture-d64-R1M-Q100k-k32", R=R, Q=Q, k=32, self_join=False)
+        k = k or 32
+        return dict(workload=f"mixture-d64-R1M-Q100k-k{k}", R=R, Q=Q, k=k, self_join=False)
     if name == "c3":
         R, Q = datagen.mixture128(2_000_000, 200_000, seed=2018)
-        return dict(workload="mixture-d128-R2M-Q200k-k100", R=R, Q=Q, k=100, self_join=False)
+        k = k or 100
+        return dict(workload=f"mixture-d128-R2M-Q200k-k{k}", R=R, Q=Q, k=k, self_join=False)
+    if name == "c4":
+        inst = datagen.batch_instances(n_inst=512, seed=2019)
+        return dict(workload="batch-512-uneven-d16-32-96-k16", inst=inst, k=k or 16)
     raise SystemExit(f"unknown config {name}")
 
 
@@ -66,25 +80,67 @@ def cpu_baseline(R, Q, k, target_s: float = 12.0):
             "sample": f"first {n} queries of the same workload, FP64 brute force (oracle), {dt:.1f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--brute-force", action="store_true", help="one cluster per side")
-    ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma"])
-    ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16", "bf16x1"])
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
-        sys.exit(2)
+def traffic_record(workload: str, precision: str):
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(f"{workload}/{precision}")
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else rec.get("bytes_per_launch")
 
+
+def copy_probe(nbytes: int = 1 << 30, reps: int = 3):
+    """A plain device copy of known size, for scaling HBM counters."""
     import torch
-    import knn_sweet_amd as kj
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), file=sys.stderr)
 
-    cfg = load_config(args.config)
+
+def time_joins(ctx, Rd, Qd, k, steps, warmup, opts):
+    import torch
+    for _ in range(warmup):
+        ctx.join(Rd, Qd, k, **opts)
+    torch.cuda.synchronize()
+    times, stats = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        _, _, st = ctx.join(Rd, Qd, k, **opts)
+        times.append((time.perf_counter() - s0) * 1e3)
+        stats.append(st)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, times, stats
+
+
+def roofline(kj, stats, nq, nr, d, k):
+    t_join = float(np.mean([s["t_join_ms"] for s in stats]))
+    evals = float(np.mean([s["n_dist_evals"] for s in stats]))
+    st = stats[-1]
+    if st["exact_mode"] and d <= 8:
+        # compulsory bytes: both point sets once, the k (index, distance) pairs out
+        b_alg = 4.0 * d * (nq + nr) + 8.0 * k * nq
+        achieved = b_alg / (t_join * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None}, t_join, evals
+    # executed distance flops: 2 d_pad per evaluated (query, reference) pair,
+    # d_pad = the kernel's padded width (32 on the matrix cores, 4 on VALU)
+    mfma = st["kernel_used"] == kj.KERNEL_MFMA
+    dpad = (d + 31) // 32 * 32 if mfma else (d + 3) // 4 * 4
+    flops = evals * 2.0 * dpad
+    bf16 = st["precision_used"] in (kj.PREC_BF16, kj.PREC_BF16X1)
+    peak = BF16_PEAK_TFLOPS if bf16 else FP32_PEAK_TFLOPS
+    achieved = flops / (t_join * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": None}, t_join, evals
+
+
+def bench_single(args, kj, cfg):
+    import torch
     R, Q, k = cfg["R"], cfg["Q"], cfg["k"]
     Rd = torch.from_numpy(R).cuda()
     Qd = Rd if cfg["self_join"] else torch.from_numpy(Q).cuda()
@@ -94,48 +150,20 @@ def main():
                 kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel],
                 precision={"auto": kj.PREC_AUTO, "fp32": kj.PREC_FP32, "bf16": kj.PREC_BF16,
                            "bf16x1": kj.PREC_BF16X1}[args.precision])
-    for _ in range(args.warmup):
-        ctx.join(Rd, Qd, k, **opts)
-    torch.cuda.synchronize()
-    times, stats = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        _, _, st = ctx.join(Rd, Qd, k, **opts)
-        times.append((time.perf_counter() - s0) * 1e3)
-        stats.append(st)
-    torch.cuda.synchronize()
-    total = time.perf_counter() - t0
+    total, times, stats = time_joins(ctx, Rd, Qd, k, args.steps, args.warmup, opts)
     nq, nr, d = Q.shape[0], R.shape[0], R.shape[1]
     value = nq * args.steps / total
-
-    t_join = float(np.mean([s["t_join_ms"] for s in stats]))
-    evals = float(np.mean([s["n_dist_evals"] for s in stats]))
+    roof, t_join, evals = roofline(kj, stats, nq, nr, d, k)
     st = stats[-1]
-    if st["exact_mode"]:
-        b_alg = 4.0 * d * (nq + nr) + 8.0 * k * nq
-        achieved = b_alg / (t_join * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None}
-    else:
-        # executed flops per (query, candidate) pair: 3d for the direct form
-        # (d subtractions, d FMAs), 2d for the GEMM form on the matrix cores
-        mfma = st["kernel_used"] == kj.KERNEL_MFMA
-        flops = evals * (2.0 if mfma else 3.0) * d
-        peak = BF16_PEAK_TFLOPS if st["precision_used"] in (kj.PREC_BF16, kj.PREC_BF16X1) else FP32_PEAK_TFLOPS
-        achieved = flops / (t_join * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None}
+    prec = PREC_NAMES[st["precision_used"]]
+    roof["traffic"] = traffic_record(cfg["workload"], prec)
     out = {
         "metric": METRIC, "value": value, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": total * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16" if "bf16" in prec else "f32",
         "data": "synthetic (seeded Gaussian mixture, knn_sweet_amd.datagen)",
-        "config": {"workload": cfg["workload"], "n_r": nr, "n_q": nq, "d": d, "k": k,
-                   "precision": {1: "fp32", 2: "bf16x3-filter+fp64-rerank",
-                                 4: "bf16x1-filter+fp64-rerank"}[st["precision_used"]],
-                   "kernel": {1: "valu", 2: "mfma"}[st["kernel_used"]],
-                   "self_join": cfg["self_join"]},
+        "config": {"workload": cfg["workload"], "n_r": nr, "n_q": nq, "d": d, "k": k, "precision": prec,
+                   "kernel": KERNEL_NAMES[st["kernel_used"]], "self_join": cfg["self_join"]},
         "p50_ms": float(np.percentile(times, 50)), "p99_ms": float(np.percentile(times, 99)),
         "roofline": roof,
         "stages_ms": {"index": st["t_index_ms"], "bounds": st["t_bounds_ms"], "join": t_join,
@@ -146,8 +174,100 @@ def main():
         "exact_mode": st["exact_mode"], "merges": st["n_merges"], "appended": st["n_appended"],
         "filter_max_abs_err": st["filter_max_abs_err"],
     }
+    if args.config == "c3" and args.precision == "auto" and not args.no_compare:
+        # the plain FP32 join of the same inputs, timed the same way
+        fsteps = max(2, args.steps // 2)
+        ftotal, ftimes, fstats = time_joins(ctx, Rd, Qd, k, fsteps, 1, dict(opts, precision=kj.PREC_FP32))
+        fvalue = nq * fsteps / ftotal
+        froof, ft_join, _ = roofline(kj, fstats, nq, nr, d, k)
+        out["plain_fp32"] = {"value": fvalue, "steps": fsteps, "ms_per_step": ftotal * 1e3 / fsteps,
+                             "p99_ms": float(np.percentile(ftimes, 99)), "join_ms": ft_join,
+                             "roofline_frac": froof["frac"], "achieved_tflops": froof["achieved"]}
+        out["bf16_vs_fp32"] = value / fvalue
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(R, Q, k)
+    return out
+
+
+def bench_batch(args, kj, cfg):
+    import torch
+    inst, k = cfg["inst"], cfg["k"]
+    dev = [(torch.from_numpy(R).cuda(), torch.from_numpy(Q).cuda(), k) for R, Q in inst]
+    torch.cuda.synchronize()
+    ctx = kj.Context(0)
+    for _ in range(args.warmup):
+        ctx.join_batch(dev)
+    torch.cuda.synchronize()
+    times, stats = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        _, st = ctx.join_batch(dev)
+        times.append((time.perf_counter() - s0) * 1e3)
+        stats.append(st)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    nq = sum(Q.shape[0] for _, Q in inst)
+    nr = sum(R.shape[0] for R, _ in inst)
+    value = nq * args.steps / total
+    st = stats[-1]
+    t_join = float(np.mean([s["t_join_ms"] for s in stats]))
+    out = {
+        "metric": METRIC, "value": value, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": total * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Gaussian mixtures, knn_sweet_amd.datagen)",
+        "config": {"workload": cfg["workload"], "instances": len(inst), "sum_n_r": nr, "sum_n_q": nq, "k": k},
+        "p50_ms": float(np.percentile(times, 50)), "p99_ms": float(np.percentile(times, 99)),
+        "stages_ms": {"index_bounds_tiles": st["t_index_ms"], "join": t_join, "fallback": st["t_fallback_ms"],
+                      "total": st["t_total_ms"]},
+        "fused_launches": st["n_launches"], "pruning_ratio": st["pruning_ratio"],
+        "dist_evals": st["n_dist_evals"], "n_overflow": st["n_overflow"],
+    }
+    # the per-instance loop over the same device inputs
+    lsteps = max(1, args.steps // 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(lsteps):
+        for Rd, Qd, kk in dev:
+            ctx.join(Rd, Qd, kk)
+    torch.cuda.synchronize()
+    lt = time.perf_counter() - t0
+    out["per_instance_loop"] = {"value": nq * lsteps / lt, "ms_per_step": lt * 1e3 / lsteps, "steps": lsteps}
+    out["batched_vs_loop"] = value / out["per_instance_loop"]["value"]
+    if not args.no_cpu:
+        R0, Q0 = max(inst, key=lambda p: p[0].shape[0])
+        cb = cpu_baseline(R0, Q0, k, target_s=6.0)
+        cb["sample"] = "largest instance: " + cb["sample"]
+        out["cpu_baseline"] = cb
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--k", type=int, default=None, help="override the workload's k")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-compare", action="store_true", help="skip the plain FP32 comparison (c3)")
+    ap.add_argument("--brute-force", action="store_true", help="one cluster per side")
+    ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma"])
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16", "bf16x1"])
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device copy first (HBM counter scale for rocprofv3 --pmc)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "a design, not a measured path)"}))
+        sys.exit(2)
+
+    import knn_sweet_amd as kj
+    if args.copy_probe:
+        copy_probe()
+    cfg = load_config(args.config, args.k)
+    out = bench_batch(args, kj, cfg) if args.config == "c4" else bench_single(args, kj, cfg)
     print(json.dumps(out))
 
 
