@@ -146,7 +146,7 @@ def bench_single(args, kj, cfg):
     Qd = Rd if cfg["self_join"] else torch.from_numpy(Q).cuda()
     torch.cuda.synchronize()
     ctx = kj.Context(0)
-    opts = dict(brute_force=args.brute_force,
+    opts = dict(brute_force=args.brute_force, tile_waves=args.tile_waves,
                 kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel],
                 precision={"auto": kj.PREC_AUTO, "fp32": kj.PREC_FP32, "bf16": kj.PREC_BF16,
                            "bf16x1": kj.PREC_BF16X1}[args.precision])
@@ -255,6 +255,7 @@ def main():
     ap.add_argument("--brute-force", action="store_true", help="one cluster per side")
     ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma"])
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16", "bf16x1"])
+    ap.add_argument("--tile-waves", type=int, default=0, help="MFMA join: waves per workgroup tile (0 = auto)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter scale for rocprofv3 --pmc)")
     args = ap.parse_args()

@@ -89,7 +89,10 @@ typedef struct {
   int32_t refine_iters;/* Lloyd steps moving each representative to its members' mean
                           (0 = default 1, negative = keep the sampled points) */
   int32_t xcd_tiles;   /* 1: place all tiles of a query cluster on one XCD (shared L2) */
-  int32_t reserved[3];
+  int32_t tile_waves;  /* MFMA kernel: waves per workgroup tile, 0 = auto, 1 = one 16-query
+                          wave, > 1 = the most waves whose LDS fits (2 or 4), walking the
+                          reference clusters in lockstep */
+  int32_t reserved[2];
 } knnj_opts;
 
 /* Device-side timings (hipEvents on the context's stream) and counters. */

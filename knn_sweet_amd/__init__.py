@@ -47,7 +47,8 @@ class Opts(ctypes.Structure):
                 ("out_squared", ctypes.c_int32), ("out_aos", ctypes.c_int32),
                 ("brute_force", ctypes.c_int32), ("tile_q", ctypes.c_int32),
                 ("kernel", ctypes.c_int32), ("refine_iters", ctypes.c_int32),
-                ("xcd_tiles", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("xcd_tiles", ctypes.c_int32), ("tile_waves", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
 
 class Stats(ctypes.Structure):
@@ -145,9 +146,10 @@ def _outputs(nq: int, k: int, aos: bool, qdev):
 
 
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
-              brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0, xcd_tiles=False) -> Opts:
+              brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0, xcd_tiles=False,
+              tile_waves=0) -> Opts:
     return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
-                tile_q, kernel, refine_iters, int(xcd_tiles))
+                tile_q, kernel, refine_iters, int(xcd_tiles), tile_waves)
 
 
 class Context:

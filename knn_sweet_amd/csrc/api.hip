@@ -226,6 +226,10 @@ knnj_status check_join_args(const knnj_matrix &R, const knnj_matrix &Q, int32_t 
     set_err("tile_q must be 0, 1, 2, 4 or 8");
     return KNNJ_EINVAL;
   }
+  if (o.tile_waves < 0 || o.tile_waves > 16) {
+    set_err("tile_waves must be in [0, 16]");
+    return KNNJ_EINVAL;
+  }
   if (o.kernel < 0 || o.kernel > KNNJ_KERNEL_MFMA) {
     set_err("unknown kernel %d", o.kernel);
     return KNNJ_EINVAL;
@@ -265,7 +269,7 @@ struct Inst {
   float *out_dist = nullptr;
   bool out_host = false, aos = false;
   // decisions
-  int kind = 0, kcap = 0, tq = 0, dpad = 0, dk = 0, m_r = 0, m_q = 0, trials = 10, refine = 1;
+  int kind = 0, kcap = 0, tq = 0, dpad = 0, dk = 0, m_r = 0, m_q = 0, trials = 10, refine = 1, waves = 1;
   bool mfma = false, bf16 = false, bf16x1 = false, exact = false, exact_valu = false, integral = false;
   bool use_mfma = false;
   double span = 0.0;
@@ -287,7 +291,7 @@ struct Inst {
   int kernel_exact() const { return use_mfma ? (exact ? 1 : 0) : ((mfma ? exact_valu : exact) ? 1 : 0); }
   // kernel instantiation: joins with equal keys can share one launch
   std::tuple<int, int, int, int, int> launch_key() const {
-    return std::make_tuple(use_mfma ? (bf16 ? (bf16x1 ? 3 : 2) : 1) : 0, kcap, kernel_exact(), use_mfma ? 0 : tq,
+    return std::make_tuple(use_mfma ? (bf16 ? (bf16x1 ? 3 : 2) : 1) : 0, kcap, kernel_exact(), tq,
                            use_mfma && dk > 128 ? 1 : 0);
   }
 };
@@ -384,7 +388,9 @@ knnj_status decide(Inst &I, const u32 *flags) {
   if (kcap < k) kcap = KNNJ_MAX_K;  // only the VALU kernel holds k > 512
   I.kcap = kcap;
   I.use_mfma = I.mfma && kcap <= 512;
-  I.tq = I.use_mfma ? 16 : (o.tile_q ? std::min(o.tile_q, join_tile_q(kcap)) : join_tile_q(kcap));
+  I.waves = o.tile_waves > 1 ? o.tile_waves : 1;  // auto: one wave per tile
+  I.tq = I.use_mfma ? join_mfma_tile_q(kcap, I.dk, I.waves)
+                   : (o.tile_q ? std::min(o.tile_q, join_tile_q(kcap)) : join_tile_q(kcap));
   // XCD-aware slots, empty ones included
   I.max_slots = (int64_t)(o.xcd_tiles ? 8 : 1) * (I.nQ / I.tq + I.m_q + 1);
   return KNNJ_OK;
@@ -513,7 +519,7 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
   if (ev_bounds) HIPCHK(hipEventRecord(ev_bounds, s));
   HIPCHK(hipMemsetAsync(I.counters, 0, 8 * sizeof(int32_t), s));
   HIPCHK(hipMemsetAsync(I.evals, 0, 4 * sizeof(u64), s));
-  launch_make_tiles(I.Qi.off, I.m_q, I.pcnt, I.tq, I.id, I.tiles, I.counters, (int)I.max_slots,
+  launch_make_tiles(I.Qi.off, I.m_q, I.theta, I.tq, I.id, I.tiles, I.counters, (int)I.max_slots,
                     I.o.xcd_tiles ? 8 : 1, s);
   HIPCHK(hipGetLastError());
   return KNNJ_OK;
@@ -528,6 +534,7 @@ void fill_args(Inst &I) {
   a.d = d;
   a.k = I.k;
   a.kcap = I.kcap;
+  a.waves = I.waves;
   a.exact = I.kernel_exact();
   const double delta = I.exact ? 0.0 : (double)(d + 4) * 0x1p-23 + 0x1p-40;
   a.delta = delta;
@@ -585,6 +592,7 @@ knnj_status rerun_overflow(Inst &I, int32_t novf, hipStream_t s) {
   b.ovf_count = I.counters + 2;
   b.tile_prof = nullptr;
   b.ovf_diag = nullptr;
+  b.waves = 1;  // one-query tiles
   const bool mfma_up = I.use_mfma && I.kcap < 512;
   if (mfma_up) {
     b.kcap = 512;

@@ -521,8 +521,9 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
 // ---------------------------------------------------------------------------
 // Query tiles: cluster A of the query side is cut into ceil(|A| / tq) tiles
 // of consecutive permuted queries, clusters taken in descending order of
-// their surviving-pair count (the work estimate) so the longest tiles start
-// first.  With nxcd = 8 the placement is XCD-aware: workgroup b runs on XCD
+// their k-th neighbour bound theta_A (the work estimate: a query's search
+// radius, hence the number of reference rows it must visit, grows with it)
+// so the longest tiles start first.  With nxcd = 8 the placement is XCD-aware: workgroup b runs on XCD
 // b % 8, so tile slot 8 i + x is the i-th tile of XCD x; clusters are dealt
 // to the XCDs round-robin and all tiles of one cluster go to the same XCD,
 // where they share the reference rows in that XCD's L2 (at the cost of
@@ -532,7 +533,7 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
 // carries `inst` (the join it belongs to in a batched launch) in its pad
 // field.  One workgroup of 1024.
 __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__ off, int m,
-                                                     const int32_t *__restrict__ pcnt, int tq, int inst,
+                                                     const double *__restrict__ theta, int tq, int inst,
                                                      Tile *tiles, int32_t *nslots, int max_slots, int nxcd) {
   __shared__ u64 ord[KNNJ_MAX_REPS];
   __shared__ int32_t part[1024];
@@ -541,7 +542,7 @@ __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__
   int P = 1;
   while (P < m) P <<= 1;
   for (int c = threadIdx.x; c < P; c += blockDim.x)
-    ord[c] = c < m ? (((u64)(0xFFFFFFFFu - (u32)pcnt[c]) << 32) | (u32)c) : KEY_MAX;
+    ord[c] = c < m ? (((u64)(0xFFFFFFFFu - __float_as_uint((float)theta[c])) << 32) | (u32)c) : KEY_MAX;
   __syncthreads();
   for (int size = 2; size <= P; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -641,9 +642,9 @@ __global__ __launch_bounds__(1024) void k_make_tiles(const int32_t *__restrict__
   }
 }
 
-void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, int inst, Tile *tiles,
+void launch_make_tiles(const int32_t *off, int m, const double *theta, int tq, int inst, Tile *tiles,
                        int32_t *nslots, int max_slots, int nxcd, hipStream_t s) {
-  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, s, off, m, pcnt, tq, inst, tiles, nslots, max_slots,
+  hipLaunchKernelGGL(k_make_tiles, dim3(1), dim3(1024), 0, s, off, m, theta, tq, inst, tiles, nslots, max_slots,
                      nxcd);
 }
 

@@ -1,10 +1,15 @@
 // k_join_mfma.h -- the join proper on the matrix cores (d >= 32).
 //
 // Same contract as k_join_valu (k_join.hip), with the distance tile in GEMM
-// form on MFMA.  One wave owns 16 queries of one query cluster (the 16 rows
-// of a 16x16 MFMA block) and streams the surviving reference clusters 16
-// candidates (columns) at a time.  For reference cluster b with
-// representative c:
+// form on MFMA.  A workgroup of W waves owns 16 W consecutive queries of one
+// query cluster; each wave owns 16 of them (the 16 rows of a 16x16 MFMA
+// block) and streams the surviving reference clusters 16 candidates
+// (columns) at a time.  The W waves walk the union of their cluster lists in
+// one order with a workgroup barrier per cluster, so the reference rows one
+// wave fetches are still in L2 when the others read them: W x fewer HBM
+// reads of reference rows than independent 16-query tiles.  Everything else
+// (thresholds, k-select lists, merges) is private to a wave and ordered with
+// wave-level syncs only.  For reference cluster b with representative c:
 //     s(q, r) = ||q-c||^2 + ||r-c||^2 - 2 (q-c).(r-c)
 // q-c is formed per (tile, cluster) into the A operand registers, with
 // ||q-c||^2 in binary32; r-c and ||r-c||^2 are precomputed (k_center.hip).
@@ -68,6 +73,19 @@ __device__ __forceinline__ D make_drop(int eb) {
 #define MF_TQ 16
 #define MF_BC 64  // per-query candidate buffer
 
+// LDS of one wave (u64 words): lists, buffers, merge scratch, then floats:
+// row norms [16], screening distances [16][16], query rows [16][DK]
+__host__ __device__ constexpr size_t mfma_wave_words(int kcap, int dk) {
+  return (size_t)MF_TQ * kcap + (size_t)MF_TQ * MF_BC + (size_t)kcap + (16 + 256 + (size_t)MF_TQ * dk) / 2;
+}
+// waves per workgroup: the most of 4, 2, 1 whose LDS fits the 160 KiB of a
+// gfx950 workgroup (plus the union slots)
+__host__ __device__ constexpr int mfma_waves(int kcap, int dk) {
+  return 4 * mfma_wave_words(kcap, dk) * 8 + 256 <= 163840   ? 4
+         : 2 * mfma_wave_words(kcap, dk) * 8 + 256 <= 163840 ? 2
+                                                               : 1;
+}
+
 // B-operand registers of one 16-column block and 32-dimension step
 template <int PREC>
 struct BOp;
@@ -84,26 +102,51 @@ struct BOp<3> {
   bf16x8_t h, l;
 };
 
-template <int KCAP, bool EXACT, int PREC, int DK>
+template <int KCAP, bool EXACT, int PREC, int DK, int W>
 __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
+  // W waves per workgroup; with W > 1 the B groups are double-buffered (the
+  // next group's loads are in flight while this one's MFMAs run)
+  constexpr bool DB = W > 1;
   // PREC: 0 fp32 operands, 1 bf16, 3 bf16 hi/lo (three products)
   constexpr bool BF16 = PREC != 0;
   constexpr bool SPLIT = PREC == 3;
   constexpr int NS = DK / 32;  // 32-dimension steps
   // 16-column blocks loaded together (B registers: NB * NS * 4 or 8 VGPRs)
-  constexpr int NB = (DK <= 128) ? 4 : 2;
+  constexpr int NB = DB ? (DK <= 128 ? 2 : 1) : (DK <= 128 ? 4 : 2);
   typedef BOp<PREC> BT;
   extern __shared__ u64 smem[];
-  u64 *lists = smem;                        // [16][KCAP]
+  const int wv = W == 1 ? 0 : (int)(threadIdx.x >> 6);
+  u32 *uslot = (u32 *)(smem + W * mfma_wave_words(KCAP, DK));  // [2][W] union slots
+  u64 *lists = smem + wv * mfma_wave_words(KCAP, DK);         // [16][KCAP]
   u64 *bufs = lists + MF_TQ * KCAP;         // [16][MF_BC]
   u64 *tmp = bufs + MF_TQ * MF_BC;          // [KCAP]
   float *fsh = (float *)(tmp + KCAP);       // [16] row norms
   float *dqs = fsh + 16;                    // [16 rows][16 cols] screening distances
   float *qsh = dqs + 256;                   // [16 rows][DK] the tile's query rows
-  const int lane = threadIdx.x, g = lane >> 4, c16 = lane & 15;
+  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
   const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
   u64 nvisit = 0;
-  const int A = tile.cluster, nq = tile.count;
+  // this wave's queries: permuted positions q0 .. q0 + nq - 1
+  const int A = tile.cluster, q0 = tile.start + 16 * wv;
+  const int nq = max(0, min(16, tile.count - 16 * wv));
+  // workgroup-wide OR of a wave-uniform value (double-buffered slots: a
+  // slot is rewritten only after the next union's barrier, which every wave
+  // reaches after reading it)
+  int uphase = 0;
+  auto wg_or = [&](u32 v) -> u32 {
+    if constexpr (W == 1) {
+      return v;
+    } else {
+      u32 *sl = uslot + uphase * W;
+      uphase ^= 1;
+      if (lane == 0) sl[wv] = v;
+      __syncthreads();
+      u32 r = 0;
+#pragma unroll
+      for (int w = 0; w < W; ++w) r |= sl[w];
+      return r;
+    }
+  };
   const int d = a.d, k = a.k, dk = a.dk, dpQ = a.Q.dpad;
   const float eps = dist_relerr(d);
   const int nst = dk / 32;
@@ -129,14 +172,13 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   typedef typename std::conditional<EXACT, NoDrop, DropLower>::type DropL;
   // this lane's slice of query row c16: dimensions 32 s + 8 g + j, in FP32
   const bool alive = c16 < nq;
-  const float *qa = a.Q.px + (int64_t)(tile.start + (alive ? c16 : 0)) * dpQ;
   // q_slice(s): this lane's 8 dimensions 32 s + 8 g + j of query row c16
-  // (re-read from the row when needed rather than held in registers)
+  // (re-read from LDS when needed rather than held in registers)
   for (int e = lane; e < MF_TQ * DK; e += 64) {
     const int r = e / DK, dim = e % DK;
-    qsh[e] = (r < nq && dim < d) ? a.Q.px[(int64_t)(tile.start + r) * dpQ + dim] : 0.f;
+    qsh[e] = (r < nq && dim < d) ? a.Q.px[(int64_t)(q0 + r) * dpQ + dim] : 0.f;
   }
-  __syncthreads();
+  wave_sync();
   const float *qrow_sh = qsh + c16 * DK + 8 * g;
   auto q_slice = [&](int s, float (&v)[8]) {
     const f32x4_t v0 = *(const f32x4_t *)(qrow_sh + 32 * s);
@@ -157,11 +199,11 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   qn += __shfl_xor(qn, 16);
   qn += __shfl_xor(qn, 32);
   if (g == 0) fsh[c16] = qn;
-  __syncthreads();
+  wave_sync();
   float qnr[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) qnr[i] = fsh[4 * g + i];
-  __syncthreads();
+  wave_sync();
   const float gE = 1.01f * ((float)(d + 3) * 5.97e-8f + 8.f * 5.97e-8f);  // gamma_{d+3} + 8u, with margin
   u64 evals = 0, nmerge = 0, npass = 0;
 
@@ -229,11 +271,18 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
       }
       need |= __shfl_xor((int)need, 16) != 0;
       need |= __shfl_xor((int)need, 32) != 0;
-      __syncthreads();
-      u64 pending = wave_ballot(need) & 0xFFFFull;
-      while (pending) {
-        const int jl = __builtin_ctzll(pending);
-        pending &= pending - 1;
+      wave_sync();
+      // clusters any wave of the workgroup needs, visited in one order
+      u32 pending = wg_or((u32)wave_ballot(need) & 0xFFFFu);
+      bool first = true;
+      for (; pending; pending &= pending - 1) {
+        const int jl = __builtin_ctz(pending);
+        // lockstep: every wave finishes the previous cluster before any
+        // starts this one (no-op for W = 1; `first` is workgroup-uniform)
+        if constexpr (W > 1) {
+          if (!first) __syncthreads();
+        }
+        first = false;
         const int b = __builtin_amdgcn_readlane(bj, jl);
         const int s0 = __builtin_amdgcn_readlane(s0j, jl), s1 = __builtin_amdgcn_readlane(s1j, jl);
         const float Rb = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Rbj), jl));
@@ -293,11 +342,11 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
         part += __shfl_xor(part, 16);
         part += __shfl_xor(part, 32);
         if (g == 0) fsh[c16] = part;
-        __syncthreads();
+        wave_sync();
         float nqr[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) nqr[i] = fsh[4 * g + i];
-        __syncthreads();
+        wave_sync();
 
         float Ei[4];
         u32 ecd[4];
@@ -427,7 +476,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
                 constexpr int WM = MF_BC - 16;
                 const bool hot = cnt[0] >= WM || cnt[1] >= WM || cnt[2] >= WM || cnt[3] >= WM;
                 if (wave_ballot(hot) != 0) {
-                  __syncthreads();
+                  wave_sync();
                   for (int r = 0; r < MF_TQ; ++r) {
                     const int gi = r >> 2, ii = r & 3;
                     int mine = cnt[0];
@@ -440,16 +489,30 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
               }
             }
           };
-          Group G0;
-          for (int g0 = pbeg; g0 < pend; g0 += 16 * NB) {
-            load_group(g0, G0);
-            compute_group(g0, G0);
+          constexpr int GS = 16 * NB;
+          if constexpr (DB) {
+            Group GA, GB;
+            load_group(pbeg, GA);
+            for (int g0 = pbeg; g0 < pend; g0 += 2 * GS) {
+              const int g1 = g0 + GS;
+              if (g1 < pend) load_group(g1, GB);
+              compute_group(g0, GA);
+              if (g1 >= pend) break;
+              if (g1 + GS < pend) load_group(g1 + GS, GA);
+              compute_group(g1, GB);
+            }
+          } else {
+            Group G0;
+            for (int g0 = pbeg; g0 < pend; g0 += GS) {
+              load_group(g0, G0);
+              compute_group(g0, G0);
+            }
           }
         }
       }
     }
   }
-  __syncthreads();
+  wave_sync();
   for (int r = 0; r < MF_TQ; ++r) {
     const int gi = r >> 2, ii = r & 3;
     int mine = cnt[0];
@@ -458,7 +521,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
       if (j == ii) mine = cnt[j];
     if (__shfl(mine, 16 * gi) > 0) merge(r);
   }
-  __syncthreads();
+  wave_sync();
 
   for (int r = 0; r < nq; ++r) {
     const int gi = r >> 2, ii = r & 3;
@@ -475,8 +538,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     const float Er = __shfl(em, 16 * gi);
     const float Vr = __shfl(ev, 16 * gi);
     const float cutoff = EXACT ? Tr : (Tr + 2.f * Er) * (1.f + 2.4e-7f);
-    emit_query<KCAP, EXACT>(a, lists + r * KCAP, tile.start + r, A, cutoff,
-                            a.Q.px + (int64_t)(tile.start + r) * dpQ, Tr, Vr);
+    emit_query<KCAP, EXACT>(a, lists + r * KCAP, q0 + r, A, cutoff, a.Q.px + (int64_t)(q0 + r) * dpQ, Tr, Vr);
   }
   if (a.tile_prof) {
     // diagnostic: the largest and smallest final threshold of the tile's rows
@@ -489,7 +551,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
       }
     tmax = wave_max(tmax);
     tmin = wave_min(tmin);
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
       u64 *tp = a.tile_prof + 4 * (size_t)blockIdx.x;
       tp[0] = clk0;
       tp[1] = wall_clock64();
@@ -504,42 +566,62 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   }
 }
 
-template <int KCAP, bool EXACT, int PREC, int DK>
-__global__ __launch_bounds__(64) void k_join_mfma(JoinArgs a) {
+template <int KCAP, bool EXACT, int PREC, int DK, int W>
+__global__ __launch_bounds__(64 * W) void k_join_mfma(JoinArgs a) {
   if ((int)blockIdx.x >= *a.ntiles) return;
   const Tile tile = a.tiles[blockIdx.x];
   if (tile.count <= 0) return;
-  mfma_tile<KCAP, EXACT, PREC, DK>(a, tile);
+  mfma_tile<KCAP, EXACT, PREC, DK, W>(a, tile);
 }
 
 // Many joins in one launch: slot b holds a tile of instance tiles[b].pad,
 // whose arguments are argv[tiles[b].pad].
-template <int KCAP, bool EXACT, int PREC, int DK>
-__global__ __launch_bounds__(64) void k_join_mfma_multi(const JoinArgs *__restrict__ argv,
-                                                        const Tile *__restrict__ tiles, int nslots) {
+template <int KCAP, bool EXACT, int PREC, int DK, int W>
+__global__ __launch_bounds__(64 * W) void k_join_mfma_multi(const JoinArgs *__restrict__ argv,
+                                                            const Tile *__restrict__ tiles, int nslots) {
   if ((int)blockIdx.x >= nslots) return;
   const Tile tile = tiles[blockIdx.x];
   if (tile.count <= 0) return;
-  mfma_tile<KCAP, EXACT, PREC, DK>(argv[tile.pad], tile);
+  mfma_tile<KCAP, EXACT, PREC, DK, W>(argv[tile.pad], tile);
 }
 
-inline size_t mfma_lds(int kcap, int dkmax) {
-  return sizeof(u64) * ((size_t)MF_TQ * kcap + (size_t)MF_TQ * MF_BC + (size_t)kcap) +
-         (16 + 256 + (size_t)MF_TQ * dkmax) * sizeof(float);
+// waves per workgroup actually launched: 1, or the most that fit when the
+// arguments ask for more than one
+inline int mfma_launch_waves(int kcap, int dkmax, int want) { return want > 1 ? mfma_waves(kcap, dkmax) : 1; }
+
+inline size_t mfma_lds(int kcap, int dkmax, int waves) {
+  return waves * mfma_wave_words(kcap, dkmax) * sizeof(u64) + 256;
 }
 
 template <int PREC>
 hipError_t launch_mfma_any(const JoinArgs &a, int64_t grid, hipStream_t s, const JoinArgs *argv = nullptr,
                            const Tile *tiles = nullptr) {
   const bool big = a.dk > 128;  // dk = roundup(d, 32)
-  const size_t lds = mfma_lds(a.kcap, big ? 256 : 128);
-#define KJ_LAUNCH(KC, EX, DKV)                                                                               \
-  do {                                                                                                       \
-    if (argv)                                                                                                \
-      hipLaunchKernelGGL((k_join_mfma_multi<KC, EX, PREC, DKV>), dim3((unsigned)grid), dim3(64), lds, s, argv, \
-                         tiles, (int)grid);                                                                  \
-    else                                                                                                     \
-      hipLaunchKernelGGL((k_join_mfma<KC, EX, PREC, DKV>), dim3((unsigned)grid), dim3(64), lds, s, a);       \
+  const int waves = mfma_launch_waves(a.kcap, big ? 256 : 128, a.waves);
+  const size_t lds = mfma_lds(a.kcap, big ? 256 : 128, waves);
+#define KJ_LAUNCH_W(KC, EX, DKV, WV)                                                                            \
+  do {                                                                                                          \
+    if (argv) {                                                                                                 \
+      hipFuncSetAttribute((const void *)k_join_mfma_multi<KC, EX, PREC, DKV, WV>,                               \
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                \
+      hipLaunchKernelGGL((k_join_mfma_multi<KC, EX, PREC, DKV, WV>), dim3((unsigned)grid), dim3(64 * WV), lds, s, \
+                         argv, tiles, (int)grid);                                                               \
+    } else {                                                                                                    \
+      hipFuncSetAttribute((const void *)k_join_mfma<KC, EX, PREC, DKV, WV>,                                     \
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                \
+      hipLaunchKernelGGL((k_join_mfma<KC, EX, PREC, DKV, WV>), dim3((unsigned)grid), dim3(64 * WV), lds, s, a);  \
+    }                                                                                                           \
+  } while (0)
+#define KJ_LAUNCH(KC, EX, DKV)                                   \
+  do {                                                           \
+    constexpr int WM = mfma_waves(KC, DKV);                      \
+    if constexpr (WM > 1) {                                      \
+      if (waves > 1) {                                           \
+        KJ_LAUNCH_W(KC, EX, DKV, WM);                            \
+        break;                                                   \
+      }                                                          \
+    }                                                            \
+    KJ_LAUNCH_W(KC, EX, DKV, 1);                                 \
   } while (0)
 #define KJ_DK(KC, EX)       \
   if (big)                  \
@@ -562,5 +644,6 @@ hipError_t launch_mfma_any(const JoinArgs &a, int64_t grid, hipStream_t s, const
 #undef KJ_EX
 #undef KJ_DK
 #undef KJ_LAUNCH
+#undef KJ_LAUNCH_W
   return hipGetLastError();
 }

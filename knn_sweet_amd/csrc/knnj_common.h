@@ -40,6 +40,16 @@ __device__ __forceinline__ float key_lower(u64 key, int ebase) {
   return key_dist(key) - 2.f * ecode_decode((u32)key >> KEY_ROW_BITS, ebase);
 }
 
+// Orders this wave's LDS accesses across its lanes: a wave's LDS
+// instructions execute in issue order, so compiler fences at wavefront scope
+// around a wave barrier suffice (no workgroup barrier; the other waves of a
+// multi-wave workgroup are not involved).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ u32 lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
 // number of set bits of `mask` below this lane

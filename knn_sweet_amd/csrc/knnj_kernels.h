@@ -26,7 +26,7 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
                                 hipStream_t s);
 // XCD-aware tile slots; *nslots = slots used (empty tiles have count 0);
 // every slot's pad field = inst
-void launch_make_tiles(const int32_t *off, int m, const int32_t *pcnt, int tq, int inst, Tile *tiles,
+void launch_make_tiles(const int32_t *off, int m, const double *theta, int tq, int inst, Tile *tiles,
                        int32_t *nslots, int max_slots, int nxcd, hipStream_t s);
 
 // k_bounds.hip
@@ -66,6 +66,7 @@ struct JoinArgs {
   float *ovf_diag;       // optional: first 64 overflow records {cutoff, k-th key, last key, first key}
   int ecode;             // MFMA filter keys carry per-entry error codes (rows < 2^26)
   int ebase;             // exponent base of those codes
+  int waves;             // MFMA join: > 1 asks for the multi-wave workgroup (tiles of 16 x waves queries)
 };
 
 // Picks the wave tile size for a candidate capacity; returns the dynamic LDS bytes.
@@ -82,7 +83,9 @@ void launch_center(const SideIndex &Ri, int d, int dk, int planes, void *rc, flo
 hipError_t launch_join_mfma_f32(const JoinArgs &a, int64_t max_tiles, hipStream_t s);
 // terms: 1 single bf16 product, 3 compensated hi/lo products
 hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles, hipStream_t s);
-size_t join_mfma_lds_bytes(int kcap);
+// queries per workgroup tile of the MFMA join (16 per wave; want_waves > 1
+// asks for the multi-wave workgroup)
+int join_mfma_tile_q(int kcap, int dk, int want_waves);
 
 // Fused launches over the tiles of many instances: slot b runs tile tiles[b]
 // of instance tiles[b].pad with arguments argv[tiles[b].pad]; proto carries

@@ -94,8 +94,9 @@ struct NoDrop {
 
 // Merges the unsorted candidate buffer buf[0..cnt) (cnt <= BCAP) into the
 // sorted list[0..KCAP), keeping the KCAP smallest keys.  Keys in list and
-// buf are distinct except for KEY_MAX padding.  tmp: KCAP scratch keys.  The
-// caller's workgroup is this single wave.  Returns the wave-wide minimum of
+// buf are distinct except for KEY_MAX padding.  tmp: KCAP scratch keys, all
+// three arrays private to the calling wave (other waves of its workgroup may
+// be elsewhere: only wave-level ordering is used).  Returns the wave-wide minimum of
 // drop(key) over the real keys that fell off the end (INFINITY if none).
 template <int KCAP, int BCAP = KNNJ_BCAP, class Drop = NoDrop>
 __device__ __forceinline__ float wave_merge(u64 *list, u64 *buf, int cnt, u64 *tmp, Drop drop = Drop()) {
@@ -107,7 +108,7 @@ __device__ __forceinline__ float wave_merge(u64 *list, u64 *buf, int cnt, u64 *t
   wave_sort_u64<EB>(v);
 #pragma unroll
   for (int u = 0; u < EB; ++u) buf[lane + 64 * u] = v[u];
-  __syncthreads();
+  wave_sync();
   float dmin = INFINITY;
 #pragma unroll
   for (int u = 0; u < KCAP / 64; ++u) {
@@ -129,10 +130,10 @@ __device__ __forceinline__ float wave_merge(u64 *list, u64 *buf, int cnt, u64 *t
     else if (x != KEY_MAX)
       dmin = fminf(dmin, drop(x));
   }
-  __syncthreads();
+  wave_sync();
 #pragma unroll
   for (int u = 0; u < KCAP / 64; ++u) list[lane + 64 * u] = tmp[lane + 64 * u];
-  __syncthreads();
+  wave_sync();
   if constexpr (Drop::tracks)
     return wave_min(dmin);
   else
