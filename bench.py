@@ -14,6 +14,9 @@ Workloads (seeded synthetic data, knn_sweet_amd.datagen):
   c4  512 independent joins, |R_i| log-uniform in [1e3, 2e5], |Q_i| = |R_i|/4,
       d_i in {16, 32, 96}, k=16: one batched call per step, the per-instance
       loop timed beside it
+  c5  serving: resident index of 4,000,000 x 96 points, 2,000 requests of
+      floor(exp U[0, ln 4096]) queries, k=50, Poisson arrivals at 70 % of the
+      measured saturation rate; p50/p99 request latency
 The roofline object describes the join kernel (the dominant one); its
 duration is the hipEvent time the library records around its launch on its
 own stream.  `traffic` is the HBM bytes per launch recorded from rocprofv3
@@ -55,6 +58,10 @@ def load_config(name: str, k: int | None):
         R, Q = datagen.mixture128(2_000_000, 200_000, seed=2018)
         k = k or 100
         return dict(workload=f"mixture-d128-R2M-Q200k-k{k}", R=R, Q=Q, k=k, self_join=False)
+    if name == "c5":
+        R = datagen.serving_set(4_000_000, seed=2020)
+        reqs = datagen.serving_batches(n_batches=2000, seed=2020)
+        return dict(workload="serving-R4M-d96-k50-batches1-4096", R=R, reqs=reqs, k=k or 50)
     if name == "c4":
         inst = datagen.batch_instances(n_inst=512, seed=2019)
         return dict(workload="batch-512-uneven-d16-32-96-k16", inst=inst, k=k or 16)
@@ -146,7 +153,8 @@ def bench_single(args, kj, cfg):
     Qd = Rd if cfg["self_join"] else torch.from_numpy(Q).cuda()
     torch.cuda.synchronize()
     ctx = kj.Context(0)
-    opts = dict(brute_force=args.brute_force, tile_waves=args.tile_waves,
+    opts = dict(brute_force=args.brute_force, tile_waves=args.tile_waves, xcd_tiles=args.xcd_tiles,
+                tile_split=args.tile_split, split_rounds=args.split_rounds,
                 kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel],
                 precision={"auto": kj.PREC_AUTO, "fp32": kj.PREC_FP32, "bf16": kj.PREC_BF16,
                            "bf16x1": kj.PREC_BF16X1}[args.precision])
@@ -243,12 +251,86 @@ def bench_batch(args, kj, cfg):
     return out
 
 
+def bench_serving(args, kj, cfg):
+    """Resident index + continuous-batching server: saturation throughput
+    (every request submitted at once), then Poisson arrivals at 70 % of it
+    with per-request latency (submit -> results written)."""
+    import concurrent.futures as cf
+    import threading
+    R, reqs, k = cfg["R"], cfg["reqs"], cfg["k"]
+    ctx = kj.Context(0)
+    t0 = time.perf_counter()
+    ix = ctx.build_index(R)
+    t_build = (time.perf_counter() - t0) * 1e3
+    info = ix.info()
+    srv = kj.Server(ix, k, slots=args.slots, max_batch_rows=4096)
+    nq = sum(q.shape[0] for q in reqs)
+    for q in reqs[:args.warmup * 8]:
+        srv.wait(srv.submit(q))
+    # saturation: everything queued at once
+    t0 = time.perf_counter()
+    tickets = [srv.submit(q) for q in reqs]
+    for t in tickets:
+        srv.wait(t)
+    t_sat = time.perf_counter() - t0
+    sat_qps, sat_rps = nq / t_sat, len(reqs) / t_sat
+    b0, r0 = srv.counters()
+    # open loop: Poisson arrivals at 70 % of the saturation request rate
+    rate = 0.7 * sat_rps
+    g = np.random.default_rng(2020)
+    gaps = g.exponential(1.0 / rate, size=len(reqs))
+    lat, qms, fused = [], [], []
+    pool = cf.ThreadPoolExecutor(16)
+    futs = []
+    start = time.perf_counter()
+    due = start
+    for q, gap in zip(reqs, gaps):
+        due += gap
+        dt = due - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        futs.append(pool.submit(srv.wait, srv.submit(q)))
+    for f in futs:
+        _, _, inf = f.result()
+        lat.append(inf["latency_ms"])
+        qms.append(inf["queue_ms"])
+        fused.append(inf["batch_requests"])
+    t_open = time.perf_counter() - start
+    b1, r1 = srv.counters()
+    pool.shutdown()
+    srv.close()
+    out = {
+        "metric": METRIC, "value": nq / t_open, "unit": "queries/s", "n_gpus": 1, "steps": len(reqs),
+        "warmup": args.warmup, "ms_per_step": t_open * 1e3 / len(reqs), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16" if info["precision"] in (kj.PREC_BF16, kj.PREC_BF16X1) else "f32",
+        "data": "synthetic (seeded Gaussian mixture, knn_sweet_amd.datagen)",
+        "config": {"workload": cfg["workload"], "n_r": R.shape[0], "d": R.shape[1], "k": k,
+                   "requests": len(reqs), "queries": nq, "slots": args.slots, "max_batch_rows": 4096,
+                   "offered_load_req_per_s": rate},
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "queue_p50_ms": float(np.percentile(qms, 50)), "queue_p99_ms": float(np.percentile(qms, 99)),
+        "saturation": {"queries_per_s": sat_qps, "requests_per_s": sat_rps, "batches": b0,
+                       "mean_requests_per_batch": len(reqs) / max(b0, 1)},
+        "open_loop_batches": b1 - b0, "mean_requests_per_batch_open": float(np.mean(fused)),
+        "index": {"build_ms": t_build, "bytes": info["bytes"], "m_r": info["m_r"],
+                  "precision": PREC_NAMES[info["precision"]]},
+    }
+    if not args.no_cpu:
+        Q = np.concatenate(reqs[:8])
+        cb = cpu_baseline(R, Q, k, target_s=10.0)
+        out["cpu_baseline"] = cb
+    ix.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--slots", type=int, default=3, help="serving (c5): concurrent batches")
     ap.add_argument("--k", type=int, default=None, help="override the workload's k")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP32 comparison (c3)")
@@ -256,6 +338,9 @@ def main():
     ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma"])
     ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16", "bf16x1"])
     ap.add_argument("--tile-waves", type=int, default=0, help="MFMA join: waves per workgroup tile (0 = auto)")
+    ap.add_argument("--xcd-tiles", action="store_true", help="place all tiles of a query cluster on one XCD")
+    ap.add_argument("--tile-split", type=int, default=0, help="MFMA join split mode (0 auto, 1 off, 4 on)")
+    ap.add_argument("--split-rounds", type=int, default=0, help="representative split rounds (0 default, -1 none)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter scale for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -268,7 +353,12 @@ def main():
     if args.copy_probe:
         copy_probe()
     cfg = load_config(args.config, args.k)
-    out = bench_batch(args, kj, cfg) if args.config == "c4" else bench_single(args, kj, cfg)
+    if args.config == "c4":
+        out = bench_batch(args, kj, cfg)
+    elif args.config == "c5":
+        out = bench_serving(args, kj, cfg)
+    else:
+        out = bench_single(args, kj, cfg)
     print(json.dumps(out))
 
 

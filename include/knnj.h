@@ -87,12 +87,18 @@ typedef struct {
   int32_t tile_q;      /* queries per wave tile of the VALU kernel, 0 = auto */
   int32_t kernel;      /* knnj_kernel: distance-tile implementation, 0 = auto */
   int32_t refine_iters;/* Lloyd steps moving each representative to its members' mean
-                          (0 = default 1, negative = keep the sampled points) */
+                          (0 = default: 1 per join, 2 for knnj_index_build; negative = keep
+                          the sampled points) */
   int32_t xcd_tiles;   /* 1: place all tiles of a query cluster on one XCD (shared L2) */
   int32_t tile_waves;  /* MFMA kernel: waves per workgroup tile, 0 = auto, 1 = one 16-query
                           wave, > 1 = the most waves whose LDS fits (2 or 4), walking the
                           reference clusters in lockstep */
-  int32_t reserved[2];
+  int32_t tile_split;  /* MFMA kernel: 0 = auto (joins with few tiles, e.g. small query
+                          batches), 1 = off, > 1 = on: the waves of a workgroup share one
+                          16-query tile, each taking every W-th block of its cluster list */
+  int32_t split_rounds;/* representative selection: rounds that replace sampled representatives
+                          by the farthest members of the widest clusters (0 = default: none
+                          per join, 16 for knnj_index_build; negative = none) */
 } knnj_opts;
 
 /* Device-side timings (hipEvents on the context's stream) and counters. */
@@ -166,6 +172,82 @@ knnj_status knnj_join(knnj_ctx *ctx, knnj_matrix R, knnj_matrix Q, int32_t k,
  * builds, bounds and tile lists. */
 knnj_status knnj_join_batch(knnj_ctx *ctx, int32_t n_inst, const knnj_instance *inst,
                             const knnj_opts *opts, knnj_stats *stats);
+
+/* ---- Resident reference index (the serving path) ----------------------
+ * The reference builds R's clusters inside every run (knnjoin.cu:1043-1068:
+ * clusterReps and the sorts that follow).  A knnj_index keeps that half of
+ * the program -- representatives, radii, the cluster-sorted rows and, for
+ * the MFMA kernel, the centred operand rows -- resident in HBM, so that
+ * query batches pay only for their own side (knnjoin.cu:1069-1141). */
+typedef struct knnj_index knnj_index;
+
+typedef struct {
+  int64_t n;          /* reference rows */
+  int32_t d;
+  int32_t m_r;        /* reference clusters */
+  int32_t kernel;     /* knnj_kernel fixed at build */
+  int32_t precision;  /* knnj_precision of the operand planes fixed at build */
+  int32_t device;
+  int32_t reserved0;
+  int64_t bytes;      /* device bytes held */
+  double t_build_ms;
+} knnj_index_desc;
+
+/* Builds the index of R on ctx's device (opts: m_r, trials, seed,
+ * refine_iters, kernel, precision).  R may be freed afterwards. */
+knnj_status knnj_index_build(knnj_ctx *ctx, knnj_matrix R, const knnj_opts *opts, knnj_index **out);
+void knnj_index_destroy(knnj_index *index);
+knnj_status knnj_index_info(const knnj_index *index, knnj_index_desc *info);
+
+/* Exact kNN of Q against the indexed R: the result of knnj_join(R, Q, k)
+ * with the index's build options.  Any number of contexts on the index's
+ * device may query one index concurrently (it is read-only). */
+knnj_status knnj_index_query(knnj_ctx *ctx, const knnj_index *index, knnj_matrix Q, int32_t k,
+                             const knnj_opts *opts, int32_t *out_idx, float *out_dist,
+                             knnj_stats *stats);
+
+/* ---- Serving: continuous batching over a resident index -----------------
+ * The reference answers one query set per process (knnjoin.cu:983-1172;
+ * its small-batch path knnjoin.cu:1103-1127).  A knnj_server owns `slots`
+ * worker threads, each with its own context (HIP stream and workspace).
+ * Requests (host rows, 1..max_batch_rows queries) are queued on arrival; a
+ * free worker fuses the oldest waiting requests while their rows fit
+ * max_batch_rows (the slot's workspace is sized for that many rows: the
+ * memory-footprint admission rule) and runs them as one knnj_index_query,
+ * so batches of different slots overlap on the device.  Every query's
+ * answer is its exact k-NN, the same alone and in any mix. */
+typedef struct knnj_server knnj_server;
+
+typedef struct {
+  int32_t k;
+  int32_t slots;           /* concurrent batches (worker threads / streams), 1..16 */
+  int64_t max_batch_rows;  /* rows fused into one batch (and the largest request) */
+  int64_t slot_budget;     /* device workspace bytes per slot, 0 = no cap */
+  knnj_opts opts;          /* query options (kernel and precision come from the index) */
+} knnj_server_config;
+
+typedef struct {
+  int32_t status;          /* knnj_status of the request */
+  int32_t batch_rows;      /* rows of the fused batch it ran in */
+  int32_t batch_requests;  /* requests in that batch */
+  int32_t slot;
+  double queue_ms;         /* submit -> its batch started */
+  double latency_ms;       /* submit -> results written */
+  char message[160];       /* error text when status != KNNJ_OK */
+} knnj_request_info;
+
+knnj_status knnj_server_create(const knnj_index *index, const knnj_server_config *cfg, knnj_server **out);
+/* Queues Q (host, n rows of stride ld floats, 0 = d); out_idx/out_dist:
+ * host [n][k], written before the request completes.  Thread-safe. */
+knnj_status knnj_server_submit(knnj_server *srv, const float *Q, int64_t n, int64_t ld, int32_t *out_idx,
+                               float *out_dist, uint64_t *ticket);
+/* Blocks until the request completes; returns its status.  Each ticket is
+ * waited for once. */
+knnj_status knnj_server_wait(knnj_server *srv, uint64_t ticket, knnj_request_info *info);
+/* Fused batches run and rows served so far. */
+knnj_status knnj_server_counters(knnj_server *srv, int64_t *batches, int64_t *rows);
+/* Serves every queued request, then stops the workers. */
+void knnj_server_destroy(knnj_server *srv);
 
 #ifdef __cplusplus
 }

@@ -48,7 +48,7 @@ class Opts(ctypes.Structure):
                 ("brute_force", ctypes.c_int32), ("tile_q", ctypes.c_int32),
                 ("kernel", ctypes.c_int32), ("refine_iters", ctypes.c_int32),
                 ("xcd_tiles", ctypes.c_int32), ("tile_waves", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("tile_split", ctypes.c_int32), ("split_rounds", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -73,8 +73,28 @@ class Instance(ctypes.Structure):
                 ("out_idx", ctypes.c_void_p), ("out_dist", ctypes.c_void_p)]
 
 
+class IndexDesc(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("d", ctypes.c_int32), ("m_r", ctypes.c_int32),
+                ("kernel", ctypes.c_int32), ("precision", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32), ("bytes", ctypes.c_int64), ("t_build_ms", ctypes.c_double)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_int32), ("slots", ctypes.c_int32), ("max_batch_rows", ctypes.c_int64),
+                ("slot_budget", ctypes.c_int64), ("opts", Opts)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("batch_rows", ctypes.c_int32),
+                ("batch_requests", ctypes.c_int32), ("slot", ctypes.c_int32),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+                ("message", ctypes.c_char * 160)]
+
+
 EXPORTS = ("knnj_version", "knnj_last_error", "knnj_ctx_create", "knnj_ctx_destroy",
-           "knnj_ctx_set_budget", "knnj_join", "knnj_join_batch")
+           "knnj_ctx_set_budget", "knnj_join", "knnj_join_batch", "knnj_index_build",
+           "knnj_index_destroy", "knnj_index_info", "knnj_index_query", "knnj_server_create",
+           "knnj_server_submit", "knnj_server_wait", "knnj_server_counters", "knnj_server_destroy")
 
 _lib = None
 
@@ -101,6 +121,25 @@ def load_library(path: str = LIB_PATH):
     lib.knnj_join_batch.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Instance),
                                     ctypes.POINTER(Opts), ctypes.POINTER(Stats)]
     lib.knnj_join_batch.restype = ctypes.c_int
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    lib.knnj_index_build.argtypes = [vp, Matrix, ctypes.POINTER(Opts), ctypes.POINTER(vp)]
+    lib.knnj_index_build.restype = ctypes.c_int
+    lib.knnj_index_destroy.argtypes = [vp]
+    lib.knnj_index_destroy.restype = None
+    lib.knnj_index_info.argtypes = [vp, ctypes.POINTER(IndexDesc)]
+    lib.knnj_index_info.restype = ctypes.c_int
+    lib.knnj_index_query.argtypes = [vp, vp, Matrix, i32, ctypes.POINTER(Opts), vp, vp, ctypes.POINTER(Stats)]
+    lib.knnj_index_query.restype = ctypes.c_int
+    lib.knnj_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+    lib.knnj_server_create.restype = ctypes.c_int
+    lib.knnj_server_submit.argtypes = [vp, vp, i64, i64, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
+    lib.knnj_server_submit.restype = ctypes.c_int
+    lib.knnj_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(RequestInfo)]
+    lib.knnj_server_wait.restype = ctypes.c_int
+    lib.knnj_server_counters.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    lib.knnj_server_counters.restype = ctypes.c_int
+    lib.knnj_server_destroy.argtypes = [vp]
+    lib.knnj_server_destroy.restype = None
     _lib = lib
     return lib
 
@@ -147,9 +186,9 @@ def _outputs(nq: int, k: int, aos: bool, qdev):
 
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
               brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0, xcd_tiles=False,
-              tile_waves=0) -> Opts:
+              tile_waves=0, tile_split=0, split_rounds=0) -> Opts:
     return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
-                tile_q, kernel, refine_iters, int(xcd_tiles), tile_waves)
+                tile_q, kernel, refine_iters, int(xcd_tiles), tile_waves, tile_split, split_rounds)
 
 
 class Context:
@@ -189,6 +228,10 @@ class Context:
         del keepR, keepQ
         return idx, dist, st.as_dict()
 
+    def build_index(self, R, **opts) -> "Index":
+        """A resident index of R on this context's device (see Index)."""
+        return Index(self, R, **opts)
+
     def join_batch(self, instances, **opts):
         """Many independent joins in one call.
 
@@ -211,6 +254,105 @@ class Context:
         _check(_lib.knnj_join_batch(self._h, len(instances), arr, ctypes.byref(o), ctypes.byref(st)))
         del keep
         return results, st.as_dict()
+
+
+class Index:
+    """A resident reference-side index (knnj_index): R's clusters, permuted
+    rows and centred operand rows kept in HBM for repeated queries."""
+
+    def __init__(self, ctx: "Context", R, **opts):
+        mR, keep, _ = _matrix(R)
+        o = make_opts(**opts)
+        h = ctypes.c_void_p()
+        _check(_lib.knnj_index_build(ctx._h, mR, ctypes.byref(o), ctypes.byref(h)))
+        del keep
+        self._h = h
+        self.ctx = ctx
+
+    def info(self) -> dict:
+        v = IndexDesc()
+        _check(_lib.knnj_index_info(self._h, ctypes.byref(v)))
+        return {name: getattr(v, name) for name, _ in v._fields_ if not name.startswith("reserved")}
+
+    def query(self, Q, k: int, ctx: "Context" = None, **opts):
+        """Exact kNN of Q against the indexed R; returns (idx, dist, stats-dict)."""
+        ctx = ctx or self.ctx
+        mQ, keepQ, qdev = _matrix(Q)
+        o = make_opts(**opts)
+        idx, dist, pi, pd = _outputs(mQ.n, k, bool(o.out_aos), keepQ if qdev else None)
+        st = Stats()
+        _check(_lib.knnj_index_query(ctx._h, self._h, mQ, k, ctypes.byref(o), pi, pd, ctypes.byref(st)))
+        del keepQ
+        return idx, dist, st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.knnj_index_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop over an Index (knnj_server).
+
+    submit() queues host query rows and returns a ticket at once; wait()
+    blocks until that request's results are written and returns
+    (idx, dist, info).  Thread-safe; ctypes releases the GIL in both calls.
+    """
+
+    def __init__(self, index: Index, k: int, slots: int = 2, max_batch_rows: int = 4096,
+                 slot_budget: int = 0, **opts):
+        cfg = ServerConfig(k, slots, max_batch_rows, slot_budget, make_opts(**opts))
+        h = ctypes.c_void_p()
+        st = _lib.knnj_server_create(index._h, ctypes.byref(cfg), ctypes.byref(h))
+        if st != KNNJ_OK:
+            raise KnnjError(st, "server creation failed")
+        self._h, self.index, self.k = h, index, k
+        self._pending = {}
+
+    def submit(self, Q) -> int:
+        a = np.ascontiguousarray(Q, dtype=np.float32)
+        if a.ndim != 2:
+            raise ValueError("expected a 2-D array")
+        idx = np.empty((a.shape[0], self.k), np.int32)
+        dist = np.empty((a.shape[0], self.k), np.float32)
+        t = ctypes.c_uint64()
+        st = _lib.knnj_server_submit(self._h, a.ctypes.data, a.shape[0], a.shape[1], idx.ctypes.data,
+                                     dist.ctypes.data, ctypes.byref(t))
+        if st != KNNJ_OK:
+            raise KnnjError(st, "submit rejected")
+        self._pending[t.value] = (a, idx, dist)
+        return t.value
+
+    def wait(self, ticket: int):
+        info = RequestInfo()
+        st = _lib.knnj_server_wait(self._h, ticket, ctypes.byref(info))
+        _, idx, dist = self._pending.pop(ticket)
+        if st != KNNJ_OK:
+            raise KnnjError(st, info.message.decode())
+        d = {name: getattr(info, name) for name, _ in info._fields_ if name != "message"}
+        return idx, dist, d
+
+    def counters(self):
+        b, r = ctypes.c_int64(), ctypes.c_int64()
+        _check(_lib.knnj_server_counters(self._h, ctypes.byref(b), ctypes.byref(r)))
+        return b.value, r.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.knnj_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default_ctx = {}

@@ -143,8 +143,21 @@ struct Alloc {
   bool arena = false;
   char *base = nullptr;
   size_t off = 0;
+  std::vector<void *> *owned = nullptr;  // set: a hipMalloc per buffer, recorded here
   template <class T>
   knnj_status get(const std::string &name, size_t count, T **out) {
+    if (owned) {
+      void *p = nullptr;
+      const hipError_t e = hipMalloc(&p, std::max<size_t>(count * sizeof(T), 256));
+      if (e != hipSuccess) {
+        set_err("hipMalloc(%zu) for %s failed: %s", count * sizeof(T), name.c_str(), hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? KNNJ_ENOMEM : KNNJ_EDEVICE;
+      }
+      owned->push_back(p);
+      off += count * sizeof(T);
+      *out = (T *)p;
+      return KNNJ_OK;
+    }
     if (!arena) return ws(c, name, count, out);
     const size_t bytes = (std::max<size_t>(count * sizeof(T), 1) + 255) & ~(size_t)255;
     *out = base ? (T *)(base + off) : nullptr;
@@ -226,6 +239,10 @@ knnj_status check_join_args(const knnj_matrix &R, const knnj_matrix &Q, int32_t 
     set_err("tile_q must be 0, 1, 2, 4 or 8");
     return KNNJ_EINVAL;
   }
+  if (o.tile_split < 0 || o.tile_split > 16) {
+    set_err("tile_split must be in [0, 16]");
+    return KNNJ_EINVAL;
+  }
   if (o.tile_waves < 0 || o.tile_waves > 16) {
     set_err("tile_waves must be in [0, 16]");
     return KNNJ_EINVAL;
@@ -252,9 +269,32 @@ struct SideBufs {
   int nblk = 0;
 };
 
+}  // namespace
+
+// A resident reference-side index (knnj_index_build): the cluster index of
+// R, its permuted rows and, for the MFMA kernel, the centred rows.
+struct knnj_index {
+  int device = 0;
+  int d = 0;
+  int64_t n = 0;
+  knnj_opts o{};     // build options
+  u32 flags[3] = {};  // input scan of R: non-finite, max |x| bits, non-integral
+  int kind = 0, dpad = 0, dk = 0, m_r = 0;
+  bool mfma = false, bf16 = false, bf16x1 = false;
+  SideIndex Ri{};
+  void *rc = nullptr;
+  float *nr = nullptr, *reps32 = nullptr, *rn32 = nullptr;
+  std::vector<void *> owned;
+  size_t bytes = 0;
+  double t_build_ms = 0.0;
+};
+
+namespace {
+
 // One planned join.
 struct Inst {
   int id = 0;  // position in a batch (the tiles' pad field)
+  const knnj_index *pre = nullptr;  // reference side prebuilt (knnj_index_query)
   // inputs: device rows, or host rows still to be copied
   const float *dR = nullptr, *dQ = nullptr;
   const float *hR = nullptr, *hQ = nullptr;
@@ -269,9 +309,11 @@ struct Inst {
   float *out_dist = nullptr;
   bool out_host = false, aos = false;
   // decisions
-  int kind = 0, kcap = 0, tq = 0, dpad = 0, dk = 0, m_r = 0, m_q = 0, trials = 10, refine = 1, waves = 1;
+  int kind = 0, kcap = 0, tq = 0, dpad = 0, dk = 0, m_r = 0, m_q = 0, trials = 10, refine = 1, waves = 1,
+      split = 1;
   bool mfma = false, bf16 = false, bf16x1 = false, exact = false, exact_valu = false, integral = false;
   bool use_mfma = false;
+  int split_rounds = 0;  // split rounds in the representative selection
   double span = 0.0;
   uint64_t seed = 2015;
   int64_t max_slots = 0;
@@ -290,9 +332,9 @@ struct Inst {
   // exact flag of the kernel that runs the tiles
   int kernel_exact() const { return use_mfma ? (exact ? 1 : 0) : ((mfma ? exact_valu : exact) ? 1 : 0); }
   // kernel instantiation: joins with equal keys can share one launch
-  std::tuple<int, int, int, int, int> launch_key() const {
+  std::tuple<int, int, int, int, int, int> launch_key() const {
     return std::make_tuple(use_mfma ? (bf16 ? (bf16x1 ? 3 : 2) : 1) : 0, kcap, kernel_exact(), tq,
-                           use_mfma && dk > 128 ? 1 : 0);
+                           use_mfma && dk > 128 ? 1 : 0, split);
   }
 };
 
@@ -316,14 +358,25 @@ void describe(Inst &I, const knnj_matrix &R, const knnj_matrix &Q, int32_t k, co
   I.user_dist = out_dist;
 }
 
+// Host rows (stride ld) to packed device rows (stride d): one linear copy
+// when the rows are already packed (a strided copy of many short rows from
+// pageable memory is far slower), a 2-D copy otherwise.
+knnj_status copy_rows_h2d(float *dst, const float *src, int64_t n, int d, int64_t ld, hipStream_t s) {
+  if (n <= 0) return KNNJ_OK;
+  if (ld == d)
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice, s));
+  else
+    HIPCHK(hipMemcpy2DAsync(dst, sizeof(float) * d, src, sizeof(float) * ld, sizeof(float) * d, n,
+                            hipMemcpyHostToDevice, s));
+  return KNNJ_OK;
+}
+
 // Device copies of host inputs (rows packed to ld = d), then the H2D copies.
 knnj_status stage_inputs(Alloc &A, Inst &I, hipStream_t s) {
   if (I.hR) {
     float *b;
     TRY(A.get("in_r", (size_t)std::max<int64_t>(I.nR, 1) * I.d, &b));
-    if (b && I.nR > 0)
-      HIPCHK(hipMemcpy2DAsync(b, sizeof(float) * I.d, I.hR, sizeof(float) * I.ldR, sizeof(float) * I.d, I.nR,
-                              hipMemcpyHostToDevice, s));
+    if (b) TRY(copy_rows_h2d(b, I.hR, I.nR, I.d, I.ldR, s));
     I.dR = b;
     I.ldR = I.d;
   }
@@ -333,9 +386,7 @@ knnj_status stage_inputs(Alloc &A, Inst &I, hipStream_t s) {
   } else if (I.hQ) {
     float *b;
     TRY(A.get("in_q", (size_t)std::max<int64_t>(I.nQ, 1) * I.d, &b));
-    if (b && I.nQ > 0)
-      HIPCHK(hipMemcpy2DAsync(b, sizeof(float) * I.d, I.hQ, sizeof(float) * I.ldQ, sizeof(float) * I.d, I.nQ,
-                              hipMemcpyHostToDevice, s));
+    if (b) TRY(copy_rows_h2d(b, I.hQ, I.nQ, I.d, I.ldQ, s));
     I.dQ = b;
     I.ldQ = I.d;
   }
@@ -372,15 +423,27 @@ knnj_status decide(Inst &I, const u32 *flags) {
   I.exact = I.mfma ? (!I.bf16 && exact_mfma) : I.exact_valu;
   I.dpad = (d + 3) & ~3;
   I.dk = I.mfma ? (d + 31) & ~31 : 0;
+  if (I.pre) {
+    // the index fixed the kernel and the operand planes of the reference rows
+    const knnj_index &X = *I.pre;
+    I.kind = X.kind;
+    I.mfma = X.mfma;
+    I.bf16 = X.bf16;
+    I.bf16x1 = X.bf16x1;
+    I.exact = I.mfma ? (!I.bf16 && exact_mfma) : I.exact_valu;
+    I.dk = X.dk;
+  }
   const bool brute = o.brute_force != 0;
   int m_r = brute ? 1 : (o.m_r > 0 ? (int)std::min<int64_t>(o.m_r, I.nR) : default_reps(I.nR));
   int m_q = brute ? 1 : (o.m_q > 0 ? (int)std::min<int64_t>(o.m_q, I.nQ) : default_reps(I.nQ));
   I.m_r = std::max(1, std::min(m_r, KNNJ_MAX_REPS));
   I.m_q = std::max(1, std::min(m_q, KNNJ_MAX_REPS));
   if (I.self) I.m_q = I.m_r;
+  if (I.pre) I.m_r = I.pre->m_r;
   I.trials = o.trials > 0 ? std::min(o.trials, 64) : 10;
   I.seed = o.seed ? o.seed : 2015;
   I.refine = o.refine_iters < 0 ? 0 : (o.refine_iters > 0 ? o.refine_iters : 1);
+  I.split_rounds = o.split_rounds > 0 ? std::min(o.split_rounds, 16) : 0;  // joins: none by default
   int kcap = I.exact    ? pow2_at_least(std::max(k, 64))
              : I.bf16x1 ? pow2_at_least(std::max(2 * k, 64))
                         : pow2_at_least(std::max(k + std::max(8, k / 8), 64));
@@ -389,8 +452,15 @@ knnj_status decide(Inst &I, const u32 *flags) {
   I.kcap = kcap;
   I.use_mfma = I.mfma && kcap <= 512;
   I.waves = o.tile_waves > 1 ? o.tile_waves : 1;  // auto: one wave per tile
-  I.tq = I.use_mfma ? join_mfma_tile_q(kcap, I.dk, I.waves)
-                   : (o.tile_q ? std::min(o.tile_q, join_tile_q(kcap)) : join_tile_q(kcap));
+  I.split = 1;
+  if (I.use_mfma && I.waves == 1 && o.tile_split != 1) {
+    // few tiles (small query sets): W waves share each tile's cluster list
+    const int sw = join_mfma_split_waves(kcap, I.dk);
+    const bool want = o.tile_split > 1 || (o.tile_split == 0 && I.nQ / 16 + I.m_q < 1024);
+    if (sw > 1 && want) I.split = I.waves = sw;
+  }
+  I.tq = I.use_mfma ? (I.split > 1 ? 16 : join_mfma_tile_q(kcap, I.dk, I.waves))
+                    : (o.tile_q ? std::min(o.tile_q, join_tile_q(kcap)) : join_tile_q(kcap));
   // XCD-aware slots, empty ones included
   I.max_slots = (int64_t)(o.xcd_tiles ? 8 : 1) * (I.nQ / I.tq + I.m_q + 1);
   return KNNJ_OK;
@@ -425,9 +495,17 @@ knnj_status alloc_side(Alloc &A, const std::string &p, int64_t n, int m, int dpa
 // Every device buffer of one join after decide(); `tiles` given: the join's
 // slot range of a batch's shared tile array.
 knnj_status alloc_inst(Alloc &A, Inst &I, Tile *tiles) {
-  TRY(alloc_side(A, "r_", I.nR, I.m_r, I.dpad, I.d, I.trials, I.rb));
+  if (I.pre) {
+    I.Ri = I.pre->Ri;
+    I.rc = I.pre->rc;
+    I.nr = I.pre->nr;
+    I.reps32 = I.pre->reps32;
+    I.rn32 = I.pre->rn32;
+  } else {
+    TRY(alloc_side(A, "r_", I.nR, I.m_r, I.dpad, I.d, I.trials, I.rb));
+  }
   if (!I.self) TRY(alloc_side(A, "q_", I.nQ, I.m_q, I.dpad, I.d, I.trials, I.qb));
-  if (I.mfma) {
+  if (I.mfma && !I.pre) {
     char *rc;
     TRY(A.get("r_rc", (size_t)I.nR * I.dk * 4, &rc));
     I.rc = rc;
@@ -469,7 +547,20 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
   uint64_t st = I.seed;
   for (auto &v : hc) v = (int32_t)(splitmix64(st) % (uint64_t)n);
   HIPCHK(hipMemcpyAsync(b.cand, hc.data(), hc.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  launch_rep_select(X, ld, d, dpad, b.cand, m, I.trials, b.partial, b.nblk, b.reps, b.rep_rows, s);
+  // m0 sampled representatives, then `rounds` split rounds, each adding the
+  // farthest members of the `step` widest clusters of the current set
+  const int rounds = (m >= 64 && n >= 4 * (int64_t)m) ? I.split_rounds : 0;
+  const int step = rounds > 0 ? std::max(1, m / (8 * rounds)) : 0;
+  int mc = m - rounds * step;
+  launch_rep_select(X, ld, d, dpad, b.cand, mc, I.trials, b.partial, rep_score_blocks(mc), b.reps, b.rep_rows, s);
+  for (int r = 0; r < rounds; ++r) {
+    launch_assign(X, n, ld, d, dpad, b.reps, mc, b.cl, b.rb, b.rn, s);
+    HIPCHK(hipMemsetAsync(b.radbits, 0, sizeof(u64) * mc, s));
+    HIPCHK(hipMemsetAsync(b.cnt, 0, sizeof(int32_t) * mc, s));
+    launch_assign_finish(X, n, ld, d, dpad, b.reps, b.cl, b.d2r, b.radbits, b.cnt, s);
+    launch_split_reps(X, n, ld, d, dpad, mc, step, b.cl, b.d2r, b.gkeys, b.reps, s);
+    mc += step;
+  }
   // Lloyd refinement: members of each sampled representative (ascending row
   // order), their mean becomes the representative
   for (int it = 0; it < I.refine; ++it) {
@@ -504,12 +595,12 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
 // Index build, centring, bounds and the tile list of one join.  Optional
 // events are recorded after the index (with centring) and after the bounds.
 knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_index, hipEvent_t ev_bounds) {
-  TRY(enqueue_side(c, I, true, s));
+  if (!I.pre) TRY(enqueue_side(c, I, true, s));
   if (I.self)
     I.Qi = I.Ri;
   else
     TRY(enqueue_side(c, I, false, s));
-  if (I.mfma) {
+  if (I.mfma && !I.pre) {
     launch_center(I.Ri, I.d, I.dk, I.bf16 ? (I.bf16x1 ? 1 : 2) : 0, I.rc, I.nr, I.reps32, I.rn32, s);
     HIPCHK(hipGetLastError());
   }
@@ -535,6 +626,8 @@ void fill_args(Inst &I) {
   a.k = I.k;
   a.kcap = I.kcap;
   a.waves = I.waves;
+  a.split = I.split;
+  a.lockstep = std::getenv("KNNJ_NOLOCK") ? 0 : 1;
   a.exact = I.kernel_exact();
   const double delta = I.exact ? 0.0 : (double)(d + 4) * 0x1p-23 + 0x1p-40;
   a.delta = delta;
@@ -593,6 +686,7 @@ knnj_status rerun_overflow(Inst &I, int32_t novf, hipStream_t s) {
   b.tile_prof = nullptr;
   b.ovf_diag = nullptr;
   b.waves = 1;  // one-query tiles
+  b.split = 1;
   const bool mfma_up = I.use_mfma && I.kcap < 512;
   if (mfma_up) {
     b.kcap = 512;
@@ -696,6 +790,89 @@ knnj_status print_tile_profile(const Inst &I, int32_t nslots, int32_t novf) {
 
 }  // namespace
 
+namespace {
+
+// The planned join after its input scan: index (both sides, or the query
+// side against a prebuilt index), bounds, tiles, the join, overflow re-runs,
+// outputs and stats.  ev[0] was recorded at the call's start.
+knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
+  hipStream_t s = c->stream;
+  hipEvent_t *ev = c->ev;
+  knnj_stats st{};
+  plan_stats(I, st);
+  if (I.nQ == 0) {
+    if (stats) *stats = st;
+    return KNNJ_OK;
+  }
+  TRY(alloc_inst(A, I, nullptr));
+
+  HIPCHK(hipEventRecord(ev[1], s));
+  TRY(enqueue_index(c, I, s, ev[2], ev[3]));
+  int32_t nslots = 0;
+  HIPCHK(hipMemcpyAsync(&nslots, I.counters, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+
+  fill_args(I);
+  const bool tile_prof = std::getenv("KNNJ_TILE_PROFILE") != nullptr;
+  if (tile_prof) {
+    u64 *tp;
+    float *dg;
+    TRY(ws(c, "tile_prof", (size_t)I.max_slots * 4, &tp));
+    TRY(ws(c, "ovf_diag", 256, &dg));
+    HIPCHK(hipMemsetAsync(tp, 0, sizeof(u64) * 4 * I.max_slots, s));
+    HIPCHK(hipMemsetAsync(dg, 0, sizeof(float) * 256, s));
+    I.args.tile_prof = tp;
+    I.args.ovf_diag = dg;
+  }
+  HIPCHK(hipEventRecord(ev[8], s));
+  HIPCHK(launch_main(I, nslots, s));
+  HIPCHK(hipEventRecord(ev[4], s));
+
+  int32_t hc[4];
+  HIPCHK(hipMemcpyAsync(hc, I.counters, sizeof(hc), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  st.n_tiles = hc[0];
+  st.n_overflow = hc[1];
+  st.n_launches = 1;
+  if (tile_prof) TRY(print_tile_profile(I, nslots, hc[1]));
+  HIPCHK(hipEventRecord(ev[5], s));
+  if (hc[1] > 0) TRY(rerun_overflow(I, hc[1], s));
+  HIPCHK(hipEventRecord(ev[6], s));
+  TRY(copy_outputs(I, s));
+  u64 hcnt[4] = {0, 0, 0, 0};
+  u32 herr = 0;
+  HIPCHK(hipMemcpyAsync(hcnt, I.evals, sizeof(hcnt), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(&herr, I.counters + 3, sizeof(herr), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipEventRecord(ev[7], s));
+  HIPCHK(hipStreamSynchronize(s));
+
+  st.t_total_ms = elapsed(ev[0], ev[7]);
+  st.t_index_ms = elapsed(ev[1], ev[2]);
+  st.t_bounds_ms = elapsed(ev[2], ev[3]);
+  st.t_join_ms = elapsed(ev[8], ev[4]);
+  st.t_fallback_ms = hc[1] > 0 ? elapsed(ev[5], ev[6]) : 0.0;
+  st.n_dist_evals = (int64_t)hcnt[0];
+  st.n_merges = (int64_t)hcnt[1];
+  st.n_appended = (int64_t)hcnt[2];
+  const double tot = (double)I.nR * (double)I.nQ;
+  st.pruning_ratio = tot > 0 ? 1.0 - (double)hcnt[0] / tot : 0.0;
+  float ferr;
+  std::memcpy(&ferr, &herr, 4);
+  st.filter_max_abs_err = ferr;
+  {
+    std::vector<int32_t> hp(I.m_q);
+    HIPCHK(hipMemcpy(hp.data(), I.pcnt, sizeof(int32_t) * I.m_q, hipMemcpyDeviceToHost));
+    int64_t t = 0;
+    for (int32_t v : hp) t += v;
+    st.n_cluster_pairs = t;
+  }
+  if (stats) *stats = st;
+  return KNNJ_OK;
+}
+
+
+}  // namespace
+
 extern "C" {
 
 const char *knnj_version(void) { return "knnj 0.2.0 (gfx950)"; }
@@ -769,7 +946,6 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   hipStream_t s = c->stream;
   hipEvent_t *ev = c->ev;
   c->host_cand.clear();
-  knnj_stats st{};
   Inst I;
   describe(I, R, Q, k, o, out_idx, out_dist);
   HIPCHK(hipEventRecord(ev[0], s));
@@ -787,75 +963,227 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   HIPCHK(hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   TRY(decide(I, hflags));
-  plan_stats(I, st);
-  if (I.nQ == 0) {
-    if (stats) *stats = st;
-    return KNNJ_OK;
+  return run_single(c, I, A, stats);
+}
+
+void knnj_index_destroy(knnj_index *x) {
+  if (!x) return;
+  hipSetDevice(x->device);
+  hipDeviceSynchronize();
+  for (void *p : x->owned) hipFree(p);
+  delete x;
+}
+
+knnj_status knnj_index_build(knnj_ctx *c, knnj_matrix R, const knnj_opts *opts, knnj_index **out) {
+  g_err.clear();
+  if (!c || !out) {
+    set_err("context or output pointer is NULL");
+    return KNNJ_EINVAL;
   }
-  TRY(alloc_inst(A, I, nullptr));
-
-  HIPCHK(hipEventRecord(ev[1], s));
-  TRY(enqueue_index(c, I, s, ev[2], ev[3]));
-  int32_t nslots = 0;
-  HIPCHK(hipMemcpyAsync(&nslots, I.counters, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-
-  fill_args(I);
-  const bool tile_prof = std::getenv("KNNJ_TILE_PROFILE") != nullptr;
-  if (tile_prof) {
-    u64 *tp;
-    float *dg;
-    TRY(ws(c, "tile_prof", (size_t)I.max_slots * 4, &tp));
-    TRY(ws(c, "ovf_diag", 256, &dg));
-    HIPCHK(hipMemsetAsync(tp, 0, sizeof(u64) * 4 * I.max_slots, s));
-    HIPCHK(hipMemsetAsync(dg, 0, sizeof(float) * 256, s));
-    I.args.tile_prof = tp;
-    I.args.ovf_diag = dg;
+  *out = nullptr;
+  knnj_opts o{};
+  if (opts) o = *opts;
+  TRY(check_matrix(R, "R"));
+  if (R.n < 1) {
+    set_err("R is empty");
+    return KNNJ_EINVAL;
   }
-  HIPCHK(hipEventRecord(ev[8], s));
-  HIPCHK(launch_main(I, nslots, s));
-  HIPCHK(hipEventRecord(ev[4], s));
-
-  int32_t hc[4];
-  HIPCHK(hipMemcpyAsync(hc, I.counters, sizeof(hc), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  st.n_tiles = hc[0];
-  st.n_overflow = hc[1];
-  st.n_launches = 1;
-  if (tile_prof) TRY(print_tile_profile(I, nslots, hc[1]));
-  HIPCHK(hipEventRecord(ev[5], s));
-  if (hc[1] > 0) TRY(rerun_overflow(I, hc[1], s));
-  HIPCHK(hipEventRecord(ev[6], s));
-  TRY(copy_outputs(I, s));
-  u64 hcnt[4] = {0, 0, 0, 0};
-  u32 herr = 0;
-  HIPCHK(hipMemcpyAsync(hcnt, I.evals, sizeof(hcnt), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(&herr, I.counters + 3, sizeof(herr), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipEventRecord(ev[7], s));
-  HIPCHK(hipStreamSynchronize(s));
-
-  st.t_total_ms = elapsed(ev[0], ev[7]);
-  st.t_index_ms = elapsed(ev[1], ev[2]);
-  st.t_bounds_ms = elapsed(ev[2], ev[3]);
-  st.t_join_ms = elapsed(ev[8], ev[4]);
-  st.t_fallback_ms = hc[1] > 0 ? elapsed(ev[5], ev[6]) : 0.0;
-  st.n_dist_evals = (int64_t)hcnt[0];
-  st.n_merges = (int64_t)hcnt[1];
-  st.n_appended = (int64_t)hcnt[2];
-  const double tot = (double)I.nR * (double)I.nQ;
-  st.pruning_ratio = tot > 0 ? 1.0 - (double)hcnt[0] / tot : 0.0;
-  float ferr;
-  std::memcpy(&ferr, &herr, 4);
-  st.filter_max_abs_err = ferr;
+  // the option checks of a join (a one-row dummy query side)
   {
-    std::vector<int32_t> hp(I.m_q);
-    HIPCHK(hipMemcpy(hp.data(), I.pcnt, sizeof(int32_t) * I.m_q, hipMemcpyDeviceToHost));
-    int64_t t = 0;
-    for (int32_t v : hp) t += v;
-    st.n_cluster_pairs = t;
+    knnj_matrix Q1 = R;
+    Q1.n = 0;
+    TRY(check_join_args(R, Q1, 1, o, nullptr, nullptr));
   }
-  if (stats) *stats = st;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  c->host_cand.clear();
+  HIPCHK(hipEventRecord(c->ev[0], s));
+  Inst I;
+  I.d = R.d;
+  I.k = 1;
+  // a resident index is built once: more split rounds and Lloyd steps by default
+  if (o.split_rounds == 0) o.split_rounds = 16;
+  if (o.refine_iters == 0) o.refine_iters = 2;
+  I.o = o;
+  I.nR = R.n;
+  I.ldR = R.ld ? R.ld : R.d;
+  (R.on_device ? I.dR : I.hR) = R.data;
+  Alloc A;
+  A.c = c;
+  TRY(stage_inputs(A, I, s));
+  u32 *flags;
+  TRY(ws(c, "flags", 8, &flags));
+  HIPCHK(hipMemsetAsync(flags, 0, 8 * sizeof(u32), s));
+  launch_scan_input(I.dR, I.nR, I.d, I.ldR, flags, s);
+  u32 hflags[3];
+  HIPCHK(hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  TRY(decide(I, hflags));
+
+  knnj_index *x = new knnj_index();
+  x->device = c->device;
+  x->d = I.d;
+  x->n = I.nR;
+  x->o = o;
+  std::memcpy(x->flags, hflags, sizeof(hflags));
+  x->kind = I.kind;
+  x->dpad = I.dpad;
+  x->dk = I.dk;
+  x->m_r = I.m_r;
+  x->mfma = I.mfma;
+  x->bf16 = I.bf16;
+  x->bf16x1 = I.bf16x1;
+  // persistent buffers owned by the index; build scratch from the context
+  Alloc P;
+  P.c = c;
+  P.owned = &x->owned;
+  SideBufs &b = I.rb;
+  const int64_t n = I.nR;
+  const int m = I.m_r;
+  auto fail = [&](knnj_status e) {
+    knnj_index_destroy(x);
+    return e;
+  };
+#define OWN(expr)                            \
+  do {                                       \
+    const knnj_status e_ = (expr);           \
+    if (e_ != KNNJ_OK) return fail(e_);      \
+  } while (0)
+  OWN(P.get("reps", (size_t)m * I.dpad, &b.reps));
+  OWN(P.get("rad", (size_t)m, &b.radbits));
+  OWN(P.get("off", (size_t)m + 1, &b.off));
+  OWN(P.get("perm", (size_t)n, &b.perm));
+  OWN(P.get("pdist", (size_t)n, &b.pdist));
+  OWN(P.get("px", (size_t)n * I.dpad, &b.px));
+  if (I.mfma) {
+    char *rc;
+    OWN(P.get("rc", (size_t)n * I.dk * 4, &rc));
+    I.rc = rc;
+    OWN(P.get("nr", (size_t)n, &I.nr));
+    OWN(P.get("reps32", (size_t)m * I.dk, &I.reps32));
+    OWN(P.get("rn32", (size_t)m, &I.rn32));
+  }
+  OWN(A.get("ix_cnt", (size_t)m, &b.cnt));
+  OWN(A.get("ix_fill", (size_t)m, &b.fill));
+  OWN(A.get("ix_cl", (size_t)n, &b.cl));
+  OWN(A.get("ix_d2r", (size_t)n, &b.d2r));
+  OWN(A.get("ix_gkeys", (size_t)n, &b.gkeys));
+  OWN(A.get("ix_cand", (size_t)m * I.trials, &b.cand));
+  OWN(A.get("ix_reprows", (size_t)m, &b.rep_rows));
+  b.nblk = rep_score_blocks(m);
+  OWN(A.get("ix_partial", (size_t)b.nblk * I.trials, &b.partial));
+  if (I.d >= 16 && ((I.d + 31) & ~31) <= 256) {
+    OWN(A.get("ix_repsbf16", (size_t)m * ((I.d + 31) & ~31), &b.rb));
+    OWN(A.get("ix_repsnrm", (size_t)m, &b.rn));
+  }
+  OWN(enqueue_side(c, I, true, s));
+  if (I.mfma) {
+    launch_center(I.Ri, I.d, I.dk, I.bf16 ? (I.bf16x1 ? 1 : 2) : 0, I.rc, I.nr, I.reps32, I.rn32, s);
+    if (hipGetLastError() != hipSuccess) {
+      set_err("centring launch failed");
+      return fail(KNNJ_EDEVICE);
+    }
+  }
+  if (hipEventRecord(c->ev[1], s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+    set_err("index build failed on the device");
+    return fail(KNNJ_EDEVICE);
+  }
+#undef OWN
+  x->Ri = I.Ri;
+  x->rc = I.rc;
+  x->nr = I.nr;
+  x->reps32 = I.reps32;
+  x->rn32 = I.rn32;
+  x->bytes = P.off;
+  x->t_build_ms = elapsed(c->ev[0], c->ev[1]);
+  *out = x;
   return KNNJ_OK;
+}
+
+knnj_status knnj_index_info(const knnj_index *x, knnj_index_desc *info) {
+  if (!x || !info) {
+    set_err("index or output pointer is NULL");
+    return KNNJ_EINVAL;
+  }
+  knnj_index_desc v{};
+  v.n = x->n;
+  v.d = x->d;
+  v.m_r = x->m_r;
+  v.kernel = x->kind;
+  v.precision = x->bf16 ? (x->bf16x1 ? KNNJ_PREC_BF16X1 : KNNJ_PREC_BF16) : KNNJ_PREC_FP32;
+  v.device = x->device;
+  v.bytes = (int64_t)x->bytes;
+  v.t_build_ms = x->t_build_ms;
+  *info = v;
+  return KNNJ_OK;
+}
+
+knnj_status knnj_index_query(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, int32_t k, const knnj_opts *opts,
+                             int32_t *out_idx, float *out_dist, knnj_stats *stats) {
+  g_err.clear();
+  if (!c || !x) {
+    set_err("context or index is NULL");
+    return KNNJ_EINVAL;
+  }
+  if (c->device != x->device) {
+    set_err("index lives on device %d, context on device %d", x->device, c->device);
+    return KNNJ_EINVAL;
+  }
+  knnj_opts o{};
+  if (opts) o = *opts;
+  // the index fixed these: a query may not ask for others
+  o.kernel = x->kind;
+  o.precision = x->bf16 ? (x->bf16x1 ? KNNJ_PREC_BF16X1 : KNNJ_PREC_BF16) : KNNJ_PREC_FP32;
+  o.brute_force = 0;
+  knnj_matrix R{};
+  R.data = nullptr;
+  R.n = x->n;
+  R.d = x->d;
+  R.ld = x->d;
+  R.on_device = 1;
+  {
+    // argument checks of a join with R of the index's shape (R.data unused)
+    knnj_matrix R1 = R;
+    R1.data = (const float *)1;
+    TRY(check_join_args(R1, Q, k, o, out_idx, out_dist));
+  }
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  c->host_cand.clear();
+  Inst I;
+  I.pre = x;
+  I.d = Q.d;
+  I.k = k;
+  I.o = o;
+  I.nR = x->n;
+  I.nQ = Q.n;
+  I.ldQ = Q.ld ? Q.ld : Q.d;
+  (Q.on_device ? I.dQ : I.hQ) = Q.data;
+  I.aos = o.out_aos != 0;
+  I.out_host = !Q.on_device;
+  I.user_idx = out_idx;
+  I.user_dist = out_dist;
+  HIPCHK(hipEventRecord(c->ev[0], s));
+  Alloc A;
+  A.c = c;
+  TRY(stage_inputs(A, I, s));
+  u32 *flags;
+  TRY(ws(c, "flags", 8, &flags));
+  HIPCHK(hipMemsetAsync(flags, 0, 8 * sizeof(u32), s));
+  if (I.nQ > 0) launch_scan_input(I.dQ, I.nQ, I.d, I.ldQ, flags, s);
+  u32 hflags[3];
+  HIPCHK(hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  // combined flags of the index's R and this Q
+  float mq, mr;
+  std::memcpy(&mq, &hflags[1], 4);
+  std::memcpy(&mr, &x->flags[1], 4);
+  const float mx = std::max(mq, mr);
+  std::memcpy(&hflags[1], &mx, 4);
+  hflags[0] |= x->flags[0];
+  hflags[2] |= x->flags[2];
+  TRY(decide(I, hflags));
+  return run_single(c, I, A, stats);
 }
 
 knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *inst, const knnj_opts *opts,
@@ -892,6 +1220,8 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   for (int i = 0; i < n_inst; ++i) {
     describe(I[i], inst[i].R, inst[i].Q, inst[i].k, o, inst[i].out_idx, inst[i].out_dist);
     I[i].id = i;
+    // the fused launches have tiles enough: no automatic split mode
+    if (I[i].o.tile_split == 0) I[i].o.tile_split = 1;
   }
 
   // inputs: one arena for the host copies, one flag quadruple per join, one sync

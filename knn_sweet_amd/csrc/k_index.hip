@@ -399,6 +399,64 @@ void launch_assign_finish(const float *X, int64_t n, int64_t ld, int d, int dpad
                      reps, cl, d2r, radbits, cnt);
 }
 
+// ---------------------------------------------------------------------------
+// Split round of the representative selection.  Uniform sampling leaves
+// some dense regions without a representative (with m draws over g equally
+// weighted components, a fraction ~exp(-m/g) of them); their points then
+// join far-away representatives, whose clusters get radii several times the
+// typical one and survive every triangle-inequality test.  After a first
+// assignment to m0 sampled representatives, the `extra` clusters of largest
+// radius give up their farthest member as a new representative (Gonzalez's
+// farthest-point rule, one point per cluster).
+
+// far[b] = max over members of (distance bits << 32 | row).
+__global__ void k_far_member(const int32_t *__restrict__ cl, const float *__restrict__ d2r, int64_t n, u64 *far) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) atomicMax(&far[cl[i]], ((u64)__float_as_uint(d2r[i]) << 32) | (u64)(u32)i);
+}
+
+// Representatives m0 .. m0 + extra - 1 = farthest members of the `extra`
+// clusters with the largest farthest-member distance (ties: lower cluster).
+// One workgroup of 1024.
+__global__ __launch_bounds__(1024) void k_split_far(const float *__restrict__ X, int64_t ld, int d, int dpad,
+                                                    int m0, int extra, const u64 *__restrict__ far, float *reps) {
+  __shared__ u64 ord[KNNJ_MAX_REPS];
+  int P = 1;
+  while (P < m0) P <<= 1;
+  for (int c = threadIdx.x; c < P; c += blockDim.x)
+    ord[c] = c < m0 ? (((u64)(0xFFFFFFFFu - (u32)(far[c] >> 32)) << 32) | (u32)c) : KEY_MAX;
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int mask = (stride == (size >> 1)) ? (size - 1) : stride;
+      for (int i = threadIdx.x; i < P; i += blockDim.x) {
+        const int j = i ^ mask;
+        if (j > i) {
+          const u64 x = ord[i], y = ord[j];
+          if (y < x) {
+            ord[i] = y;
+            ord[j] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = threadIdx.x; e < extra * dpad; e += blockDim.x) {
+    const int j = e / dpad, t = e % dpad;
+    const int c = (int)(u32)ord[j];
+    const int64_t row = (int64_t)(u32)far[c];
+    reps[(int64_t)(m0 + j) * dpad + t] = t < d ? X[row * ld + t] : 0.f;
+  }
+}
+
+void launch_split_reps(const float *X, int64_t n, int64_t ld, int d, int dpad, int m0, int extra,
+                       const int32_t *cl, const float *d2r, u64 *far, float *reps, hipStream_t s) {
+  hipMemsetAsync(far, 0, sizeof(u64) * m0, s);
+  hipLaunchKernelGGL(k_far_member, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cl, d2r, n, far);
+  hipLaunchKernelGGL(k_split_far, dim3(1), dim3(1024), 0, s, X, ld, d, dpad, m0, extra, (const u64 *)far, reps);
+}
+
 void launch_count(const int32_t *cl, int64_t n, int32_t *cnt, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cl, n, cnt);

@@ -81,8 +81,8 @@ __host__ __device__ constexpr size_t mfma_wave_words(int kcap, int dk) {
 // waves per workgroup: the most of 4, 2, 1 whose LDS fits the 160 KiB of a
 // gfx950 workgroup (plus the union slots)
 __host__ __device__ constexpr int mfma_waves(int kcap, int dk) {
-  return 4 * mfma_wave_words(kcap, dk) * 8 + 256 <= 163840   ? 4
-         : 2 * mfma_wave_words(kcap, dk) * 8 + 256 <= 163840 ? 2
+  return 4 * mfma_wave_words(kcap, dk) * 8 + 1024 <= 163840   ? 4
+         : 2 * mfma_wave_words(kcap, dk) * 8 + 1024 <= 163840 ? 2
                                                                : 1;
 }
 
@@ -116,7 +116,11 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   typedef BOp<PREC> BT;
   extern __shared__ u64 smem[];
   const int wv = W == 1 ? 0 : (int)(threadIdx.x >> 6);
+  // workgroup-shared slots after the W wave regions (1 KiB)
   u32 *uslot = (u32 *)(smem + W * mfma_wave_words(KCAP, DK));  // [2][W] union slots
+  u64 *sthr = (u64 *)(uslot + 64);                              // [16] split mode: shared thresholds
+  float *semax = (float *)(sthr + 16);                          // [W][16] split mode: per-wave E_max
+  float *sevl = semax + W * 16;                                 // [W][16] split mode: evicted bounds
   u64 *lists = smem + wv * mfma_wave_words(KCAP, DK);         // [16][KCAP]
   u64 *bufs = lists + MF_TQ * KCAP;         // [16][MF_BC]
   u64 *tmp = bufs + MF_TQ * MF_BC;          // [KCAP]
@@ -126,9 +130,15 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
   const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
   u64 nvisit = 0;
+  // Split mode (a.split > 1, W > 1): every wave serves the same (up to) 16
+  // queries and takes every W-th block of 16 listed clusters, lowering its
+  // thresholds from the others' published k-th keys (each is an upper bound
+  // of the final one); at the end wave 0 merges the W lists and emits.  It
+  // cuts a tile's latency W-fold where tiles are too few to fill the chip.
+  const bool splitm = W > 1 && a.split > 1;
   // this wave's queries: permuted positions q0 .. q0 + nq - 1
-  const int A = tile.cluster, q0 = tile.start + 16 * wv;
-  const int nq = max(0, min(16, tile.count - 16 * wv));
+  const int A = tile.cluster, q0 = tile.start + (splitm ? 0 : 16 * wv);
+  const int nq = max(0, min(16, tile.count - (splitm ? 0 : 16 * wv)));
   // workgroup-wide OR of a wave-uniform value (double-buffered slots: a
   // slot is rewritten only after the next union's barrier, which every wave
   // reaches after reading it)
@@ -155,6 +165,10 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
 
   const double th = a.theta[A];
   const float T0 = (float)(th * th * (1.0 + 1e-6));
+  if (splitm) {
+    if (threadIdx.x < 16) sthr[threadIdx.x] = ((u64)__float_as_uint(T0) << 32) | 0xFFFFFFFFull;
+    __syncthreads();  // workgroup-uniform: a.split and W are
+  }
   // per-lane state of this lane's four C rows r = 4g + i
   u64 thr[4];
   int cnt[4];
@@ -208,15 +222,9 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   u64 evals = 0, nmerge = 0, npass = 0;
 
   // merge of query r (wave-uniform r)
-  auto merge = [&](int r) {
+  // after list r changed: its k-th key bounds row r's threshold
+  auto absorb = [&](int r, float dropped) {
     const int gi = r >> 2, ii = r & 3;
-    int mine = cnt[0];
-#pragma unroll
-    for (int j = 1; j < 4; ++j)
-      if (j == ii) mine = cnt[j];
-    const int cr = __shfl(mine, 16 * gi);
-    const float dropped = wave_merge<KCAP, MF_BC>(lists + r * KCAP, bufs + r * MF_BC, cr, tmp, make_drop<DropL>(ebase));
-    ++nmerge;
     const u64 kth = lists[r * KCAP + k - 1];
     u64 t = KEY_MAX;
     if (kth != KEY_MAX) t = EXACT ? kth - 1 : (((u64)__float_as_uint(key_dist(kth)) << 32) | 0xFFFFFFFFull);
@@ -229,288 +237,414 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
           evl[j] = fminf(evl[j], dropped);
         }
     }
+    if (splitm && lane == 16 * gi && t != KEY_MAX) atomicMin(sthr + r, t);
+  };
+  auto merge = [&](int r) {
+    const int gi = r >> 2, ii = r & 3;
+    int mine = cnt[0];
+#pragma unroll
+    for (int j = 1; j < 4; ++j)
+      if (j == ii) mine = cnt[j];
+    const int cr = __shfl(mine, 16 * gi);
+    const float dropped = wave_merge<KCAP, MF_BC>(lists + r * KCAP, bufs + r * MF_BC, cr, tmp, make_drop<DropL>(ebase));
+    ++nmerge;
+    absorb(r, dropped);
   };
 
   const int np = a.pcnt[A];
   const int32_t *pl = a.plist + (int64_t)A * a.R.m;
-  for (int pb = 0; pb < np; pb += 16) {
-    // ---- screening of 16 listed clusters: ||q - c||^2 = ||q||^2 + ||c||^2 - 2 q.c
-    // on v_mfma_f32_16x16x4_f32 (rows = queries, columns = representatives),
-    // error <= gE (||q|| + ||c||)^2; a cluster is kept if d(q,c) - r_b can be
-    // <= theta_q for some query.
-    const int pj = pb + c16;
-    const bool lv = pj < np;
-    const int bj = lv ? pl[pj] : pl[pb];
-    // this lane's cluster metadata, read once here and broadcast per visit
-    const int s0j = a.R.off[bj], s1j = a.R.off[bj + 1];
-    const float Rbj = (float)(a.R.rad[bj] * (1.0 + 1e-6));
-    {
-      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-      const float *cr = a.reps32 + (int64_t)bj * dk + 8 * g;
-      for (int s = 0; s < nst; ++s) {
-        float qv[8];
-        q_slice(s, qv);
-        const f32x4_t c0 = *(const f32x4_t *)(cr + 32 * s);
-        const f32x4_t c1 = *(const f32x4_t *)(cr + 32 * s + 4);
+  // q.c for this lane's four rows and the representative of lane c16's
+  // cluster bj (columns), on v_mfma_f32_16x16x4_f32
+  auto center_dots = [&](const int bj) {
+    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+    const float *cr = a.reps32 + (int64_t)bj * dk + 8 * g;
+    for (int s = 0; s < nst; ++s) {
+      float qv[8];
+      q_slice(s, qv);
+      const f32x4_t c0 = *(const f32x4_t *)(cr + 32 * s);
+      const f32x4_t c1 = *(const f32x4_t *)(cr + 32 * s + 4);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[j], c0[j], acc, 0, 0, 0);
+      for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[j], c0[j], acc, 0, 0, 0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[4 + j], c1[j], acc, 0, 0, 0);
+      for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[4 + j], c1[j], acc, 0, 0, 0);
+    }
+    return acc;
+  };
+
+  // Screening and visits of one block of up to 16 clusters: lane c16's
+  // cluster bj, taking part if lv.  lock: the multi-wave lockstep applies.
+  // cslot: the seed slot of lane c16's cluster (-1: not a seed).  The seed
+  // block (seedmode) may serve any row; the sweep skips the seeds (served).
+  auto process_block = [&](const int bj, const bool lv, const bool lock, const int cslot, const bool seedmode) {
+      // ---- screening of 16 listed clusters: ||q - c||^2 = ||q||^2 + ||c||^2 - 2 q.c
+      // on v_mfma_f32_16x16x4_f32 (rows = queries, columns = representatives),
+      // error <= gE (||q|| + ||c||)^2; a cluster is kept if d(q,c) - r_b can be
+      // <= theta_q for some query.
+      // this lane's cluster metadata, read once here and broadcast per visit
+      const int s0j = a.R.off[bj], s1j = a.R.off[bj + 1];
+      const float Rbj = (float)(a.R.rad[bj] * (1.0 + 1e-6));
+      {
+        const f32x4_t acc = center_dots(bj);
+        const float cn = a.rn32[bj];
+        const float rbj = Rbj;
+        bool need = false;
+  #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float sqc = (qnr[i] + cn) - 2.f * acc[i];
+          const float nsum = sqrtf(qnr[i]) + sqrtf(cn);
+          const float dlo = sqrtf(fmaxf(sqc - gE * nsum * nsum, 0.f));
+          const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
+          need |= lv && live[i] && (seedmode || cslot < 0) && !(dlo * (1.f - eps) - rbj > thi);
+          dqs[(4 * g + i) * 16 + c16] = dlo;
+        }
+        need |= __shfl_xor((int)need, 16) != 0;
+        need |= __shfl_xor((int)need, 32) != 0;
+        wave_sync();
+        // clusters any wave of the workgroup needs, visited in one order
+        const u32 own = (u32)wave_ballot(need) & 0xFFFFu;
+        u32 pending = (lock && a.lockstep && !splitm) ? wg_or(own) : own;
+        bool first = true;
+        for (; pending; pending &= pending - 1) {
+          const int jl = __builtin_ctz(pending);
+          // lockstep: every wave finishes the previous cluster before any
+          // starts this one (no-op for W = 1; `first` is workgroup-uniform)
+          if constexpr (W > 1) {
+            if (!first && lock && a.lockstep && !splitm) __syncthreads();
+          }
+          first = false;
+          const int b = __builtin_amdgcn_readlane(bj, jl);
+          const int s0 = __builtin_amdgcn_readlane(s0j, jl), s1 = __builtin_amdgcn_readlane(s1j, jl);
+          const float Rb = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Rbj), jl));
+          // first 64 chunk-start distances of the cluster, requested now so the
+          // load overlaps the A-operand build below
+          const int nch = (s1 - s0 + 63) >> 6;
+          const float cmin0 = lane < nch ? a.R.pdist[s0 + 64 * lane] : INFINITY;
+          const float cnext0 = (lane == 63 && nch > 64) ? a.R.pdist[s0 + 64 * 64] : INFINITY;
+          // re-check with the thresholds as they stand now
+          bool use[4];
+          bool anyl = false;
+  #pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float dlo = dqs[(4 * g + i) * 16 + jl];
+            const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
+            use[i] = live[i] && !(dlo * (1.f - eps) - Rb > thi);
+            anyl |= use[i];
+          }
+          if (wave_ballot(anyl) == 0) continue;
+          ++nvisit;
+          // A operand: q - c for row c16, dimensions of k-group g
+          const float *rep = a.reps32 + (int64_t)b * dk + 8 * g;
+          typename std::conditional<BF16, bf16x8_t, float[8]>::type av[NS];
+          bf16x8_t avl[SPLIT ? NS : 1];
+          float part = 0.f;
+  #pragma unroll
+          for (int s = 0; s < NS; ++s) {
+            if (s < nst) {
+              const f32x4_t c0 = *(const f32x4_t *)(rep + 32 * s);
+              const f32x4_t c1 = *(const f32x4_t *)(rep + 32 * s + 4);
+              float qv[8];
+              q_slice(s, qv);
+  #pragma unroll
+              for (int j = 0; j < 8; ++j) {
+                const float v = alive ? qv[j] - (j < 4 ? c0[j] : c1[j - 4]) : 0.f;
+                part = __builtin_fmaf(v, v, part);
+                if constexpr (BF16) {
+                  const __bf16 h = (__bf16)v;
+                  av[s][j] = h;
+                  if constexpr (SPLIT) avl[s][j] = (__bf16)(v - (float)h);
+                } else {
+                  av[s][j] = v;
+                }
+              }
+            } else {
+  #pragma unroll
+              for (int j = 0; j < 8; ++j) {
+                if constexpr (BF16) {
+                  av[s][j] = (__bf16)0.f;
+                  if constexpr (SPLIT) avl[s][j] = (__bf16)0.f;
+                } else {
+                  av[s][j] = 0.f;
+                }
+              }
+            }
+          }
+          part += __shfl_xor(part, 16);
+          part += __shfl_xor(part, 32);
+          if (g == 0) fsh[c16] = part;
+          wave_sync();
+          float nqr[4];
+  #pragma unroll
+          for (int i = 0; i < 4; ++i) nqr[i] = fsh[4 * g + i];
+          wave_sync();
+
+          float Ei[4], dqbv[4];
+          u32 ecd[4];
+          float lo = INFINITY, hi = -INFINITY;
+  #pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float dqb = sqrtf(nqr[i]);
+            dqbv[i] = dqb;
+            const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
+            use[i] = use[i] && !(dqb * (1.f - 2.f * eps) - Rb > thi);
+            Ei[i] = EXACT ? 0.f : a.c1 * dqb * (1.f + eps) * Rb + a.c2 * (nqr[i] + Rb * Rb);
+            ecd[i] = (!EXACT && a.ecode) ? (ecode_encode(Ei[i], ebase) << KEY_ROW_BITS) : 0u;
+            if (use[i]) {
+              lo = fminf(lo, dqb * (1.f - 2.f * eps) - thi);
+              hi = fmaxf(hi, dqb * (1.f + 2.f * eps) + thi);
+            }
+          }
+          lo = wave_min(lo);
+          hi = wave_max(hi);
+          if (!(lo <= hi)) continue;
+          lo = lo > 0.f ? lo * (1.f - eps) : -INFINITY;
+          hi = hi * (1.f + eps);
+
+          for (int cg = 0; cg < nch; cg += 64) {
+            const int c = cg + lane;
+            const float cmin = cg == 0 ? cmin0 : (c < nch ? a.R.pdist[s0 + 64 * c] : INFINITY);
+            float nmin = __shfl_down(cmin, 1);
+            if (lane == 63)
+              nmin = cg == 0 ? cnext0 : ((c + 1 < nch) ? a.R.pdist[s0 + 64 * (c + 1)] : INFINITY);
+            const bool needc = (c < nch) && (cmin <= hi) && (nmin >= lo);
+            const u64 m = wave_ballot(needc);
+            if (m == 0) {
+              if (__shfl(cmin, 0) > hi) break;
+              continue;
+            }
+            const int pbeg = s0 + 64 * (cg + __builtin_ctzll(m));
+            const int pend = min(s1, s0 + 64 * (cg + 64 - __builtin_clzll(m)));
+            // B operands, norms and rows of NB 16-column blocks, loaded
+            // together so one memory latency covers NB blocks of MFMA work
+            struct Group {
+              BT bo[NB][NS];
+              float nrv[NB];
+              u32 rowv[NB];
+              bool validv[NB];
+            };
+            auto load_group = [&](int g0, Group &G) {
+  #pragma unroll
+              for (int nb = 0; nb < NB; ++nb) {
+                const int p = g0 + 16 * nb + c16;
+                G.validv[nb] = p < pend;
+                const int pp = G.validv[nb] ? p : g0;
+  #pragma unroll
+                for (int s = 0; s < NS; ++s)
+                  if (s < nst) {
+                    if constexpr (SPLIT) {
+                      const __bf16 *rr = (const __bf16 *)a.rc + (int64_t)pp * 2 * dk + 8 * g + 32 * s;
+                      G.bo[nb][s].h = *(const bf16x8_t *)rr;
+                      G.bo[nb][s].l = *(const bf16x8_t *)(rr + dk);
+                    } else if constexpr (BF16) {
+                      G.bo[nb][s].h = *(const bf16x8_t *)((const __bf16 *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s);
+                    } else {
+                      const float *rr = (const float *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s;
+                      G.bo[nb][s].f0 = *(const f32x4_t *)rr;
+                      G.bo[nb][s].f1 = *(const f32x4_t *)(rr + 4);
+                    }
+                  }
+                G.nrv[nb] = a.nr[pp];
+                G.rowv[nb] = EXACT ? (u32)a.R.perm[pp] : (u32)pp;
+              }
+            };
+            auto compute_group = [&](int g0, Group &G) {
+              if constexpr (!EXACT) {
+                // rows are sorted by distance to the representative, so the
+                // group's last row bounds ||r - c|| for all of it: a tighter
+                // R in the error bound than the cluster radius
+                const float pd = a.R.pdist[min(g0 + 16 * NB, pend) - 1];
+                const float Rg = fminf(Rb, pd * (1.f + 1e-6f));
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                  Ei[i] = a.c1 * dqbv[i] * (1.f + eps) * Rg + a.c2 * (nqr[i] + Rg * Rg);
+                  ecd[i] = a.ecode ? (ecode_encode(Ei[i], ebase) << KEY_ROW_BITS) : 0u;
+                }
+              }
+              // compute stage
+  #pragma unroll
+              for (int nb = 0; nb < NB; ++nb) {
+                if (g0 + 16 * nb >= pend) break;
+                const bool valid = G.validv[nb];
+                f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+  #pragma unroll
+                for (int s = 0; s < NS; ++s)
+                  if (s < nst) {
+                    if constexpr (SPLIT) {
+                      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(avl[s], G.bo[nb][s].h, acc, 0, 0, 0);
+                      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].l, acc, 0, 0, 0);
+                      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].h, acc, 0, 0, 0);
+                    } else if constexpr (BF16) {
+                      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].h, acc, 0, 0, 0);
+                    } else {
+  #pragma unroll
+                      for (int j = 0; j < 4; ++j)
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][j], G.bo[nb][s].f0[j], acc, 0, 0, 0);
+  #pragma unroll
+                      for (int j = 0; j < 4; ++j)
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][4 + j], G.bo[nb][s].f1[j], acc, 0, 0, 0);
+                    }
+                  }
+                const float nrb = G.nrv[nb];
+                const u32 row = G.rowv[nb];
+                bool pass[4];
+                u64 key[4];
+                bool anyp = false;
+  #pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                  const float sf = fmaxf((nqr[i] + nrb) - 2.f * acc[i], 0.f);
+                  if (EXACT) {
+                    key[i] = make_key(sf, row);
+                    pass[i] = valid && use[i] && key[i] <= thr[i];
+                  } else {
+                    key[i] = make_key(sf + Ei[i], row | ecd[i]);
+                    pass[i] = valid && use[i] && (sf - Ei[i]) <= key_dist(thr[i]);
+                  }
+                  anyp |= pass[i];
+                  evals += __popcll(wave_ballot(valid && use[i]));
+                }
+                if (wave_ballot(anyp) != 0) {
+                  const u32 below = (1u << c16) - 1u;
+  #pragma unroll
+                  for (int i = 0; i < 4; ++i) {
+                    const u64 mk = wave_ballot(pass[i]);
+                    const u32 gm = (u32)(mk >> (16 * g)) & 0xFFFFu;
+                    if (pass[i]) bufs[(4 * g + i) * MF_BC + cnt[i] + __popc(gm & below)] = key[i];
+                    cnt[i] += __popc(gm);
+                    npass += __popcll(mk);
+                    // every entry ever appended for this row has E <= emax
+                    if (gm) emax[i] = fmaxf(emax[i], Ei[i]);
+                  }
+                  // a 16-column block appends at most 16 keys per query, so a
+                  // buffer below MF_BC - 16 has room for the next block
+                  constexpr int WM = MF_BC - 16;
+                  const bool hot = cnt[0] >= WM || cnt[1] >= WM || cnt[2] >= WM || cnt[3] >= WM;
+                  if (wave_ballot(hot) != 0) {
+                    wave_sync();
+                    for (int r = 0; r < MF_TQ; ++r) {
+                      const int gi = r >> 2, ii = r & 3;
+                      int mine = cnt[0];
+  #pragma unroll
+                      for (int j = 1; j < 4; ++j)
+                        if (j == ii) mine = cnt[j];
+                      if (__shfl(mine, 16 * gi) >= WM) merge(r);
+                    }
+                  }
+                }
+              }
+            };
+            constexpr int GS = 16 * NB;
+            if constexpr (DB) {
+              Group GA, GB;
+              load_group(pbeg, GA);
+              for (int g0 = pbeg; g0 < pend; g0 += 2 * GS) {
+                const int g1 = g0 + GS;
+                if (g1 < pend) load_group(g1, GB);
+                compute_group(g0, GA);
+                if (g1 >= pend) break;
+                if (g1 + GS < pend) load_group(g1 + GS, GA);
+                compute_group(g1, GB);
+              }
+            } else {
+              Group G0;
+              for (int g0 = pbeg; g0 < pend; g0 += GS) {
+                load_group(g0, G0);
+                compute_group(g0, G0);
+              }
+            }
+          }
+        }
       }
+  
+  };
+
+  // ---- seed: each query's nearest listed cluster (by centre distance) is
+  // visited first, so thresholds are tight before the sweep; a query cluster
+  // mixing several dense regions would otherwise start from its loose
+  // cluster bound and visit everything its sweep meets first.
+  int seedv = -1;  // lane c16: seed cluster of row c16, -1 if none or a duplicate
+  {
+    float bestd[4];
+    int bestj[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      bestd[i] = INFINITY;
+      bestj[i] = -1;
+    }
+    for (int pb = 0; pb < np; pb += 16) {
+      const int pj = pb + c16;
+      const bool lv = pj < np;
+      const int bj = lv ? pl[pj] : pl[pb];
+      const f32x4_t acc = center_dots(bj);
       const float cn = a.rn32[bj];
-      const float rbj = Rbj;
-      bool need = false;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float sqc = (qnr[i] + cn) - 2.f * acc[i];
-        const float nsum = sqrtf(qnr[i]) + sqrtf(cn);
-        const float dlo = sqrtf(fmaxf(sqc - gE * nsum * nsum, 0.f));
-        const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
-        need |= lv && live[i] && !(dlo * (1.f - eps) - rbj > thi);
-        dqs[(4 * g + i) * 16 + c16] = dlo;
-      }
-      need |= __shfl_xor((int)need, 16) != 0;
-      need |= __shfl_xor((int)need, 32) != 0;
-      wave_sync();
-      // clusters any wave of the workgroup needs, visited in one order
-      u32 pending = wg_or((u32)wave_ballot(need) & 0xFFFFu);
-      bool first = true;
-      for (; pending; pending &= pending - 1) {
-        const int jl = __builtin_ctz(pending);
-        // lockstep: every wave finishes the previous cluster before any
-        // starts this one (no-op for W = 1; `first` is workgroup-uniform)
-        if constexpr (W > 1) {
-          if (!first) __syncthreads();
-        }
-        first = false;
-        const int b = __builtin_amdgcn_readlane(bj, jl);
-        const int s0 = __builtin_amdgcn_readlane(s0j, jl), s1 = __builtin_amdgcn_readlane(s1j, jl);
-        const float Rb = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Rbj), jl));
-        // first 64 chunk-start distances of the cluster, requested now so the
-        // load overlaps the A-operand build below
-        const int nch = (s1 - s0 + 63) >> 6;
-        const float cmin0 = lane < nch ? a.R.pdist[s0 + 64 * lane] : INFINITY;
-        const float cnext0 = (lane == 63 && nch > 64) ? a.R.pdist[s0 + 64 * 64] : INFINITY;
-        // re-check with the thresholds as they stand now
-        bool use[4];
-        bool anyl = false;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float dlo = dqs[(4 * g + i) * 16 + jl];
-          const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
-          use[i] = live[i] && !(dlo * (1.f - eps) - Rb > thi);
-          anyl |= use[i];
-        }
-        if (wave_ballot(anyl) == 0) continue;
-        ++nvisit;
-        // A operand: q - c for row c16, dimensions of k-group g
-        const float *rep = a.reps32 + (int64_t)b * dk + 8 * g;
-        typename std::conditional<BF16, bf16x8_t, float[8]>::type av[NS];
-        bf16x8_t avl[SPLIT ? NS : 1];
-        float part = 0.f;
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-          if (s < nst) {
-            const f32x4_t c0 = *(const f32x4_t *)(rep + 32 * s);
-            const f32x4_t c1 = *(const f32x4_t *)(rep + 32 * s + 4);
-            float qv[8];
-            q_slice(s, qv);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-              const float v = alive ? qv[j] - (j < 4 ? c0[j] : c1[j - 4]) : 0.f;
-              part = __builtin_fmaf(v, v, part);
-              if constexpr (BF16) {
-                const __bf16 h = (__bf16)v;
-                av[s][j] = h;
-                if constexpr (SPLIT) avl[s][j] = (__bf16)(v - (float)h);
-              } else {
-                av[s][j] = v;
-              }
-            }
-          } else {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-              if constexpr (BF16) {
-                av[s][j] = (__bf16)0.f;
-                if constexpr (SPLIT) avl[s][j] = (__bf16)0.f;
-              } else {
-                av[s][j] = 0.f;
-              }
-            }
-          }
-        }
-        part += __shfl_xor(part, 16);
-        part += __shfl_xor(part, 32);
-        if (g == 0) fsh[c16] = part;
-        wave_sync();
-        float nqr[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) nqr[i] = fsh[4 * g + i];
-        wave_sync();
-
-        float Ei[4];
-        u32 ecd[4];
-        float lo = INFINITY, hi = -INFINITY;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float dqb = sqrtf(nqr[i]);
-          const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
-          use[i] = use[i] && !(dqb * (1.f - 2.f * eps) - Rb > thi);
-          Ei[i] = EXACT ? 0.f : a.c1 * dqb * (1.f + eps) * Rb + a.c2 * (nqr[i] + Rb * Rb);
-          ecd[i] = (!EXACT && a.ecode) ? (ecode_encode(Ei[i], ebase) << KEY_ROW_BITS) : 0u;
-          if (use[i]) {
-            lo = fminf(lo, dqb * (1.f - 2.f * eps) - thi);
-            hi = fmaxf(hi, dqb * (1.f + 2.f * eps) + thi);
-          }
-        }
-        lo = wave_min(lo);
-        hi = wave_max(hi);
-        if (!(lo <= hi)) continue;
-        lo = lo > 0.f ? lo * (1.f - eps) : -INFINITY;
-        hi = hi * (1.f + eps);
-
-        for (int cg = 0; cg < nch; cg += 64) {
-          const int c = cg + lane;
-          const float cmin = cg == 0 ? cmin0 : (c < nch ? a.R.pdist[s0 + 64 * c] : INFINITY);
-          float nmin = __shfl_down(cmin, 1);
-          if (lane == 63)
-            nmin = cg == 0 ? cnext0 : ((c + 1 < nch) ? a.R.pdist[s0 + 64 * (c + 1)] : INFINITY);
-          const bool needc = (c < nch) && (cmin <= hi) && (nmin >= lo);
-          const u64 m = wave_ballot(needc);
-          if (m == 0) {
-            if (__shfl(cmin, 0) > hi) break;
-            continue;
-          }
-          const int pbeg = s0 + 64 * (cg + __builtin_ctzll(m));
-          const int pend = min(s1, s0 + 64 * (cg + 64 - __builtin_clzll(m)));
-          // B operands, norms and rows of NB 16-column blocks, loaded
-          // together so one memory latency covers NB blocks of MFMA work
-          struct Group {
-            BT bo[NB][NS];
-            float nrv[NB];
-            u32 rowv[NB];
-            bool validv[NB];
-          };
-          auto load_group = [&](int g0, Group &G) {
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-              const int p = g0 + 16 * nb + c16;
-              G.validv[nb] = p < pend;
-              const int pp = G.validv[nb] ? p : g0;
-#pragma unroll
-              for (int s = 0; s < NS; ++s)
-                if (s < nst) {
-                  if constexpr (SPLIT) {
-                    const __bf16 *rr = (const __bf16 *)a.rc + (int64_t)pp * 2 * dk + 8 * g + 32 * s;
-                    G.bo[nb][s].h = *(const bf16x8_t *)rr;
-                    G.bo[nb][s].l = *(const bf16x8_t *)(rr + dk);
-                  } else if constexpr (BF16) {
-                    G.bo[nb][s].h = *(const bf16x8_t *)((const __bf16 *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s);
-                  } else {
-                    const float *rr = (const float *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s;
-                    G.bo[nb][s].f0 = *(const f32x4_t *)rr;
-                    G.bo[nb][s].f1 = *(const f32x4_t *)(rr + 4);
-                  }
-                }
-              G.nrv[nb] = a.nr[pp];
-              G.rowv[nb] = EXACT ? (u32)a.R.perm[pp] : (u32)pp;
-            }
-          };
-          auto compute_group = [&](int g0, Group &G) {
-            // compute stage
-#pragma unroll
-            for (int nb = 0; nb < NB; ++nb) {
-              if (g0 + 16 * nb >= pend) break;
-              const bool valid = G.validv[nb];
-              f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-              for (int s = 0; s < NS; ++s)
-                if (s < nst) {
-                  if constexpr (SPLIT) {
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(avl[s], G.bo[nb][s].h, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].l, acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].h, acc, 0, 0, 0);
-                  } else if constexpr (BF16) {
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].h, acc, 0, 0, 0);
-                  } else {
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][j], G.bo[nb][s].f0[j], acc, 0, 0, 0);
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][4 + j], G.bo[nb][s].f1[j], acc, 0, 0, 0);
-                  }
-                }
-              const float nrb = G.nrv[nb];
-              const u32 row = G.rowv[nb];
-              bool pass[4];
-              u64 key[4];
-              bool anyp = false;
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                const float sf = fmaxf((nqr[i] + nrb) - 2.f * acc[i], 0.f);
-                if (EXACT) {
-                  key[i] = make_key(sf, row);
-                  pass[i] = valid && use[i] && key[i] <= thr[i];
-                } else {
-                  key[i] = make_key(sf + Ei[i], row | ecd[i]);
-                  pass[i] = valid && use[i] && (sf - Ei[i]) <= key_dist(thr[i]);
-                }
-                anyp |= pass[i];
-                evals += __popcll(wave_ballot(valid && use[i]));
-              }
-              if (wave_ballot(anyp) != 0) {
-                const u32 below = (1u << c16) - 1u;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                  const u64 mk = wave_ballot(pass[i]);
-                  const u32 gm = (u32)(mk >> (16 * g)) & 0xFFFFu;
-                  if (pass[i]) bufs[(4 * g + i) * MF_BC + cnt[i] + __popc(gm & below)] = key[i];
-                  cnt[i] += __popc(gm);
-                  npass += __popcll(mk);
-                  // every entry ever appended for this row has E <= emax
-                  if (gm) emax[i] = fmaxf(emax[i], Ei[i]);
-                }
-                // a 16-column block appends at most 16 keys per query, so a
-                // buffer below MF_BC - 16 has room for the next block
-                constexpr int WM = MF_BC - 16;
-                const bool hot = cnt[0] >= WM || cnt[1] >= WM || cnt[2] >= WM || cnt[3] >= WM;
-                if (wave_ballot(hot) != 0) {
-                  wave_sync();
-                  for (int r = 0; r < MF_TQ; ++r) {
-                    const int gi = r >> 2, ii = r & 3;
-                    int mine = cnt[0];
-#pragma unroll
-                    for (int j = 1; j < 4; ++j)
-                      if (j == ii) mine = cnt[j];
-                    if (__shfl(mine, 16 * gi) >= WM) merge(r);
-                  }
-                }
-              }
-            }
-          };
-          constexpr int GS = 16 * NB;
-          if constexpr (DB) {
-            Group GA, GB;
-            load_group(pbeg, GA);
-            for (int g0 = pbeg; g0 < pend; g0 += 2 * GS) {
-              const int g1 = g0 + GS;
-              if (g1 < pend) load_group(g1, GB);
-              compute_group(g0, GA);
-              if (g1 >= pend) break;
-              if (g1 + GS < pend) load_group(g1 + GS, GA);
-              compute_group(g1, GB);
-            }
-          } else {
-            Group G0;
-            for (int g0 = pbeg; g0 < pend; g0 += GS) {
-              load_group(g0, G0);
-              compute_group(g0, G0);
-            }
-          }
+        if (lv && sqc < bestd[i]) {
+          bestd[i] = sqc;
+          bestj[i] = bj;
         }
       }
     }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      for (int off = 1; off < 16; off <<= 1) {
+        const float od = __shfl_xor(bestd[i], off);
+        const int oj = __shfl_xor(bestj[i], off);
+        if (od < bestd[i] || (od == bestd[i] && oj < bestj[i])) {
+          bestd[i] = od;
+          bestj[i] = oj;
+        }
+      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int v = __shfl(bestj[i], 16 * (c16 >> 2));
+      if ((c16 & 3) == i) seedv = v;
+    }
+    if (c16 >= nq) seedv = -1;
+    // one slot per distinct seed: its first row's lane
+    int first = c16;
+    for (int j = 0; j < 15; ++j) {
+      const int vj = __shfl(seedv, j);
+      if (j < first && vj == seedv) first = j;
+    }
+    if (first != c16) seedv = -1;
+    // split mode: wave 0 visits the seeds and publishes its thresholds
+    // (cluster 0 stands in for lanes without a seed: never visited)
+    if ((!splitm || wv == 0) && np > 0) {
+      process_block(seedv >= 0 ? seedv : 0, seedv >= 0, false, c16, true);
+      if (splitm) {
+        wave_sync();
+        for (int r = 0; r < MF_TQ; ++r) {
+          const int gi = r >> 2, ii = r & 3;
+          int mine = cnt[0];
+#pragma unroll
+          for (int j = 1; j < 4; ++j)
+            if (j == ii) mine = cnt[j];
+          if (__shfl(mine, 16 * gi) > 0) merge(r);
+        }
+      }
+    }
+    if (splitm) __syncthreads();
+  }
+  // seed slot of cluster bj, -1 if it is no row's seed
+  auto seed_slot = [&](const int bj) {
+    int cs = -1;
+    for (int j = 0; j < 16; ++j)
+      if (__builtin_amdgcn_readlane(seedv, j) == bj) cs = j;
+    return cs;
+  };
+
+  for (int pb = splitm ? 16 * wv : 0; pb < np; pb += splitm ? 16 * W : 16) {
+    if (splitm) {
+      // thresholds the other waves have reached
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const u64 t = __atomic_load_n(sthr + 4 * g + i, __ATOMIC_RELAXED);
+        if (t < thr[i]) thr[i] = t;
+      }
+    }
+    const int pj = pb + c16;
+    const int bj = pj < np ? pl[pj] : pl[pb];
+    process_block(bj, pj < np, true, seed_slot(bj), false);
   }
   wave_sync();
   for (int r = 0; r < MF_TQ; ++r) {
@@ -522,8 +656,37 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     if (__shfl(mine, 16 * gi) > 0) merge(r);
   }
   wave_sync();
+  if (splitm) {
+    // wave 0 takes over the other waves' lists, E_max and evicted bounds
+    if (c16 == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        semax[wv * 16 + 4 * g + i] = emax[i];
+        sevl[wv * 16 + 4 * g + i] = evl[i];
+      }
+    }
+    __syncthreads();
+    if (wv == 0) {
+      for (int w = 1; w < W; ++w) {
+        u64 *lw = smem + w * mfma_wave_words(KCAP, DK);
+        for (int r = 0; r < nq; ++r) {
+          const float dropped =
+              wave_merge<KCAP, KCAP>(lists + r * KCAP, lw + r * KCAP, KCAP, tmp, make_drop<DropL>(ebase));
+          absorb(r, dropped);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          emax[i] = fmaxf(emax[i], semax[w * 16 + 4 * g + i]);
+          evl[i] = fminf(evl[i], sevl[w * 16 + 4 * g + i]);
+        }
+      }
+      wave_sync();
+    }
+  }
+  // in split mode only wave 0 holds the merged lists
+  const int nemit = (splitm && wv != 0) ? 0 : nq;
 
-  for (int r = 0; r < nq; ++r) {
+  for (int r = 0; r < nemit; ++r) {
     const int gi = r >> 2, ii = r & 3;
     u64 t = thr[0];
     float em = emax[0], ev = evl[0];
@@ -540,7 +703,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     const float cutoff = EXACT ? Tr : (Tr + 2.f * Er) * (1.f + 2.4e-7f);
     emit_query<KCAP, EXACT>(a, lists + r * KCAP, q0 + r, A, cutoff, a.Q.px + (int64_t)(q0 + r) * dpQ, Tr, Vr);
   }
-  if (a.tile_prof) {
+  if (a.tile_prof && wv == 0) {
     // diagnostic: the largest and smallest final threshold of the tile's rows
     float tmax = 0.f, tmin = INFINITY;
 #pragma unroll
@@ -590,7 +753,7 @@ __global__ __launch_bounds__(64 * W) void k_join_mfma_multi(const JoinArgs *__re
 inline int mfma_launch_waves(int kcap, int dkmax, int want) { return want > 1 ? mfma_waves(kcap, dkmax) : 1; }
 
 inline size_t mfma_lds(int kcap, int dkmax, int waves) {
-  return waves * mfma_wave_words(kcap, dkmax) * sizeof(u64) + 256;
+  return waves * mfma_wave_words(kcap, dkmax) * sizeof(u64) + 1024;
 }
 
 template <int PREC>

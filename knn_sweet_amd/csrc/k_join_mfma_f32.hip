@@ -1,6 +1,8 @@
 // k_join_mfma_f32.hip -- plain FP32 MFMA join kernels (v_mfma_f32_16x16x4_f32).
 #include "k_join_mfma.h"
 
+int join_mfma_split_waves(int kcap, int dk) { return mfma_waves(kcap, dk > 128 ? 256 : 128); }
+
 int join_mfma_tile_q(int kcap, int dk, int want_waves) {
   return MF_TQ * mfma_launch_waves(kcap, dk > 128 ? 256 : 128, want_waves);
 }

@@ -16,6 +16,10 @@ void launch_assign(const float *X, int64_t n, int64_t ld, int d, int dpad, const
 void launch_assign_finish(const float *X, int64_t n, int64_t ld, int d, int dpad, const float *reps,
                           const int32_t *cl, float *d2r, u64 *radbits, int32_t *cnt, hipStream_t s);
 void launch_count(const int32_t *cl, int64_t n, int32_t *cnt, hipStream_t s);
+// split round: representatives m0 .. m0+extra-1 = farthest members of the
+// extra widest clusters (cl, d2r from an assignment to the first m0; far: m0 scratch)
+void launch_split_reps(const float *X, int64_t n, int64_t ld, int d, int dpad, int m0, int extra,
+                       const int32_t *cl, const float *d2r, u64 *far, float *reps, hipStream_t s);
 void launch_centroid(const float *X, int64_t ld, int d, int dpad, int m, const int32_t *off, const int32_t *perm,
                      int integral, float *reps, hipStream_t s);
 void launch_bucket_rows(const int32_t *cl, int64_t n, int m, const int32_t *cnt, int32_t *off, int32_t *fill,
@@ -67,6 +71,8 @@ struct JoinArgs {
   int ecode;             // MFMA filter keys carry per-entry error codes (rows < 2^26)
   int ebase;             // exponent base of those codes
   int waves;             // MFMA join: > 1 asks for the multi-wave workgroup (tiles of 16 x waves queries)
+  int lockstep;          // multi-wave workgroup: 1 = waves walk the union of their cluster lists together
+  int split;             // multi-wave workgroup: > 1 = its waves split one 16-query tile's cluster list
 };
 
 // Picks the wave tile size for a candidate capacity; returns the dynamic LDS bytes.
@@ -86,6 +92,8 @@ hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles
 // queries per workgroup tile of the MFMA join (16 per wave; want_waves > 1
 // asks for the multi-wave workgroup)
 int join_mfma_tile_q(int kcap, int dk, int want_waves);
+// waves of the MFMA join's split mode for (kcap, dk): 1 if it does not fit
+int join_mfma_split_waves(int kcap, int dk);
 
 // Fused launches over the tiles of many instances: slot b runs tile tiles[b]
 // of instance tiles[b].pad with arguments argv[tiles[b].pad]; proto carries

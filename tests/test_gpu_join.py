@@ -191,3 +191,28 @@ def test_xcd_tile_placement(ctx):
     b = ctx.join(R, Q, 16, xcd_tiles=True)
     assert (a[0] == b[0]).all() and (a[1] == b[1]).all()
     assert_matches_oracle(R, Q, 16, b[0], b[1])
+
+
+@pytest.mark.parametrize("d,k,prec", [(64, 32, kj.PREC_FP32), (96, 50, kj.PREC_BF16), (128, 200, kj.PREC_BF16),
+                                      (64, 16, kj.PREC_BF16X1)])
+def test_split_mode_matches_single_wave_tiles(ctx, d, k, prec):
+    """Split mode (the waves of a workgroup share one tile's cluster list,
+    thresholds shared through LDS, lists merged at the end) and one-wave
+    tiles give the oracle's answer."""
+    mix = datagen.mixture64_model(seed=300 + d, d=d, comps=40)
+    R = mix.sample(datagen.rng(d + 301), 12000)
+    Q = mix.sample(datagen.rng(d + 302), 700)
+    a_idx, a_dist, sa = ctx.join(R, Q, k, precision=prec, tile_split=4)
+    b_idx, b_dist, sb = ctx.join(R, Q, k, precision=prec, tile_split=1)
+    assert sa["tile_q"] == 16 and sb["tile_q"] == 16
+    assert np.array_equal(a_idx, b_idx) and np.array_equal(a_dist, b_dist)
+    assert_matches_oracle(R, Q, k, a_idx, a_dist)
+
+
+def test_split_mode_exact_integer(ctx):
+    R = datagen.skin_like(9000, seed=310)
+    Q = datagen.skin_like(800, seed=311)
+    # integer inputs small enough for exact MFMA keys
+    idx, dist, st = ctx.join(R / 1, Q / 1, 100, kernel=kj.KERNEL_MFMA, tile_split=4)
+    assert st["exact_mode"] == 1
+    assert_matches_oracle(R, Q, 100, idx, dist)
