@@ -54,9 +54,25 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
+
+
+CLI = os.path.join(HERE, "knnjoin")
+
+
+def build_cli() -> str:
+    """The reference-compatible command-line driver, linked against libknnj.so."""
+    src = os.path.join(HERE, "cli", "knnjoin.cpp")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(os.path.dirname(HERE), 'include')}", src,
+           f"-L{HERE}", "-lknnj", "-Wl,-rpath,$ORIGIN", "-o", CLI + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"knnjoin build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(CLI + ".tmp", CLI)
+    return CLI
 
 
 if __name__ == "__main__":
