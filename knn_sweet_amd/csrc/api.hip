@@ -72,6 +72,11 @@ struct knnj_ctx {
   // representative draws of the current call, kept alive for the async H2D
   std::vector<std::vector<int32_t>> host_cand;
   hipEvent_t ev[10] = {};
+  // knnj_join_batch: streams the instances' index builds are spread over
+  // (default priority, created on first use) and their completion events
+  static constexpr int kAux = 4;
+  hipStream_t aux[kAux] = {};
+  hipEvent_t aux_ev[kAux + 1] = {};
 };
 
 #define HIPCHK(expr)                                                                      \
@@ -919,6 +924,10 @@ void knnj_ctx_destroy(knnj_ctx *c) {
     if (kv.second.p) hipFree(kv.second.p);
   for (auto &e : c->ev)
     if (e) hipEventDestroy(e);
+  for (auto &e : c->aux_ev)
+    if (e) hipEventDestroy(e);
+  for (auto &x : c->aux)
+    if (x) hipStreamDestroy(x);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1294,7 +1303,31 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   }
 
   HIPCHK(hipEventRecord(ev[1], s));
-  for (int id : order) TRY(enqueue_index(c, I[id], s, nullptr, nullptr));
+  // index builds, bounds and tile lists: independent per instance, so they
+  // are spread over the auxiliary streams and overlap on the device (most
+  // are many small launches); the fused joins wait for all of them
+  if (!c->aux[0]) {
+    for (auto &x : c->aux) HIPCHK(hipStreamCreate(&x));
+    for (auto &e : c->aux_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  HIPCHK(hipEventRecord(c->aux_ev[knnj_ctx::kAux], s));
+  for (int j = 0; j < knnj_ctx::kAux; ++j) HIPCHK(hipStreamWaitEvent(c->aux[j], c->aux_ev[knnj_ctx::kAux], 0));
+  {
+    // largest instances first, each to the stream with the least queued rows
+    std::vector<int> by_size(order);
+    std::stable_sort(by_size.begin(), by_size.end(),
+                     [&](int a, int b) { return I[a].nR + I[a].nQ > I[b].nR + I[b].nQ; });
+    int64_t load[knnj_ctx::kAux] = {};
+    for (int id : by_size) {
+      const int j = (int)(std::min_element(load, load + knnj_ctx::kAux) - load);
+      load[j] += I[id].nR + I[id].nQ;
+      TRY(enqueue_index(c, I[id], c->aux[j], nullptr, nullptr));
+    }
+  }
+  for (int j = 0; j < knnj_ctx::kAux; ++j) {
+    HIPCHK(hipEventRecord(c->aux_ev[j], c->aux[j]));
+    HIPCHK(hipStreamWaitEvent(s, c->aux_ev[j], 0));
+  }
   HIPCHK(hipEventRecord(ev[2], s));
   JoinArgs *dargs;
   TRY(ws(c, "batch_args", (size_t)n_inst, &dargs));
