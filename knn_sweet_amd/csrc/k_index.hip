@@ -249,7 +249,7 @@ __global__ void k_count(const int32_t *__restrict__ cl, int64_t n, int32_t *cnt)
 // x and c rounded to bf16 and FP32 accumulation (v_mfma_f32_16x16x32_bf16).
 // The argmin only has to be good, not exact: every later bound uses the
 // binary64 distance to whichever representative was chosen.  One wave takes
-// 32 rows (two 16-row A blocks) against all representatives, 16 at a time;
+// 64 rows (four 16-row A blocks) against all representatives, 16 at a time;
 // ties go to the lower representative index.
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 as_bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float as_f32x4;
@@ -258,78 +258,87 @@ template <int DK>
 __global__ __launch_bounds__(256) void k_assign_mfma(const float *__restrict__ X, int64_t n, int64_t ld, int d,
                                                      const __bf16 *__restrict__ rb, const float *__restrict__ rn,
                                                      int m, int dk, int32_t *cl) {
-  constexpr int NS = DK / 32;
+  // one wave: AB = 4 blocks of 16 rows against 16 representatives at a time;
+  // the next 16 representatives' operands are loaded while this block's
+  // MFMAs run
+  constexpr int NS = DK / 32, AB = 4;
   const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
-  const int64_t base = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
+  const int64_t base = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * AB);
   if (base >= n) return;
   const int nst = dk / 32;
-  as_bf16x8 a0[NS], a1[NS];
-  const int64_t p0 = base + c16, p1 = base + 16 + c16;
+  as_bf16x8 a[AB][NS];
 #pragma unroll
-  for (int s = 0; s < NS; ++s)
+  for (int ab = 0; ab < AB; ++ab) {
+    const int64_t p = base + 16 * ab + c16;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int dim = 32 * s + 8 * g + j;
-      const bool okd = s < nst && dim < d;
-      a0[s][j] = (__bf16)((okd && p0 < n) ? X[p0 * ld + dim] : 0.f);
-      a1[s][j] = (__bf16)((okd && p1 < n) ? X[p1 * ld + dim] : 0.f);
-    }
-  float b0[4], b1[4];
-  int i0[4], i1[4];
+    for (int s = 0; s < NS; ++s)
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    b0[i] = b1[i] = INFINITY;
-    i0[i] = i1[i] = 0;
+      for (int j = 0; j < 8; ++j) {
+        const int dim = 32 * s + 8 * g + j;
+        a[ab][s][j] = (__bf16)((s < nst && dim < d && p < n) ? X[p * ld + dim] : 0.f);
+      }
   }
-  for (int r0 = 0; r0 < m; r0 += 16) {
+  float best[AB][4];
+  int bidx[AB][4];
+#pragma unroll
+  for (int ab = 0; ab < AB; ++ab)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      best[ab][i] = INFINITY;
+      bidx[ab][i] = 0;
+    }
+  auto load_b = [&](int r0, as_bf16x8 (&bv)[NS], float &nr) {
     const int rep = r0 + c16;
     const bool ok = rep < m;
     const __bf16 *rr = rb + (int64_t)(ok ? rep : 0) * dk + 8 * g;
-    as_f32x4 c0 = {0.f, 0.f, 0.f, 0.f}, c1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int s = 0; s < NS; ++s)
-      if (s < nst) {
-        const as_bf16x8 bv = *(const as_bf16x8 *)(rr + 32 * s);
-        c0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[s], bv, c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[s], bv, c1, 0, 0, 0);
+      if (s < nst) bv[s] = *(const as_bf16x8 *)(rr + 32 * s);
+    nr = ok ? rn[rep] : INFINITY;
+  };
+  as_bf16x8 bc[NS], bn[NS];
+  float nrc, nrn = INFINITY;
+  load_b(0, bc, nrc);
+  for (int r0 = 0; r0 < m; r0 += 16) {
+    if (r0 + 16 < m) load_b(r0 + 16, bn, nrn);
+    const int rep = r0 + c16;
+#pragma unroll
+    for (int ab = 0; ab < AB; ++ab) {
+      as_f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        if (s < nst) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ab][s], bc[s], c, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float v = nrc - 2.f * c[i];
+        if (v < best[ab][i]) {
+          best[ab][i] = v;
+          bidx[ab][i] = rep;
+        }
       }
-    const float nr = ok ? rn[rep] : INFINITY;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) bc[s] = bn[s];
+    nrc = nrn;
+  }
+#pragma unroll
+  for (int ab = 0; ab < AB; ++ab)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const float s0 = nr - 2.f * c0[i], s1 = nr - 2.f * c1[i];
-      if (s0 < b0[i]) {
-        b0[i] = s0;
-        i0[i] = rep;
-      }
-      if (s1 < b1[i]) {
-        b1[i] = s1;
-        i1[i] = rep;
-      }
-    }
-  }
+      float b = best[ab][i];
+      int bi = bidx[ab][i];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-#pragma unroll
-    for (int msk = 1; msk < 16; msk <<= 1) {
-      float ov = __shfl_xor(b0[i], msk);
-      int oi = __shfl_xor(i0[i], msk);
-      if (ov < b0[i] || (ov == b0[i] && oi < i0[i])) {
-        b0[i] = ov;
-        i0[i] = oi;
+      for (int msk = 1; msk < 16; msk <<= 1) {
+        const float ov = __shfl_xor(b, msk);
+        const int oi = __shfl_xor(bi, msk);
+        if (ov < b || (ov == b && oi < bi)) {
+          b = ov;
+          bi = oi;
+        }
       }
-      ov = __shfl_xor(b1[i], msk);
-      oi = __shfl_xor(i1[i], msk);
-      if (ov < b1[i] || (ov == b1[i] && oi < i1[i])) {
-        b1[i] = ov;
-        i1[i] = oi;
-      }
+      const int64_t row = base + 16 * ab + 4 * g + i;
+      if (c16 == 0 && row < n) cl[row] = bi;
     }
-    if (c16 == 0) {
-      const int64_t r0w = base + 4 * g + i, r1w = base + 16 + 4 * g + i;
-      if (r0w < n) cl[r0w] = i0[i];
-      if (r1w < n) cl[r1w] = i1[i];
-    }
-  }
 }
 
 // bf16 copy of the representatives (row width dk) and their FP32 squared norms.
@@ -376,7 +385,7 @@ void launch_assign(const float *X, int64_t n, int64_t ld, int d, int dpad, const
   const int dk = (d + 31) & ~31;
   if (d >= 16 && dk <= 256 && rb && rn) {
     hipLaunchKernelGGL(k_reps_bf16, dim3(m), dim3(64), 0, s, reps, m, d, dpad, dk, rb, rn);
-    const dim3 g((unsigned)((n + 127) / 128));
+    const dim3 g((unsigned)((n + 255) / 256));  // 64 rows per wave, 4 waves
     if (dk <= 128)
       hipLaunchKernelGGL(k_assign_mfma<128>, g, dim3(256), 0, s, X, n, ld, d, rb, rn, m, dk, cl);
     else
