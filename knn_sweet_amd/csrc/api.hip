@@ -656,9 +656,15 @@ void fill_args(Inst &I) {
   a.dk = I.dk;
   // per-entry error codes in the MFMA filter keys: rows must fit 26 bits;
   // codes span 63 binades below the largest possible squared distance
-  a.ecode = (I.use_mfma && !I.exact && I.nR < (int64_t)(1 << 26)) ? 1 : 0;
+  // the finest code that leaves the rows their field: 2^lgs steps per octave
+  int lgs = -1;
+  for (int t = 3; t >= 0 && lgs < 0; --t)
+    if (I.nR <= ((int64_t)1 << (26 - t))) lgs = t;
+  a.ecode = (I.use_mfma && !I.exact && lgs >= 0) ? 1 : 0;
   const double smax = std::max(1e-30, (double)d * I.span * I.span);
-  a.ebase = std::ilogb(smax) + 1 - 62;
+  a.ec.lgs = std::max(lgs, 0);
+  a.ec.rowbits = 26 - a.ec.lgs;
+  a.ec.ebase = std::ilogb(smax) + 1 - 62;
   const double u = 0x1p-24, ub = 0x1p-8;
   auto gam = [&](double n) { return n * u / (1.0 - n * u); };
   // |computed dot - (q-c).(r-c)| <= cdot ||q-c|| ||r-c||

@@ -73,11 +73,11 @@ __device__ __forceinline__ void emit_query(const JoinArgs &a, const u64 *L, int 
 #pragma unroll
   for (int u = 0; u < E; ++u) {
     const u64 key = L[lane + 64 * u];
-    const bool ok = key != KEY_MAX && (a.ecode ? key_lower(key, a.ebase) <= thr_l : key_dist(key) <= cutoff);
+    const bool ok = key != KEY_MAX && (a.ecode ? key_lower(key, a.ec) <= thr_l : key_dist(key) <= cutoff);
     hs[u] = KEY_MAX;
     rs[u] = 0xFFFFFFFFu;
     if (ok) {
-      const int p = (int)(a.ecode ? (key_row(key) & KEY_ROW_MASK) : key_row(key));
+      const int p = (int)(a.ecode ? key_row_of(key, a.ec) : key_row(key));
       const double s64 = sq64_pair(qrow, a.R.px + (int64_t)p * dpR, d);
       hs[u] = (u64)__double_as_longlong(s64);
       rs[u] = (u32)a.R.perm[p];

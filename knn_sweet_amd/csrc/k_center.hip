@@ -12,15 +12,21 @@
 #include "knnj_common.h"
 #include "knnj_kernels.h"
 
-__global__ __launch_bounds__(256) void k_center(SideIndex Ri, int d, int dk, int bf16, void *rc, float *nr,
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 kc_bf16x8;
+
+// Workgroup per cluster.  Wave 0 first writes the representative (padded to
+// dk) and its squared norm.  Rows: a group of G lanes (G = the power of two
+// >= dk/8) takes one row, lane c dimensions 8c .. 8c+7 with 16-byte loads
+// and stores; 64/G rows per wave step.
+template <int G>
+__global__ __launch_bounds__(256) void k_center(SideIndex Ri, int d, int dk, int planes, void *rc, float *nr,
                                                 float *reps32, float *rn32) {
-  // bf16: 0 fp32 rows, 1 one bf16 plane, 2 hi/lo bf16 planes
+  // planes: 0 fp32 rows, 1 one bf16 plane, 2 hi/lo bf16 planes
   const int b = blockIdx.x;
   const int s0 = Ri.off[b], s1 = Ri.off[b + 1];
   const float *rep = Ri.reps + (int64_t)b * Ri.dpad;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (wave == 0) {
-    // the representative itself, padded to dk, and its squared norm
     float part = 0.f;
     for (int t = lane; t < dk; t += 64) {
       const float v = t < d ? rep[t] : 0.f;
@@ -30,28 +36,75 @@ __global__ __launch_bounds__(256) void k_center(SideIndex Ri, int d, int dk, int
     part = wave_sum(part);
     if (lane == 0) rn32[b] = part;
   }
-  for (int p = s0 + wave; p < s1; p += 4) {
-    const float *x = Ri.px + (int64_t)p * Ri.dpad;
+  constexpr int RPW = 64 / G;
+  const int sub = lane / G, c = lane % G;
+  const bool act = 8 * c < dk;
+  float rv[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) rv[u] = (8 * c + u < d) ? rep[8 * c + u] : 0.f;
+  for (int p0 = s0 + wave * RPW; p0 < s1; p0 += 4 * RPW) {
+    const int p = p0 + sub;
+    const bool ok = p < s1 && act;
+    float v[8];
     float part = 0.f;
-    for (int t = lane; t < dk; t += 64) {
-      const float v = t < d ? x[t] - rep[t] : 0.f;
-      part = __builtin_fmaf(v, v, part);
-      if (bf16 == 2) {
-        const __bf16 h = (__bf16)v;
-        ((__bf16 *)rc)[(int64_t)p * 2 * dk + t] = h;
-        ((__bf16 *)rc)[(int64_t)p * 2 * dk + dk + t] = (__bf16)(v - (float)h);
-      } else if (bf16 == 1) {
-        ((__bf16 *)rc)[(int64_t)p * dk + t] = (__bf16)v;
+    if (ok) {
+      const float *x = Ri.px + (int64_t)p * Ri.dpad + 8 * c;
+      float xv[8];
+      if (8 * c < Ri.dpad) {
+        const float4 x0 = *(const float4 *)x;
+        xv[0] = x0.x; xv[1] = x0.y; xv[2] = x0.z; xv[3] = x0.w;
       } else {
-        ((float *)rc)[(int64_t)p * dk + t] = v;
+        xv[0] = xv[1] = xv[2] = xv[3] = 0.f;
+      }
+      if (8 * c + 4 < Ri.dpad) {
+        const float4 x1 = *(const float4 *)(x + 4);
+        xv[4] = x1.x; xv[5] = x1.y; xv[6] = x1.z; xv[7] = x1.w;
+      } else {
+        xv[4] = xv[5] = xv[6] = xv[7] = 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        v[u] = (8 * c + u < d) ? xv[u] - rv[u] : 0.f;
+        part = __builtin_fmaf(v[u], v[u], part);
       }
     }
-    part = wave_sum(part);
-    if (lane == 0) nr[p] = part;
+#pragma unroll
+    for (int msk = 1; msk < G; msk <<= 1) part += __shfl_xor(part, msk);
+    if (!ok) continue;
+    if (planes == 2) {
+      kc_bf16x8 h, l;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        h[u] = (__bf16)v[u];
+        l[u] = (__bf16)(v[u] - (float)h[u]);
+      }
+      *(kc_bf16x8 *)((__bf16 *)rc + (int64_t)p * 2 * dk + 8 * c) = h;
+      *(kc_bf16x8 *)((__bf16 *)rc + (int64_t)p * 2 * dk + dk + 8 * c) = l;
+    } else if (planes == 1) {
+      kc_bf16x8 h;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) h[u] = (__bf16)v[u];
+      *(kc_bf16x8 *)((__bf16 *)rc + (int64_t)p * dk + 8 * c) = h;
+    } else {
+      float *o = (float *)rc + (int64_t)p * dk + 8 * c;
+      *(float4 *)o = make_float4(v[0], v[1], v[2], v[3]);
+      *(float4 *)(o + 4) = make_float4(v[4], v[5], v[6], v[7]);
+    }
+    if (c == 0) nr[p] = part;
   }
 }
 
 void launch_center(const SideIndex &Ri, int d, int dk, int planes, void *rc, float *nr, float *reps32, float *rn32,
                    hipStream_t s) {
-  hipLaunchKernelGGL(k_center, dim3(Ri.m), dim3(256), 0, s, Ri, d, dk, planes, rc, nr, reps32, rn32);
+  int G = 1;
+  while (G < dk / 8) G <<= 1;
+#define KJ_CENTER(GG) \
+  hipLaunchKernelGGL(k_center<GG>, dim3(Ri.m), dim3(256), 0, s, Ri, d, dk, planes, rc, nr, reps32, rn32)
+  switch (G) {
+    case 4: KJ_CENTER(4); break;
+    case 8: KJ_CENTER(8); break;
+    case 16: KJ_CENTER(16); break;
+    default: KJ_CENTER(32); break;
+  }
+#undef KJ_CENTER
 }

@@ -11,6 +11,8 @@
 // counting sort, the per-cluster sort runs on the device, and the rows are
 // physically permuted into cluster order so the join streams contiguous
 // memory.
+#include <algorithm>
+
 #include "knnj_common.h"
 #include "knnj_kernels.h"
 #include "wave_sort.h"
@@ -44,8 +46,56 @@ __global__ void k_scan_input(const float *__restrict__ X, int64_t n, int d, int6
   }
 }
 
+// Packed, 16-byte aligned rows: one flat float4 stream.
+__global__ void k_scan_input4(const float4 *__restrict__ X4, int64_t n4, u32 *flags) {
+  u32 bad = 0, nonint = 0;
+  float mx = 0.f;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n4; e += (int64_t)gridDim.x * blockDim.x) {
+    const float4 v = X4[e];
+    const float x[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!isfinite(x[j])) {
+        bad = 1;
+      } else {
+        mx = fmaxf(mx, fabsf(x[j]));
+        if (x[j] != rintf(x[j])) nonint = 1;
+      }
+    }
+  }
+  // one set of atomics per workgroup (thousands on one address serialise)
+  __shared__ u32 sh[3][4];
+  bad = __ballot(bad) != 0;
+  nonint = __ballot(nonint) != 0;
+  mx = wave_max(mx);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][w] = bad;
+    sh[1][w] = __float_as_uint(mx);
+    sh[2][w] = nonint;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 b = 0, x = 0, ni = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+      b |= sh[0][i];
+      x = max(x, sh[1][i]);
+      ni |= sh[2][i];
+    }
+    if (b) atomicOr(&flags[0], 1u);
+    atomicMax(&flags[1], x);
+    if (ni) atomicOr(&flags[2], 1u);
+  }
+}
+
 void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags, hipStream_t s) {
   const int64_t total = n * (int64_t)d;
+  if (ld == d && total % 4 == 0 && ((uintptr_t)X & 15) == 0 && total > 0) {
+    const int64_t n4 = total / 4;
+    int64_t blocks = std::min<int64_t>((n4 + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_scan_input4, dim3((unsigned)blocks), dim3(256), 0, s, (const float4 *)X, n4, flags);
+    return;
+  }
   int64_t blocks = (total + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   if (blocks < 1) blocks = 1;
@@ -367,15 +417,46 @@ __global__ void k_reps_bf16(const float *__restrict__ reps, int m, int d, int dp
 __global__ __launch_bounds__(256) void k_centroid(const float *__restrict__ X, int64_t ld, int d, int dpad,
                                                   const int32_t *__restrict__ off,
                                                   const int32_t *__restrict__ perm, int integral, float *reps) {
+  // wave w sums members w, w+4, ... for 256 dimensions at a time (lane t
+  // holds t, t+64, t+128, t+192); the four partial sums are added in wave
+  // order, so the result does not depend on scheduling
+  __shared__ double part[4][256];
   const int b = blockIdx.x;
   const int s0 = off[b], s1 = off[b + 1];
   if (s1 <= s0) return;
-  for (int t = threadIdx.x; t < d; t += blockDim.x) {
-    double acc = 0.0;
-    for (int p = s0; p < s1; ++p) acc += (double)X[(int64_t)perm[p] * ld + t];
-    float v = (float)(acc / (double)(s1 - s0));
-    if (integral) v = rintf(v);
-    reps[(int64_t)b * dpad + t] = v;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int t0 = 0; t0 < d; t0 += 256) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    // members p, p+4, p+8, p+12 per step: independent loads in flight,
+    // still summed in ascending member order
+    for (int p = s0 + w; p < s1; p += 16) {
+      int rows[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) rows[q] = p + 4 * q < s1 ? perm[p + 4 * q] : -1;
+      float v[4][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          v[q][u] = (rows[q] >= 0 && t0 + lane + 64 * u < d) ? X[(int64_t)rows[q] * ld + t0 + lane + 64 * u] : 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (rows[q] >= 0) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc[u] += (double)v[q][u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) part[w][lane + 64 * u] = acc[u];
+    __syncthreads();
+    const int t = t0 + threadIdx.x;
+    if (t < d) {
+      const double sum = ((part[0][threadIdx.x] + part[1][threadIdx.x]) + part[2][threadIdx.x]) + part[3][threadIdx.x];
+      float v = (float)(sum / (double)(s1 - s0));
+      if (integral) v = rintf(v);
+      reps[(int64_t)b * dpad + t] = v;
+    }
+    __syncthreads();
   }
 }
 
@@ -401,9 +482,63 @@ void launch_assign(const float *X, int64_t n, int64_t ld, int d, int dpad, const
     hipLaunchKernelGGL(k_assign<32>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, m, cl);
 }
 
+// The same for 16-byte aligned rows with ld % 4 == 0: a group of L = dpad/4
+// lanes (a power of two, <= 64) reads one row with float4 loads and reduces
+// its binary64 partial sums across the group; 64/L rows per wave step.
+template <int L>
+__global__ __launch_bounds__(256) void k_assign_finish4(const float *__restrict__ X, int64_t n, int64_t ld, int d,
+                                                        int dpad, const float *__restrict__ reps,
+                                                        const int32_t *__restrict__ cl, float *d2r, u64 *radbits,
+                                                        int32_t *cnt) {
+  constexpr int RPW = 64 / L;  // rows per wave step
+  const int lane = threadIdx.x & 63, sub = lane / L, j = lane % L;
+  const int64_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t i = wave * RPW + sub;
+  const bool ok = i < n;
+  const int b = ok ? cl[i] : 0;
+  double acc = 0.0;
+  for (int c = j; 4 * c < d; c += L) {
+    const float4 x = ok ? *(const float4 *)(X + i * ld + 4 * c) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 r = *(const float4 *)(reps + (int64_t)b * dpad + 4 * c);
+    const float xa[4] = {x.x, x.y, x.z, x.w}, ra[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (4 * c + u < d) {
+        const double t = (double)xa[u] - (double)ra[u];
+        acc += t * t;
+      }
+  }
+#pragma unroll
+  for (int msk = 1; msk < L; msk <<= 1) acc += __shfl_xor(acc, msk);
+  if (ok && j == 0) {
+    const double dd = sqrt(acc);
+    d2r[i] = (float)dd;
+    atomicMax(&radbits[b], (u64)__double_as_longlong(dd));
+    atomicAdd(&cnt[b], 1);
+  }
+}
+
 void launch_assign_finish(const float *X, int64_t n, int64_t ld, int d, int dpad, const float *reps,
                           const int32_t *cl, float *d2r, u64 *radbits, int32_t *cnt, hipStream_t s) {
   if (n <= 0) return;
+  const bool vec = ld % 4 == 0 && ((uintptr_t)X & 15) == 0 && dpad <= 256;
+  if (vec) {
+    int L = 1;
+    while (L < dpad / 4) L <<= 1;
+    const int64_t waves = (n + (64 / L) - 1) / (64 / L);
+    const dim3 g((unsigned)((waves + 3) / 4));
+#define KJ_FIN(LL) hipLaunchKernelGGL(k_assign_finish4<LL>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, cl, d2r, radbits, cnt)
+    switch (L) {
+      case 1: KJ_FIN(1); return;
+      case 2: KJ_FIN(2); return;
+      case 4: KJ_FIN(4); return;
+      case 8: KJ_FIN(8); return;
+      case 16: KJ_FIN(16); return;
+      case 32: KJ_FIN(32); return;
+      case 64: KJ_FIN(64); return;
+    }
+#undef KJ_FIN
+  }
   hipLaunchKernelGGL(k_assign_finish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, X, n, ld, d, dpad,
                      reps, cl, d2r, radbits, cnt);
 }

@@ -59,13 +59,13 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
 // lower bound of an evicted filter key (per-entry error code)
 struct DropLower {
   static constexpr bool tracks = true;
-  int eb;
-  __device__ float operator()(u64 key) const { return key_lower(key, eb); }
+  ECode ec;
+  __device__ float operator()(u64 key) const { return key_lower(key, ec); }
 };
 template <class D>
-__device__ __forceinline__ D make_drop(int eb) {
+__device__ __forceinline__ D make_drop(const ECode &ec) {
   if constexpr (D::tracks)
-    return D{eb};
+    return D{ec};
   else
     return D{};
 }
@@ -182,7 +182,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     evl[i] = INFINITY;
     live[i] = (4 * g + i) < nq;
   }
-  const int ebase = a.ebase;
+  const ECode ec = a.ec;
   typedef typename std::conditional<EXACT, NoDrop, DropLower>::type DropL;
   // this lane's slice of query row c16: dimensions 32 s + 8 g + j, in FP32
   const bool alive = c16 < nq;
@@ -246,7 +246,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     for (int j = 1; j < 4; ++j)
       if (j == ii) mine = cnt[j];
     const int cr = __shfl(mine, 16 * gi);
-    const float dropped = wave_merge<KCAP, MF_BC>(lists + r * KCAP, bufs + r * MF_BC, cr, tmp, make_drop<DropL>(ebase));
+    const float dropped = wave_merge<KCAP, MF_BC>(lists + r * KCAP, bufs + r * MF_BC, cr, tmp, make_drop<DropL>(ec));
     ++nmerge;
     absorb(r, dropped);
   };
@@ -387,7 +387,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
             const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
             use[i] = use[i] && !(dqb * (1.f - 2.f * eps) - Rb > thi);
             Ei[i] = EXACT ? 0.f : a.c1 * dqb * (1.f + eps) * Rb + a.c2 * (nqr[i] + Rb * Rb);
-            ecd[i] = (!EXACT && a.ecode) ? (ecode_encode(Ei[i], ebase) << KEY_ROW_BITS) : 0u;
+            ecd[i] = (!EXACT && a.ecode) ? (ecode_encode(Ei[i], ec) << ec.rowbits) : 0u;
             if (use[i]) {
               lo = fminf(lo, dqb * (1.f - 2.f * eps) - thi);
               hi = fmaxf(hi, dqb * (1.f + 2.f * eps) + thi);
@@ -456,7 +456,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                   Ei[i] = a.c1 * dqbv[i] * (1.f + eps) * Rg + a.c2 * (nqr[i] + Rg * Rg);
-                  ecd[i] = a.ecode ? (ecode_encode(Ei[i], ebase) << KEY_ROW_BITS) : 0u;
+                  ecd[i] = a.ecode ? (ecode_encode(Ei[i], ec) << ec.rowbits) : 0u;
                 }
               }
               // compute stage
@@ -671,7 +671,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
         u64 *lw = smem + w * mfma_wave_words(KCAP, DK);
         for (int r = 0; r < nq; ++r) {
           const float dropped =
-              wave_merge<KCAP, KCAP>(lists + r * KCAP, lw + r * KCAP, KCAP, tmp, make_drop<DropL>(ebase));
+              wave_merge<KCAP, KCAP>(lists + r * KCAP, lw + r * KCAP, KCAP, tmp, make_drop<DropL>(ec));
           absorb(r, dropped);
         }
 #pragma unroll

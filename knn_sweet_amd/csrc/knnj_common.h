@@ -22,22 +22,40 @@ __device__ __forceinline__ u64 make_key(float s, u32 row) {
 __device__ __forceinline__ float key_dist(u64 k) { return __uint_as_float((u32)(k >> 32)); }
 __device__ __forceinline__ u32 key_row(u64 k) { return (u32)k; }
 
-// Filter keys of the MFMA kernel carry, above a 26-bit row field, a 6-bit
-// code c of the entry's error bound: E <= 2^(c + ebase), c = 63 unbounded.
-// A list entry's lower bound is then L = key distance - 2 * decode(c).
-#define KEY_ROW_BITS 26
-#define KEY_ROW_MASK ((1u << KEY_ROW_BITS) - 1u)
-__device__ __forceinline__ u32 ecode_encode(float E, int ebase) {
+// Filter keys of the MFMA kernel carry, above a row field of `rowbits`
+// bits, a code c of the entry's error bound E with 2^lgs steps per octave:
+// decode(c) = 2^(c / 2^lgs + ebase) >= E, the top code meaning unbounded.
+// (rowbits = 26 - lgs: a 6 + lgs bit code covers 64 octaves.)  A list
+// entry's lower bound is key distance - 2 decode(c).
+struct ECode {
+  int ebase;    // codes start at 2^ebase
+  int lgs;      // log2 of the steps per octave, 0..3
+  int rowbits;  // width of the row field
+};
+__device__ __forceinline__ u32 ecode_top(const ECode &ec) { return (64u << ec.lgs) - 1u; }
+__device__ __forceinline__ u32 key_row_of(u64 key, const ECode &ec) {
+  return (u32)key & ((1u << ec.rowbits) - 1u);
+}
+__device__ __forceinline__ u32 ecode_encode(float E, const ECode &ec) {
   if (!(E > 0.f)) return 0u;
-  int e = ilogbf(E) + 1 - ebase;  // 2^(e + ebase) > E
-  if (e < 0) e = 0;
-  return e > 62 ? 63u : (u32)e;
+  const int e = ilogbf(E);
+  if (e < ec.ebase) return 0u;  // E < 2^ebase = decode(0)
+  const int S = 1 << ec.lgs;
+  // j >= S log2(m), m = E / 2^e in [1, 2): the 1e-3 margin exceeds the
+  // error of the hardware log2 by orders of magnitude
+  const float t = (float)S * __log2f(ldexpf(E, -e));
+  const int j = min(S, (int)ceilf(t + 1e-3f));
+  const u32 c = ((u32)(e - ec.ebase) << ec.lgs) + (u32)j;
+  return c >= ecode_top(ec) ? ecode_top(ec) : c;
 }
-__device__ __forceinline__ float ecode_decode(u32 c, int ebase) {
-  return c >= 63u ? INFINITY : ldexpf(1.f, (int)c + ebase);
+__device__ __forceinline__ float ecode_decode(u32 c, const ECode &ec) {
+  if (c >= ecode_top(ec)) return INFINITY;
+  const int j = (int)(c & ((1u << ec.lgs) - 1u));
+  const float up = exp2f((float)j / (float)(1 << ec.lgs)) * (1.f + 0x1p-20f);
+  return ldexpf(up, (int)(c >> ec.lgs) + ec.ebase);
 }
-__device__ __forceinline__ float key_lower(u64 key, int ebase) {
-  return key_dist(key) - 2.f * ecode_decode((u32)key >> KEY_ROW_BITS, ebase);
+__device__ __forceinline__ float key_lower(u64 key, const ECode &ec) {
+  return key_dist(key) - 2.f * ecode_decode((u32)key >> ec.rowbits, ec);
 }
 
 // Orders this wave's LDS accesses across its lanes: a wave's LDS

@@ -69,7 +69,7 @@ struct JoinArgs {
   u64 *tile_prof;
   float *ovf_diag;       // optional: first 64 overflow records {cutoff, k-th key, last key, first key}
   int ecode;             // MFMA filter keys carry per-entry error codes (rows < 2^26)
-  int ebase;             // exponent base of those codes
+  ECode ec;              // their layout: base, steps per octave, row field width
   int waves;             // MFMA join: > 1 asks for the multi-wave workgroup (tiles of 16 x waves queries)
   int lockstep;          // multi-wave workgroup: 1 = waves walk the union of their cluster lists together
   int split;             // multi-wave workgroup: > 1 = its waves split one 16-query tile's cluster list
