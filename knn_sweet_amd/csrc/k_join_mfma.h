@@ -74,9 +74,9 @@ __device__ __forceinline__ D make_drop(const ECode &ec) {
 #define MF_BC 64  // per-query candidate buffer
 
 // LDS of one wave (u64 words): lists, buffers, merge scratch, then floats:
-// row norms [16], screening distances [16][16], query rows [16][DK]
+// row norms [16], screening distances [16][16], query rows [16][DK + 4]
 __host__ __device__ constexpr size_t mfma_wave_words(int kcap, int dk) {
-  return (size_t)MF_TQ * kcap + (size_t)MF_TQ * MF_BC + (size_t)kcap + (16 + 256 + (size_t)MF_TQ * dk) / 2;
+  return (size_t)MF_TQ * kcap + (size_t)MF_TQ * MF_BC + (size_t)kcap + (16 + 256 + (size_t)MF_TQ * (dk + 4)) / 2;
 }
 // waves per workgroup: the most of 4, 2, 1 whose LDS fits the 160 KiB of a
 // gfx950 workgroup (plus the union slots)
@@ -126,7 +126,9 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   u64 *tmp = bufs + MF_TQ * MF_BC;          // [KCAP]
   float *fsh = (float *)(tmp + KCAP);       // [16] row norms
   float *dqs = fsh + 16;                    // [16 rows][16 cols] screening distances
-  float *qsh = dqs + 256;                   // [16 rows][DK] the tile's query rows
+  // [16 rows][DK + 4] the tile's query rows; the 4-float pad spreads the 16
+  // rows a 16-byte read touches over different LDS banks
+  float *qsh = dqs + 256;
   const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
   const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
   u64 nvisit = 0;
@@ -190,10 +192,10 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   // (re-read from LDS when needed rather than held in registers)
   for (int e = lane; e < MF_TQ * DK; e += 64) {
     const int r = e / DK, dim = e % DK;
-    qsh[e] = (r < nq && dim < d) ? a.Q.px[(int64_t)(q0 + r) * dpQ + dim] : 0.f;
+    qsh[r * (DK + 4) + dim] = (r < nq && dim < d) ? a.Q.px[(int64_t)(q0 + r) * dpQ + dim] : 0.f;
   }
   wave_sync();
-  const float *qrow_sh = qsh + c16 * DK + 8 * g;
+  const float *qrow_sh = qsh + c16 * (DK + 4) + 8 * g;
   auto q_slice = [&](int s, float (&v)[8]) {
     const f32x4_t v0 = *(const f32x4_t *)(qrow_sh + 32 * s);
     const f32x4_t v1 = *(const f32x4_t *)(qrow_sh + 32 * s + 4);
