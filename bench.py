@@ -69,20 +69,20 @@ def load_config(name: str, k: int | None):
 
 
 def cpu_baseline(R, Q, k, target_s: float = 12.0):
-    """FP64 brute-force oracle on a bounded query sample (calibrated to ~target_s)."""
+    """FP64 brute-force oracle on a bounded query sample: the sample grows
+    until one timed pass takes at least half of target_s (or covers Q)."""
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     threads = min(threads, 16)
+    oracle.bruteforce(R, Q[:min(8, Q.shape[0])], k, nthreads=threads)  # warm-up: threads, pages
     n = min(32, Q.shape[0])
-    t0 = time.perf_counter()
-    oracle.bruteforce(R, Q[:n], k, nthreads=threads)
-    dt = time.perf_counter() - t0
-    n2 = int(min(Q.shape[0], max(n, n * target_s / max(dt, 1e-3))))
-    if n2 > n:
+    while True:
         t0 = time.perf_counter()
-        oracle.bruteforce(R, Q[:n2], k, nthreads=threads)
+        oracle.bruteforce(R, Q[:n], k, nthreads=threads)
         dt = time.perf_counter() - t0
-        n = n2
+        if dt >= 0.5 * target_s or n >= Q.shape[0]:
+            break
+        n = int(min(Q.shape[0], max(2 * n, n * target_s / max(dt, 1e-3))))
     return {"value": n / dt, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"first {n} queries of the same workload, FP64 brute force (oracle), {dt:.1f} s"}
 
