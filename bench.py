@@ -155,6 +155,7 @@ def bench_single(args, kj, cfg):
     ctx = kj.Context(0)
     opts = dict(brute_force=args.brute_force, tile_waves=args.tile_waves, xcd_tiles=args.xcd_tiles,
                 tile_split=args.tile_split, split_rounds=args.split_rounds,
+                refine_iters=args.refine_iters, tile_q=args.tile_q,
                 kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel],
                 precision={"auto": kj.PREC_AUTO, "fp32": kj.PREC_FP32, "bf16": kj.PREC_BF16,
                            "bf16x1": kj.PREC_BF16X1}[args.precision])
@@ -341,6 +342,8 @@ def main():
     ap.add_argument("--xcd-tiles", action="store_true", help="place all tiles of a query cluster on one XCD")
     ap.add_argument("--tile-split", type=int, default=0, help="MFMA join split mode (0 auto, 1 off, 4 on)")
     ap.add_argument("--split-rounds", type=int, default=0, help="representative split rounds (0 default, -1 none)")
+    ap.add_argument("--refine-iters", type=int, default=0, help="Lloyd steps (0 default, -1 none)")
+    ap.add_argument("--tile-q", type=int, default=0, help="VALU join: queries per wave tile (0 auto)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter scale for rocprofv3 --pmc)")
     args = ap.parse_args()

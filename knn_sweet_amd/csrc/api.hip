@@ -685,55 +685,57 @@ hipError_t launch_main(const Inst &I, int64_t grid, hipStream_t s) {
   return launch_join(I.args, I.tq, grid, s);
 }
 
-// Overflowed queries of one join, one per tile, at a larger capacity: the
-// MFMA kernel at 512 if it ran below that, else (and as the last resort)
-// the VALU kernel at 1024.
+// Overflowed queries of one join, one per tile, re-run at larger
+// capacities: on the MFMA kernel at twice the capacity (at most 512; where
+// that capacity's LDS still allows several waves, they split each query's
+// cluster list), then on the VALU kernel at 1024; KNNJ_EOVERFLOW only if
+// that overflows too.  Each stage's overflows feed the next.
 knnj_status rerun_overflow(Inst &I, int32_t novf, hipStream_t s) {
-  JoinArgs b = I.args;
-  b.tiles = I.ovf_tiles;
-  b.ntiles = I.counters + 1;
-  b.ovf_tiles = I.ovf2_tiles;
-  b.ovf_count = I.counters + 2;
-  b.tile_prof = nullptr;
-  b.ovf_diag = nullptr;
-  b.waves = 1;  // one-query tiles
-  b.split = 1;
-  const bool mfma_up = I.use_mfma && I.kcap < 512;
-  if (mfma_up) {
-    b.kcap = 512;
-    if (I.bf16)
-      HIPCHK(launch_join_mfma_bf16(b, I.bf16x1 ? 1 : 3, novf, s));
-    else
-      HIPCHK(launch_join_mfma_f32(b, novf, s));
-  } else {
-    b.kcap = KNNJ_MAX_K;
-    b.exact = I.exact_valu ? 1 : 0;
-    b.ecode = 0;
-    HIPCHK(launch_join(b, 1, novf, s));
-  }
-  int32_t h2 = 0;
-  HIPCHK(hipMemcpyAsync(&h2, I.counters + 2, sizeof(h2), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  if (h2 == 0) return KNNJ_OK;
-  if (!mfma_up) {
-    set_err("%d queries overflowed the largest candidate capacity (%d)", h2, KNNJ_MAX_K);
-    return KNNJ_EOVERFLOW;
-  }
-  JoinArgs c2 = b;
-  c2.tiles = I.ovf2_tiles;
-  c2.ntiles = I.counters + 2;
-  c2.ovf_tiles = I.ovf_tiles;
-  c2.ovf_count = I.counters + 4;
-  c2.kcap = KNNJ_MAX_K;
-  c2.exact = I.exact_valu ? 1 : 0;
-  c2.ecode = 0;
-  HIPCHK(launch_join(c2, 1, h2, s));
-  int32_t h3 = 0;
-  HIPCHK(hipMemcpyAsync(&h3, I.counters + 4, sizeof(h3), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  if (h3 > 0) {
-    set_err("%d queries overflowed the largest candidate capacity (%d)", h3, KNNJ_MAX_K);
-    return KNNJ_EOVERFLOW;
+  Tile *in = I.ovf_tiles, *out = I.ovf2_tiles;
+  int32_t *in_cnt = I.counters + 1, *out_cnt = I.counters + 2, *spare = I.counters + 4;
+  int kc = I.kcap;
+  bool mfma = I.use_mfma;
+  int32_t n = novf;
+  while (n > 0) {
+    JoinArgs b = I.args;
+    b.tiles = in;
+    b.ntiles = in_cnt;
+    b.ovf_tiles = out;
+    b.ovf_count = out_cnt;
+    b.tile_prof = nullptr;
+    b.ovf_diag = nullptr;
+    b.waves = 1;  // one-query tiles
+    b.split = 1;
+    HIPCHK(hipMemsetAsync(out_cnt, 0, sizeof(int32_t), s));
+    if (mfma && kc < 512) {
+      kc = std::min(512, 2 * kc);
+      b.kcap = kc;
+      const int sw = join_mfma_split_waves(kc, I.dk);
+      if (sw > 1) b.split = b.waves = sw;
+      if (I.bf16)
+        HIPCHK(launch_join_mfma_bf16(b, I.bf16x1 ? 1 : 3, n, s));
+      else
+        HIPCHK(launch_join_mfma_f32(b, n, s));
+    } else if (kc < KNNJ_MAX_K) {
+      mfma = false;
+      kc = KNNJ_MAX_K;
+      b.kcap = kc;
+      b.exact = I.exact_valu ? 1 : 0;
+      b.ecode = 0;
+      HIPCHK(launch_join(b, 1, n, s));
+    } else {
+      set_err("%d queries overflowed the largest candidate capacity (%d)", n, KNNJ_MAX_K);
+      return KNNJ_EOVERFLOW;
+    }
+    int32_t h = 0;
+    HIPCHK(hipMemcpyAsync(&h, out_cnt, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    n = h;
+    std::swap(in, out);
+    // the next stage reads this stage's count and writes into the third slot
+    int32_t *next_out = (in_cnt == I.counters + 1) ? spare : in_cnt;
+    in_cnt = out_cnt;
+    out_cnt = next_out;
   }
   return KNNJ_OK;
 }
