@@ -155,7 +155,7 @@ def bench_single(args, kj, cfg):
     ctx = kj.Context(0)
     opts = dict(brute_force=args.brute_force, tile_waves=args.tile_waves, xcd_tiles=args.xcd_tiles,
                 tile_split=args.tile_split, split_rounds=args.split_rounds,
-                refine_iters=args.refine_iters, tile_q=args.tile_q,
+                refine_iters=args.refine_iters, tile_q=args.tile_q, autotune=not args.no_autotune,
                 kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel],
                 precision={"auto": kj.PREC_AUTO, "fp32": kj.PREC_FP32, "bf16": kj.PREC_BF16,
                            "bf16x1": kj.PREC_BF16X1}[args.precision])
@@ -181,7 +181,7 @@ def bench_single(args, kj, cfg):
         "dist_pairs_per_s": evals / (t_join * 1e-3), "m_q": st["m_q"], "m_r": st["m_r"],
         "kcap": st["kcap"], "tile_q": st["tile_q"], "n_overflow": st["n_overflow"],
         "exact_mode": st["exact_mode"], "merges": st["n_merges"], "appended": st["n_appended"],
-        "filter_max_abs_err": st["filter_max_abs_err"],
+        "filter_max_abs_err": st["filter_max_abs_err"], "autotune_choice": st["tuned"],
     }
     if args.config == "c3" and args.precision == "auto" and not args.no_compare:
         # the plain FP32 join of the same inputs, timed the same way
@@ -344,6 +344,9 @@ def main():
     ap.add_argument("--split-rounds", type=int, default=0, help="representative split rounds (0 default, -1 none)")
     ap.add_argument("--refine-iters", type=int, default=0, help="Lloyd steps (0 default, -1 none)")
     ap.add_argument("--tile-q", type=int, default=0, help="VALU join: queries per wave tile (0 auto)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the fixed default rules instead of the library's autotuner (which picks "
+                         "representative count, Lloyd step and VALU tile on the first warm-up call)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (HBM counter scale for rocprofv3 --pmc)")
     args = ap.parse_args()

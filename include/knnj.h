@@ -99,6 +99,11 @@ typedef struct {
   int32_t split_rounds;/* representative selection: rounds that replace sampled representatives
                           by the farthest members of the widest clusters (0 = default: none
                           per join, 16 for knnj_index_build; negative = none) */
+  int32_t autotune;    /* knnj_join: 1 = on the first call of a problem shape (d, k, log2 of
+                          both sizes, self-join) try the representative count x Lloyd step x
+                          VALU tile candidates the other fields leave open, keep the fastest
+                          for this context and use it for later calls of that shape */
+  int32_t reserved[15];
 } knnj_opts;
 
 /* Device-side timings (hipEvents on the context's stream) and counters. */
@@ -125,7 +130,8 @@ typedef struct {
   int32_t n_launches;       /* join kernel launches (knnj_join_batch: fused launches) */
   int64_t n_merges;         /* k-select buffer merges */
   int64_t n_appended;       /* candidates that passed a query's threshold */
-  int32_t reserved[4];
+  int32_t tuned;            /* autotune: 0 off, else 1 + the candidate the context chose */
+  int32_t reserved[3];
 } knnj_stats;
 
 /* One independent join of a batch (knnj_join_batch). */

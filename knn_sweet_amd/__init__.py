@@ -48,7 +48,8 @@ class Opts(ctypes.Structure):
                 ("brute_force", ctypes.c_int32), ("tile_q", ctypes.c_int32),
                 ("kernel", ctypes.c_int32), ("refine_iters", ctypes.c_int32),
                 ("xcd_tiles", ctypes.c_int32), ("tile_waves", ctypes.c_int32),
-                ("tile_split", ctypes.c_int32), ("split_rounds", ctypes.c_int32)]
+                ("tile_split", ctypes.c_int32), ("split_rounds", ctypes.c_int32),
+                ("autotune", ctypes.c_int32), ("reserved", ctypes.c_int32 * 15)]
 
 
 class Stats(ctypes.Structure):
@@ -61,7 +62,7 @@ class Stats(ctypes.Structure):
                 ("tile_q", ctypes.c_int32), ("kcap", ctypes.c_int32), ("n_tiles", ctypes.c_int64),
                 ("filter_max_abs_err", ctypes.c_double), ("kernel_used", ctypes.c_int32),
                 ("n_launches", ctypes.c_int32), ("n_merges", ctypes.c_int64),
-                ("n_appended", ctypes.c_int64), ("reserved", ctypes.c_int32 * 4)]
+                ("n_appended", ctypes.c_int64), ("tuned", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_
@@ -186,9 +187,10 @@ def _outputs(nq: int, k: int, aos: bool, qdev):
 
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
               brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0, xcd_tiles=False,
-              tile_waves=0, tile_split=0, split_rounds=0) -> Opts:
+              tile_waves=0, tile_split=0, split_rounds=0, autotune=False) -> Opts:
     return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
-                tile_q, kernel, refine_iters, int(xcd_tiles), tile_waves, tile_split, split_rounds)
+                tile_q, kernel, refine_iters, int(xcd_tiles), tile_waves, tile_split, split_rounds,
+                int(autotune))
 
 
 class Context:

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -77,6 +78,8 @@ struct knnj_ctx {
   static constexpr int kAux = 4;
   hipStream_t aux[kAux] = {};
   hipEvent_t aux_ev[kAux + 1] = {};
+  // autotune: problem shape -> chosen candidate (index into the candidate list)
+  std::map<std::vector<int64_t>, int> tuned;
 };
 
 #define HIPCHK(expr)                                                                      \
@@ -949,16 +952,12 @@ knnj_status knnj_ctx_set_budget(knnj_ctx *c, int64_t bytes) {
   return KNNJ_OK;
 }
 
-knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, const knnj_opts *opts,
+}  // extern "C"
+
+namespace {
+
+knnj_status join_once(knnj_ctx *c, const knnj_matrix &R, const knnj_matrix &Q, int32_t k, const knnj_opts &o,
                       int32_t *out_idx, float *out_dist, knnj_stats *stats) {
-  g_err.clear();
-  if (!c) {
-    set_err("context is NULL");
-    return KNNJ_EINVAL;
-  }
-  knnj_opts o{};
-  if (opts) o = *opts;
-  TRY(check_join_args(R, Q, k, o, out_idx, out_dist));
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   hipEvent_t *ev = c->ev;
@@ -981,6 +980,88 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   HIPCHK(hipStreamSynchronize(s));
   TRY(decide(I, hflags));
   return run_single(c, I, A, stats);
+}
+
+// Autotune candidates for the fields the caller left open (0): Lloyd step
+// on/off, representatives 1x / 2x the default count, and for the VALU
+// kernel 2 or 4 queries per tile.  Every candidate computes the same exact
+// answer; only the time differs.
+std::vector<knnj_opts> tune_candidates(const knnj_opts &o, const knnj_matrix &R, const knnj_matrix &Q, bool self) {
+  std::vector<knnj_opts> v;
+  const int d = R.d;
+  const bool valu = o.kernel == KNNJ_KERNEL_VALU ||
+                    (o.kernel == KNNJ_KERNEL_AUTO && !(d >= 32 && d <= 256) && o.precision != KNNJ_PREC_BF16 &&
+                     o.precision != KNNJ_PREC_BF16X1);
+  const int refines[2] = {o.refine_iters ? o.refine_iters : 1, -1};
+  const int nref = o.refine_iters ? 1 : 2;
+  const int scales[2] = {1, 2};
+  const int nsc = (o.m_r || o.m_q || o.brute_force) ? 1 : 2;
+  const int tqs[2] = {o.tile_q ? o.tile_q : 2, 4};
+  const int ntq = (valu && !o.tile_q) ? 2 : 1;
+  for (int a = 0; a < nref; ++a)
+    for (int b = 0; b < nsc; ++b)
+      for (int t = 0; t < ntq; ++t) {
+        knnj_opts x = o;
+        x.autotune = 0;
+        x.refine_iters = refines[a];
+        if (scales[b] > 1) {
+          x.m_r = std::min(KNNJ_MAX_REPS, (int)std::min<int64_t>(R.n, 2 * default_reps(R.n)));
+          x.m_q = self ? x.m_r : std::min(KNNJ_MAX_REPS, (int)std::min<int64_t>(Q.n, 2 * default_reps(Q.n)));
+        }
+        if (valu) x.tile_q = tqs[t];
+        v.push_back(x);
+      }
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, const knnj_opts *opts,
+                      int32_t *out_idx, float *out_dist, knnj_stats *stats) {
+  g_err.clear();
+  if (!c) {
+    set_err("context is NULL");
+    return KNNJ_EINVAL;
+  }
+  knnj_opts o{};
+  if (opts) o = *opts;
+  TRY(check_join_args(R, Q, k, o, out_idx, out_dist));
+  if (!o.autotune || Q.n == 0) return join_once(c, R, Q, k, o, out_idx, out_dist, stats);
+  const int64_t ldR = R.ld ? R.ld : R.d, ldQ = Q.ld ? Q.ld : Q.d;
+  const bool self = R.data == Q.data && R.n == Q.n && ldR == ldQ && R.on_device == Q.on_device;
+  auto lg = [](int64_t n) { return (int64_t)std::ceil(std::log2((double)std::max<int64_t>(n, 1))); };
+  const std::vector<int64_t> sig = {R.d, k, lg(R.n), lg(Q.n), self, o.precision, o.kernel, o.m_r, o.m_q,
+                                    o.refine_iters, o.tile_q, o.brute_force};
+  const std::vector<knnj_opts> cand = tune_candidates(o, R, Q, self);
+  auto it = c->tuned.find(sig);
+  if (it != c->tuned.end()) {
+    const knnj_status e = join_once(c, R, Q, k, cand[it->second], out_idx, out_dist, stats);
+    if (stats) stats->tuned = 1 + it->second;
+    return e;
+  }
+  // first call of this shape: time every candidate (each writes the same
+  // exact answer), keep the fastest
+  int best = 0;
+  double best_ms = 0.0;
+  knnj_stats st{};
+  for (size_t i = 0; i < cand.size(); ++i) {
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    TRY(join_once(c, R, Q, k, cand[i], out_idx, out_dist, &st));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (i == 0 || ms < best_ms) {
+      best = (int)i;
+      best_ms = ms;
+      if (stats) *stats = st;
+    }
+  }
+  c->tuned[sig] = best;
+  if (stats) stats->tuned = 1 + best;
+  return KNNJ_OK;
 }
 
 void knnj_index_destroy(knnj_index *x) {

@@ -29,7 +29,7 @@ def test_struct_layouts_match_header():
     # field order and sizes of the ctypes mirrors (x86-64 ABI)
     assert ctypes.sizeof(kj.Matrix) == 40
     assert kj.Opts.seed.offset == 16
-    assert ctypes.sizeof(kj.Opts) == 16 + 8 + 4 * 4 + 6 * 4
+    assert ctypes.sizeof(kj.Opts) == 128
     assert kj.Stats.n_dist_evals.offset == 40
 
 
@@ -53,3 +53,37 @@ def test_null_context_is_rejected():
     rc = lib.knnj_join(None, m, m, 1, None, None, None, None)
     assert rc == kj.KNNJ_EINVAL
     assert b"context" in lib.knnj_last_error()
+
+
+def test_struct_layouts_match_c_compiler(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler
+    gives the header's structs."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    mirrors = {"knnj_matrix": kj.Matrix, "knnj_opts": kj.Opts, "knnj_stats": kj.Stats,
+               "knnj_instance": kj.Instance, "knnj_index_desc": kj.IndexDesc,
+               "knnj_server_config": kj.ServerConfig, "knnj_request_info": kj.RequestInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "knnj.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run([gcc, f"-I{inc}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            cname, fname, v = ln.split()
+            got[(cname, fname)] = int(v)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

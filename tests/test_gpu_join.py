@@ -216,3 +216,19 @@ def test_split_mode_exact_integer(ctx):
     idx, dist, st = ctx.join(R / 1, Q / 1, 100, kernel=kj.KERNEL_MFMA, tile_split=4)
     assert st["exact_mode"] == 1
     assert_matches_oracle(R, Q, 100, idx, dist)
+
+
+@pytest.mark.parametrize("d,k", [(3, 50), (64, 16)])
+def test_autotune_gives_the_same_answer(ctx, d, k):
+    """The autotuner times its candidates on the first call of a shape and
+    reuses the choice; every call returns the oracle's answer."""
+    mix = datagen.mixture64_model(seed=400 + d, d=d, comps=30)
+    R = mix.sample(datagen.rng(d + 401), 9000)
+    Q = mix.sample(datagen.rng(d + 402), 800)
+    tctx = kj.Context(0)
+    a_idx, a_dist, sa = tctx.join(R, Q, k, autotune=True)
+    b_idx, b_dist, sb = tctx.join(R, Q, k, autotune=True)
+    assert sa["tuned"] >= 1 and sb["tuned"] == sa["tuned"]
+    assert np.array_equal(a_idx, b_idx) and np.array_equal(a_dist, b_dist)
+    assert_matches_oracle(R, Q, k, a_idx, a_dist)
+    tctx.close()
