@@ -204,14 +204,16 @@ def bench_batch(args, kj, cfg):
     dev = [(torch.from_numpy(R).cuda(), torch.from_numpy(Q).cuda(), k) for R, Q in inst]
     torch.cuda.synchronize()
     ctx = kj.Context(0)
+    # the batch's default (no Lloyd step) for both the batched call and the loop
+    bopts = dict(refine_iters=args.refine_iters or -1, split_rounds=args.split_rounds)
     for _ in range(args.warmup):
-        ctx.join_batch(dev)
+        ctx.join_batch(dev, **bopts)
     torch.cuda.synchronize()
     times, stats = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
-        _, st = ctx.join_batch(dev)
+        _, st = ctx.join_batch(dev, **bopts)
         times.append((time.perf_counter() - s0) * 1e3)
         stats.append(st)
     torch.cuda.synchronize()
@@ -239,7 +241,7 @@ def bench_batch(args, kj, cfg):
     t0 = time.perf_counter()
     for _ in range(lsteps):
         for Rd, Qd, kk in dev:
-            ctx.join(Rd, Qd, kk)
+            ctx.join(Rd, Qd, kk, **bopts)
     torch.cuda.synchronize()
     lt = time.perf_counter() - t0
     out["per_instance_loop"] = {"value": nq * lsteps / lt, "ms_per_step": lt * 1e3 / lsteps, "steps": lsteps}

@@ -87,8 +87,8 @@ typedef struct {
   int32_t tile_q;      /* queries per wave tile of the VALU kernel, 0 = auto */
   int32_t kernel;      /* knnj_kernel: distance-tile implementation, 0 = auto */
   int32_t refine_iters;/* Lloyd steps moving each representative to its members' mean
-                          (0 = default: 1 per join, 2 for knnj_index_build; negative = keep
-                          the sampled points) */
+                          (0 = default: 1 per join, none per knnj_join_batch instance, 2 for
+                          knnj_index_build; negative = keep the sampled points) */
   int32_t xcd_tiles;   /* 1: place all tiles of a query cluster on one XCD (shared L2) */
   int32_t tile_waves;  /* MFMA kernel: waves per workgroup tile, 0 = auto, 1 = one 16-query
                           wave, > 1 = the most waves whose LDS fits (2 or 4), walking the
