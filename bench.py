@@ -206,14 +206,16 @@ def bench_batch(args, kj, cfg):
     ctx = kj.Context(0)
     # the batch's default (no Lloyd step) for both the batched call and the loop
     bopts = dict(refine_iters=args.refine_iters or -1, split_rounds=args.split_rounds)
-    for _ in range(args.warmup):
-        ctx.join_batch(dev, **bopts)
+    gopts = dict(bopts, graph=args.graph)
+    outs, _ = ctx.join_batch(dev, **gopts)  # warm-up; with --graph it captures the plan
+    for _ in range(max(0, args.warmup - 1)):
+        ctx.join_batch(dev, outputs=outs, **gopts)
     torch.cuda.synchronize()
     times, stats = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
-        _, st = ctx.join_batch(dev, **bopts)
+        _, st = ctx.join_batch(dev, outputs=outs, **gopts)
         times.append((time.perf_counter() - s0) * 1e3)
         stats.append(st)
     torch.cuda.synchronize()
@@ -232,7 +234,8 @@ def bench_batch(args, kj, cfg):
         "p50_ms": float(np.percentile(times, 50)), "p99_ms": float(np.percentile(times, 99)),
         "stages_ms": {"index_bounds_tiles": st["t_index_ms"], "join": t_join, "fallback": st["t_fallback_ms"],
                       "total": st["t_total_ms"]},
-        "fused_launches": st["n_launches"], "pruning_ratio": st["pruning_ratio"],
+        "fused_launches": st["n_launches"], "graph": {0: "eager", 1: "captured", 2: "replayed"}[st["graph"]],
+        "pruning_ratio": st["pruning_ratio"],
         "dist_evals": st["n_dist_evals"], "n_overflow": st["n_overflow"],
     }
     # the per-instance loop over the same device inputs
@@ -346,6 +349,8 @@ def main():
     ap.add_argument("--split-rounds", type=int, default=0, help="representative split rounds (0 default, -1 none)")
     ap.add_argument("--refine-iters", type=int, default=0, help="Lloyd steps (0 default, -1 none)")
     ap.add_argument("--tile-q", type=int, default=0, help="VALU join: queries per wave tile (0 auto)")
+    ap.add_argument("--graph", action="store_true",
+                    help="batched workload: capture the plan once and replay it as a HIP graph")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the fixed default rules instead of the library's autotuner (which picks "
                          "representative count, Lloyd step and VALU tile on the first warm-up call)")

@@ -103,7 +103,11 @@ typedef struct {
                           both sizes, self-join) try the representative count x Lloyd step x
                           VALU tile candidates the other fields leave open, keep the fastest
                           for this context and use it for later calls of that shape */
-  int32_t reserved[15];
+  int32_t graph;       /* knnj_join_batch: 1 = capture the index builds and fused joins as a
+                          HIP graph and replay it while the plan (every pointer, size and
+                          decision) is unchanged; the input scan and overflow re-runs stay
+                          outside it */
+  int32_t reserved[14];
 } knnj_opts;
 
 /* Device-side timings (hipEvents on the context's stream) and counters. */
@@ -131,7 +135,8 @@ typedef struct {
   int64_t n_merges;         /* k-select buffer merges */
   int64_t n_appended;       /* candidates that passed a query's threshold */
   int32_t tuned;            /* autotune: 0 off, else 1 + the candidate the context chose */
-  int32_t reserved[3];
+  int32_t graph;            /* knnj_join_batch: 0 eager, 1 captured this call, 2 replayed */
+  int32_t reserved[2];
 } knnj_stats;
 
 /* One independent join of a batch (knnj_join_batch). */

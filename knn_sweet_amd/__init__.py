@@ -49,7 +49,7 @@ class Opts(ctypes.Structure):
                 ("kernel", ctypes.c_int32), ("refine_iters", ctypes.c_int32),
                 ("xcd_tiles", ctypes.c_int32), ("tile_waves", ctypes.c_int32),
                 ("tile_split", ctypes.c_int32), ("split_rounds", ctypes.c_int32),
-                ("autotune", ctypes.c_int32), ("reserved", ctypes.c_int32 * 15)]
+                ("autotune", ctypes.c_int32), ("graph", ctypes.c_int32), ("reserved", ctypes.c_int32 * 14)]
 
 
 class Stats(ctypes.Structure):
@@ -62,7 +62,8 @@ class Stats(ctypes.Structure):
                 ("tile_q", ctypes.c_int32), ("kcap", ctypes.c_int32), ("n_tiles", ctypes.c_int64),
                 ("filter_max_abs_err", ctypes.c_double), ("kernel_used", ctypes.c_int32),
                 ("n_launches", ctypes.c_int32), ("n_merges", ctypes.c_int64),
-                ("n_appended", ctypes.c_int64), ("tuned", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
+                ("n_appended", ctypes.c_int64), ("tuned", ctypes.c_int32), ("graph", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_
@@ -187,10 +188,10 @@ def _outputs(nq: int, k: int, aos: bool, qdev):
 
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
               brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0, xcd_tiles=False,
-              tile_waves=0, tile_split=0, split_rounds=0, autotune=False) -> Opts:
+              tile_waves=0, tile_split=0, split_rounds=0, autotune=False, graph=False) -> Opts:
     return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
                 tile_q, kernel, refine_iters, int(xcd_tiles), tile_waves, tile_split, split_rounds,
-                int(autotune))
+                int(autotune), int(graph))
 
 
 class Context:
@@ -234,11 +235,14 @@ class Context:
         """A resident index of R on this context's device (see Index)."""
         return Index(self, R, **opts)
 
-    def join_batch(self, instances, **opts):
+    def join_batch(self, instances, outputs=None, **opts):
         """Many independent joins in one call.
 
         instances: sequence of (R, Q, k).  Returns ([(idx, dist), ...], stats-dict)
         with each result equal to ``join(R, Q, k)`` of that instance alone.
+        outputs: optional [(idx, dist), ...] from an earlier call to write into
+        (with ``graph=True`` a plan whose pointers do not change replays its
+        captured HIP graph).
         """
         o = make_opts(**opts)
         arr = (Instance * max(len(instances), 1))()
@@ -248,7 +252,12 @@ class Context:
             mQ, keepQ, qdev = _matrix(Q)
             if R is Q:
                 mQ = mR
-            idx, dist, pi, pd = _outputs(mQ.n, k, bool(o.out_aos), keepQ if qdev else None)
+            if outputs is not None:
+                idx, dist = outputs[i]
+                pi = idx.data_ptr() if _is_torch(idx) else idx.ctypes.data
+                pd = None if dist is None else (dist.data_ptr() if _is_torch(dist) else dist.ctypes.data)
+            else:
+                idx, dist, pi, pd = _outputs(mQ.n, k, bool(o.out_aos), keepQ if qdev else None)
             arr[i] = Instance(mR, mQ, k, pi, pd)
             keep += [keepR, keepQ]
             results.append((idx, dist))

@@ -56,13 +56,6 @@ struct DevBuf {
   size_t n = 0;
 };
 
-uint64_t splitmix64(uint64_t &s) {
-  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-  return z ^ (z >> 31);
-}
-
 }  // namespace
 
 struct knnj_ctx {
@@ -70,8 +63,6 @@ struct knnj_ctx {
   hipStream_t stream = nullptr;
   std::map<std::string, DevBuf> ws;
   int64_t budget = 0;
-  // representative draws of the current call, kept alive for the async H2D
-  std::vector<std::vector<int32_t>> host_cand;
   hipEvent_t ev[10] = {};
   // knnj_join_batch: streams the instances' index builds are spread over
   // (default priority, created on first use) and their completion events
@@ -80,6 +71,10 @@ struct knnj_ctx {
   hipEvent_t aux_ev[kAux + 1] = {};
   // autotune: problem shape -> chosen candidate (index into the candidate list)
   std::map<std::vector<int64_t>, int> tuned;
+  // knnj_join_batch with opts.graph: the captured index builds + fused joins
+  // of the last plan, and that plan's signature
+  hipGraphExec_t batch_exec = nullptr;
+  std::vector<unsigned char> batch_sig;
 };
 
 #define HIPCHK(expr)                                                                      \
@@ -543,6 +538,21 @@ knnj_status alloc_inst(Alloc &A, Inst &I, Tile *tiles) {
   return KNNJ_OK;
 }
 
+// The SideIndex view of one side's buffers (host-side only).
+void fill_side_index(Inst &I, bool ref_side) {
+  SideBufs &b = ref_side ? I.rb : I.qb;
+  SideIndex &out = ref_side ? I.Ri : I.Qi;
+  out.n = ref_side ? I.nR : I.nQ;
+  out.m = ref_side ? I.m_r : I.m_q;
+  out.dpad = I.dpad;
+  out.reps = b.reps;
+  out.rad = (const double *)b.radbits;
+  out.off = b.off;
+  out.perm = b.perm;
+  out.pdist = b.pdist;
+  out.px = b.px;
+}
+
 // One side's cluster index: sampled representatives, Lloyd refinement,
 // assignment, radii, and the cluster-sorted permuted copy.
 knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
@@ -550,11 +560,7 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
   const float *X = ref_side ? I.dR : I.dQ;
   const int64_t n = ref_side ? I.nR : I.nQ, ld = ref_side ? I.ldR : I.ldQ;
   const int m = ref_side ? I.m_r : I.m_q, d = I.d, dpad = I.dpad;
-  c->host_cand.emplace_back((size_t)m * I.trials);
-  std::vector<int32_t> &hc = c->host_cand.back();
-  uint64_t st = I.seed;
-  for (auto &v : hc) v = (int32_t)(splitmix64(st) % (uint64_t)n);
-  HIPCHK(hipMemcpyAsync(b.cand, hc.data(), hc.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  launch_draw(b.cand, m * I.trials, I.seed, n, s);
   // m0 sampled representatives, then `rounds` split rounds, each adding the
   // farthest members of the `step` widest clusters of the current set
   const int rounds = (m >= 64 && n >= 4 * (int64_t)m) ? I.split_rounds : 0;
@@ -563,8 +569,8 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
   launch_rep_select(X, ld, d, dpad, b.cand, mc, I.trials, b.partial, rep_score_blocks(mc), b.reps, b.rep_rows, s);
   for (int r = 0; r < rounds; ++r) {
     launch_assign(X, n, ld, d, dpad, b.reps, mc, b.cl, b.rb, b.rn, s);
-    HIPCHK(hipMemsetAsync(b.radbits, 0, sizeof(u64) * mc, s));
-    HIPCHK(hipMemsetAsync(b.cnt, 0, sizeof(int32_t) * mc, s));
+    launch_zero(b.radbits, sizeof(u64) * mc, s);
+    launch_zero(b.cnt, sizeof(int32_t) * mc, s);
     launch_assign_finish(X, n, ld, d, dpad, b.reps, b.cl, b.d2r, b.radbits, b.cnt, s);
     launch_split_reps(X, n, ld, d, dpad, mc, step, b.cl, b.d2r, b.gkeys, b.reps, s);
     mc += step;
@@ -573,30 +579,21 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
   // order), their mean becomes the representative
   for (int it = 0; it < I.refine; ++it) {
     launch_assign(X, n, ld, d, dpad, b.reps, m, b.cl, b.rb, b.rn, s);
-    HIPCHK(hipMemsetAsync(b.cnt, 0, sizeof(int32_t) * m, s));
-    HIPCHK(hipMemsetAsync(b.fill, 0, sizeof(int32_t) * m, s));
+    launch_zero(b.cnt, sizeof(int32_t) * m, s);
+    launch_zero(b.fill, sizeof(int32_t) * m, s);
     launch_count(b.cl, n, b.cnt, s);
     launch_bucket_rows(b.cl, n, m, b.cnt, b.off, b.fill, b.perm, b.gkeys, s);
     launch_centroid(X, ld, d, dpad, m, b.off, b.perm, I.integral ? 1 : 0, b.reps, s);
   }
   launch_assign(X, n, ld, d, dpad, b.reps, m, b.cl, b.rb, b.rn, s);
-  HIPCHK(hipMemsetAsync(b.radbits, 0, sizeof(u64) * m, s));
-  HIPCHK(hipMemsetAsync(b.cnt, 0, sizeof(int32_t) * m, s));
-  HIPCHK(hipMemsetAsync(b.fill, 0, sizeof(int32_t) * m, s));
+  launch_zero(b.radbits, sizeof(u64) * m, s);
+  launch_zero(b.cnt, sizeof(int32_t) * m, s);
+  launch_zero(b.fill, sizeof(int32_t) * m, s);
   launch_assign_finish(X, n, ld, d, dpad, b.reps, b.cl, b.d2r, b.radbits, b.cnt, s);
   launch_bucket_sort_permute(X, n, ld, d, dpad, m, b.cl, b.cnt, b.off, b.fill, b.perm, b.d2r, b.pdist, b.gkeys,
                              b.px, s);
   HIPCHK(hipGetLastError());
-  SideIndex &out = ref_side ? I.Ri : I.Qi;
-  out.n = n;
-  out.m = m;
-  out.dpad = dpad;
-  out.reps = b.reps;
-  out.rad = (const double *)b.radbits;
-  out.off = b.off;
-  out.perm = b.perm;
-  out.pdist = b.pdist;
-  out.px = b.px;
+  fill_side_index(I, ref_side);
   return KNNJ_OK;
 }
 
@@ -616,8 +613,8 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
   launch_bounds(I.Qi, I.Ri, I.d, I.k, I.theta, I.plist, I.pcnt, s);
   HIPCHK(hipGetLastError());
   if (ev_bounds) HIPCHK(hipEventRecord(ev_bounds, s));
-  HIPCHK(hipMemsetAsync(I.counters, 0, 8 * sizeof(int32_t), s));
-  HIPCHK(hipMemsetAsync(I.evals, 0, 4 * sizeof(u64), s));
+  launch_zero(I.counters, 8 * sizeof(int32_t), s);
+  launch_zero(I.evals, 4 * sizeof(u64), s);
   launch_make_tiles(I.Qi.off, I.m_q, I.theta, I.tq, I.id, I.tiles, I.counters, (int)I.max_slots,
                     I.o.xcd_tiles ? 8 : 1, s);
   HIPCHK(hipGetLastError());
@@ -935,6 +932,7 @@ void knnj_ctx_destroy(knnj_ctx *c) {
     if (kv.second.p) hipFree(kv.second.p);
   for (auto &e : c->ev)
     if (e) hipEventDestroy(e);
+  if (c->batch_exec) hipGraphExecDestroy(c->batch_exec);
   for (auto &e : c->aux_ev)
     if (e) hipEventDestroy(e);
   for (auto &x : c->aux)
@@ -961,7 +959,6 @@ knnj_status join_once(knnj_ctx *c, const knnj_matrix &R, const knnj_matrix &Q, i
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   hipEvent_t *ev = c->ev;
-  c->host_cand.clear();
   Inst I;
   describe(I, R, Q, k, o, out_idx, out_dist);
   HIPCHK(hipEventRecord(ev[0], s));
@@ -1094,7 +1091,6 @@ knnj_status knnj_index_build(knnj_ctx *c, knnj_matrix R, const knnj_opts *opts, 
   }
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
-  c->host_cand.clear();
   HIPCHK(hipEventRecord(c->ev[0], s));
   Inst I;
   I.d = R.d;
@@ -1247,7 +1243,6 @@ knnj_status knnj_index_query(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, in
   }
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
-  c->host_cand.clear();
   Inst I;
   I.pre = x;
   I.d = Q.d;
@@ -1312,7 +1307,6 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   hipEvent_t *ev = c->ev;
-  c->host_cand.clear();
   HIPCHK(hipEventRecord(ev[0], s));
   std::vector<Inst> I(n_inst);
   for (int i = 0; i < n_inst; ++i) {
@@ -1395,61 +1389,135 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
     for (int id : order) TRY(alloc_inst(A, I[id], all_tiles + slot0[id]));
   }
 
-  HIPCHK(hipEventRecord(ev[1], s));
-  // index builds, bounds and tile lists: independent per instance, so they
-  // are spread over the auxiliary streams and overlap on the device (most
-  // are many small launches); the fused joins wait for all of them
+  // kernel arguments: fixed by the plan (buffers and decisions), so they are
+  // filled and copied before any work is queued
+  for (int id : order) {
+    Inst &x = I[id];
+    fill_side_index(x, true);
+    if (x.self)
+      x.Qi = x.Ri;
+    else
+      fill_side_index(x, false);
+    fill_args(x);
+  }
+  JoinArgs *dargs;
+  TRY(ws(c, "batch_args", (size_t)n_inst, &dargs));
+  std::vector<JoinArgs> hargs(n_inst);
+  for (int id : order) hargs[id] = I[id].args;
+  HIPCHK(hipMemcpyAsync(dargs, hargs.data(), sizeof(JoinArgs) * n_inst, hipMemcpyHostToDevice, s));
   if (!c->aux[0]) {
     for (auto &x : c->aux) HIPCHK(hipStreamCreate(&x));
     for (auto &e : c->aux_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
-  HIPCHK(hipEventRecord(c->aux_ev[knnj_ctx::kAux], s));
-  for (int j = 0; j < knnj_ctx::kAux; ++j) HIPCHK(hipStreamWaitEvent(c->aux[j], c->aux_ev[knnj_ctx::kAux], 0));
-  {
-    // largest instances first, each to the stream with the least queued rows
-    std::vector<int> by_size(order);
-    std::stable_sort(by_size.begin(), by_size.end(),
-                     [&](int a, int b) { return I[a].nR + I[a].nQ > I[b].nR + I[b].nQ; });
-    int64_t load[knnj_ctx::kAux] = {};
-    for (int id : by_size) {
-      const int j = (int)(std::min_element(load, load + knnj_ctx::kAux) - load);
-      load[j] += I[id].nR + I[id].nQ;
-      TRY(enqueue_index(c, I[id], c->aux[j], nullptr, nullptr));
-    }
-  }
-  for (int j = 0; j < knnj_ctx::kAux; ++j) {
-    HIPCHK(hipEventRecord(c->aux_ev[j], c->aux[j]));
-    HIPCHK(hipStreamWaitEvent(s, c->aux_ev[j], 0));
-  }
-  HIPCHK(hipEventRecord(ev[2], s));
-  JoinArgs *dargs;
-  TRY(ws(c, "batch_args", (size_t)n_inst, &dargs));
-  std::vector<JoinArgs> hargs(n_inst);
-  for (int id : order) {
-    fill_args(I[id]);
-    hargs[id] = I[id].args;
-  }
-  HIPCHK(hipMemcpyAsync(dargs, hargs.data(), sizeof(JoinArgs) * n_inst, hipMemcpyHostToDevice, s));
-  HIPCHK(hipEventRecord(ev[8], s));
+
   int n_launch = 0;
-  for (size_t g = 0; g < order.size();) {
-    size_t h = g;
-    while (h < order.size() && I[order[h]].launch_key() == I[order[g]].launch_key()) ++h;
-    const Inst &P = I[order[g]];
-    const int64_t first = slot0[order[g]];
-    const int64_t last = slot0[order[h - 1]] + I[order[h - 1]].max_slots;
-    const int nslots = (int)(last - first);
-    const Tile *t0 = all_tiles + first;
-    if (P.use_mfma) {
-      if (P.bf16)
-        HIPCHK(launch_join_mfma_bf16_multi(P.args, P.bf16x1 ? 1 : 3, dargs, t0, nslots, s));
-      else
-        HIPCHK(launch_join_mfma_f32_multi(P.args, dargs, t0, nslots, s));
-    } else {
-      HIPCHK(launch_join_multi(P.args, P.tq, dargs, t0, nslots, s));
+  // index builds, bounds and tile lists: independent per instance, so they
+  // are spread over the auxiliary streams and overlap on the device (most
+  // are many small launches); the fused joins wait for all of them
+  auto enqueue_all = [&](bool timed) -> knnj_status {
+    HIPCHK(hipEventRecord(c->aux_ev[knnj_ctx::kAux], s));
+    for (int j = 0; j < knnj_ctx::kAux; ++j) HIPCHK(hipStreamWaitEvent(c->aux[j], c->aux_ev[knnj_ctx::kAux], 0));
+    {
+      // largest instances first, each to the stream with the least queued rows
+      std::vector<int> by_size(order);
+      std::stable_sort(by_size.begin(), by_size.end(),
+                       [&](int a, int b) { return I[a].nR + I[a].nQ > I[b].nR + I[b].nQ; });
+      int64_t load[knnj_ctx::kAux] = {};
+      for (int id : by_size) {
+        const int j = (int)(std::min_element(load, load + knnj_ctx::kAux) - load);
+        load[j] += I[id].nR + I[id].nQ;
+        TRY(enqueue_index(c, I[id], c->aux[j], nullptr, nullptr));
+      }
     }
-    ++n_launch;
-    g = h;
+    for (int j = 0; j < knnj_ctx::kAux; ++j) {
+      HIPCHK(hipEventRecord(c->aux_ev[j], c->aux[j]));
+      HIPCHK(hipStreamWaitEvent(s, c->aux_ev[j], 0));
+    }
+    if (timed) {
+      HIPCHK(hipEventRecord(ev[2], s));
+      HIPCHK(hipEventRecord(ev[8], s));
+    }
+    n_launch = 0;
+    for (size_t g = 0; g < order.size();) {
+      size_t h = g;
+      while (h < order.size() && I[order[h]].launch_key() == I[order[g]].launch_key()) ++h;
+      const Inst &P = I[order[g]];
+      const int64_t first = slot0[order[g]];
+      const int64_t last = slot0[order[h - 1]] + I[order[h - 1]].max_slots;
+      const int nslots = (int)(last - first);
+      const Tile *t0 = all_tiles + first;
+      if (P.use_mfma) {
+        if (P.bf16)
+          HIPCHK(launch_join_mfma_bf16_multi(P.args, P.bf16x1 ? 1 : 3, dargs, t0, nslots, s));
+        else
+          HIPCHK(launch_join_mfma_f32_multi(P.args, dargs, t0, nslots, s));
+      } else {
+        HIPCHK(launch_join_multi(P.args, P.tq, dargs, t0, nslots, s));
+      }
+      ++n_launch;
+      g = h;
+    }
+    return KNNJ_OK;
+  };
+
+  HIPCHK(hipEventRecord(ev[1], s));
+  bool graphed = false;
+  int graph_state = 0;
+  // fused launches of this plan (one per kernel instantiation)
+  int n_groups = 0;
+  for (size_t g = 0; g < order.size(); ++g)
+    if (g == 0 || !(I[order[g]].launch_key() == I[order[g - 1]].launch_key())) ++n_groups;
+  if (o.graph) {
+    // the plan's signature: every kernel argument and every host-side
+    // parameter of the queued work
+    std::vector<unsigned char> sig(reinterpret_cast<const unsigned char *>(hargs.data()),
+                                   reinterpret_cast<const unsigned char *>(hargs.data() + n_inst));
+    auto put = [&](const void *p, size_t n) {
+      sig.insert(sig.end(), (const unsigned char *)p, (const unsigned char *)p + n);
+    };
+    for (int id : order) {
+      const Inst &x = I[id];
+      const int64_t v[] = {x.id, (int64_t)(uintptr_t)x.dR, (int64_t)(uintptr_t)x.dQ, x.ldR, x.ldQ, x.nR, x.nQ,
+                           x.m_r, x.m_q, x.trials, (int64_t)x.seed, x.refine, x.split_rounds, x.integral,
+                           x.bf16, x.bf16x1, x.mfma, x.tq, x.max_slots, slot0[id], x.self};
+      put(v, sizeof(v));
+    }
+    put(&all_tiles, sizeof(all_tiles));
+    put(&dargs, sizeof(dargs));
+    if (c->batch_exec && sig == c->batch_sig) {
+      HIPCHK(hipGraphLaunch(c->batch_exec, s));
+      graphed = true;
+      graph_state = 2;
+    } else {
+      if (c->batch_exec) {
+        hipGraphExecDestroy(c->batch_exec);
+        c->batch_exec = nullptr;
+        c->batch_sig.clear();
+      }
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      const knnj_status e = enqueue_all(false);
+      hipGraph_t graph = nullptr;
+      const hipError_t ee = hipStreamEndCapture(s, &graph);
+      if (e != KNNJ_OK || ee != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        if (e != KNNJ_OK) return e;
+        set_err("graph capture failed: %s", hipGetErrorString(ee));
+        return KNNJ_EDEVICE;
+      }
+      const hipError_t ie = hipGraphInstantiate(&c->batch_exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        c->batch_exec = nullptr;
+        set_err("graph instantiation failed: %s", hipGetErrorString(ie));
+        return KNNJ_EDEVICE;
+      }
+      c->batch_sig = std::move(sig);
+      HIPCHK(hipGraphLaunch(c->batch_exec, s));
+      graphed = true;
+      graph_state = 1;
+    }
+  } else {
+    TRY(enqueue_all(true));
   }
   HIPCHK(hipEventRecord(ev[4], s));
   // counters of every join in one copy: the counter and evaluation blocks
@@ -1487,8 +1555,10 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   HIPCHK(hipEventRecord(ev[6], s));
   HIPCHK(hipStreamSynchronize(s));
   st.t_total_ms = elapsed(ev[0], ev[6]);
-  st.t_index_ms = elapsed(ev[1], ev[2]);  // index builds, centring, bounds and tile lists of all joins
-  st.t_join_ms = elapsed(ev[8], ev[4]);
+  // index builds, centring, bounds and tile lists of all joins; a replayed
+  // graph reports its whole time as join time
+  st.t_index_ms = graphed ? 0.0 : elapsed(ev[1], ev[2]);
+  st.t_join_ms = graphed ? elapsed(ev[1], ev[4]) : elapsed(ev[8], ev[4]);
   st.t_fallback_ms = novf > 0 ? elapsed(ev[5], ev[6]) : 0.0;
   st.n_dist_evals = evals;
   st.n_merges = merges;
@@ -1498,7 +1568,9 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   st.pruning_ratio = tot > 0 ? 1.0 - (double)evals / tot : 0.0;
   st.filter_max_abs_err = ferr;
   if (!order.empty()) plan_stats(I[order[0]], st);
-  st.n_launches = n_launch;
+  st.n_launches = n_groups;
+  st.graph = graph_state;
+  (void)n_launch;
   if (stats) *stats = st;
   return KNNJ_OK;
 }

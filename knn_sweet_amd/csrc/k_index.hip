@@ -18,6 +18,36 @@
 #include "wave_sort.h"
 
 // ---------------------------------------------------------------------------
+// Zero fill (a kernel, not a memset node, so the enqueue path can be
+// captured into a HIP graph and replayed).
+__global__ void k_zero32(u32 *p, int64_t n) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+void launch_zero(void *p, size_t bytes, hipStream_t s) {
+  const int64_t n = (int64_t)(bytes / 4);
+  if (n > 0) hipLaunchKernelGGL(k_zero32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (u32 *)p, n);
+}
+
+// Representative draws: element i is SplitMix64's output number i (the
+// state after i + 1 increments, mixed) modulo n -- the same sequence a host
+// loop of splitmix64() calls gives, computed independently per element.
+__global__ void k_draw(int32_t *cand, int count, uint64_t seed, int64_t n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint64_t z = seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  cand[i] = (int32_t)(z % (uint64_t)n);
+}
+
+void launch_draw(int32_t *cand, int count, uint64_t seed, int64_t n, hipStream_t s) {
+  if (count > 0) hipLaunchKernelGGL(k_draw, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, cand, count, seed, n);
+}
+
+// ---------------------------------------------------------------------------
 // Input scan: non-finite values, non-integers, max |x|.
 // flags[0]: any non-finite; flags[1]: float bits of max |x|; flags[2]: any non-integer.
 __global__ void k_scan_input(const float *__restrict__ X, int64_t n, int d, int64_t ld, u32 *flags) {
@@ -596,7 +626,7 @@ __global__ __launch_bounds__(1024) void k_split_far(const float *__restrict__ X,
 
 void launch_split_reps(const float *X, int64_t n, int64_t ld, int d, int dpad, int m0, int extra,
                        const int32_t *cl, const float *d2r, u64 *far, float *reps, hipStream_t s) {
-  hipMemsetAsync(far, 0, sizeof(u64) * m0, s);
+  launch_zero(far, sizeof(u64) * m0, s);
   hipLaunchKernelGGL(k_far_member, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cl, d2r, n, far);
   hipLaunchKernelGGL(k_split_far, dim3(1), dim3(1024), 0, s, X, ld, d, dpad, m0, extra, (const u64 *)far, reps);
 }

@@ -6,6 +6,9 @@
 #include "knnj_common.h"
 
 // k_index.hip
+void launch_zero(void *p, size_t bytes, hipStream_t s);  // bytes: a multiple of 4
+// cand[i] = SplitMix64 output i (state seed, i + 1 increments) mod n
+void launch_draw(int32_t *cand, int count, uint64_t seed, int64_t n, hipStream_t s);
 void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags, hipStream_t s);
 int rep_score_blocks(int m);  // partial sums per trial of launch_rep_select
 void launch_rep_select(const float *X, int64_t ld, int d, int dpad, const int32_t *cand, int m, int trials,

@@ -103,3 +103,33 @@ def test_batch_errors_name_the_instance(ctx):
     assert "instance 1" in str(e.value) and "non-finite" in str(e.value)
     res, st = ctx.join_batch([])
     assert res == [] and st["n_dist_evals"] == 0
+
+
+def test_batch_graph_replays_match_eager(ctx):
+    """The captured plan replays bit-identically: three replays, with eager
+    work and synchronisations between them, each equal to the eager batch;
+    a changed plan is captured anew."""
+    import torch
+    inst = datagen.batch_instances(n_inst=8, seed=98, r_lo=500, r_hi=8000)
+    dev = [(_dev(R), _dev(Q), 16) for R, Q in inst]
+    eager, st0 = ctx.join_batch(dev)
+    assert st0["graph"] == 0
+    outs, st1 = ctx.join_batch(dev, graph=True)
+    assert st1["graph"] == 1
+    X = _dev(inst[0][0])
+    for rep in range(3):
+        for o in outs:
+            o[0].fill_(-1)
+            o[1].fill_(-1.0)
+        ctx.join(X, X, 5)  # eager work between replays
+        torch.cuda.synchronize()
+        _, st = ctx.join_batch(dev, outputs=outs, graph=True)
+        assert st["graph"] == 2, rep
+        for i, ((a_i, a_d), (e_i, e_d)) in enumerate(zip(outs, eager)):
+            _assert_same(a_i, e_i, f"replay {rep} instance {i} idx")
+            _assert_same(a_d, e_d, f"replay {rep} instance {i} dist")
+    # a different plan (one instance fewer) is captured again and still exact
+    res, st = ctx.join_batch(dev[:-1], outputs=outs[:-1], graph=True)
+    assert st["graph"] == 1
+    for (R, Q), (idx, dist) in zip(inst[:-1], res):
+        assert_matches_oracle(R, Q, 16, _host(idx), _host(dist))
