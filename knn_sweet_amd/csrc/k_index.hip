@@ -325,94 +325,149 @@ __global__ void k_count(const int32_t *__restrict__ cl, int64_t n, int32_t *cnt)
   if (i < n) atomicAdd(&cnt[cl[i]], 1);
 }
 
-// Nearest representative on the matrix cores: argmin_c ||c||^2 - 2 x.c with
-// x and c rounded to bf16 and FP32 accumulation (v_mfma_f32_16x16x32_bf16).
-// The argmin only has to be good, not exact: every later bound uses the
-// binary64 distance to whichever representative was chosen.  One wave takes
-// 64 rows (four 16-row A blocks) against all representatives, 16 at a time;
-// ties go to the lower representative index.
+// Nearest representative on the matrix cores.  Rows x and representatives
+// c are shifted by the first representative c0 (so an offset common to the
+// data costs no bf16 precision) and rounded to bf16: x~, c~.  The MFMA
+// accumulators start at -||x~||^2 / 2, so D = ||c~||^2 - 2 acc is the FP32
+// squared distance ||x~ - c~||^2.  The argmin only has to be good, not exact:
+// every later bound uses the binary64 distance to whichever representative
+// was chosen.
+//
+// DK (a multiple of 32, >= dk) is the compiled width: columns dk .. DK-1 are
+// zero in both operands.  A workgroup is 4 waves x (16 AB) rows, A operands in registers; the
+// representatives stream through LDS in chunks of 64 (double-buffered, one
+// barrier per chunk, rows padded by 16 bytes against bank conflicts), so
+// every chunk is read from L2 once per workgroup.  Per (row, 16-rep block)
+// the running minimum is one signed 32-bit min over a key: D's bits with the
+// low `nb` bits replaced by the block number (D >= 0 up to rounding, and a
+// negative D -- a distance that rounded below zero -- is smaller than every
+// positive one).  Equal keys go to the lower representative.
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 as_bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float as_f32x4;
 
 template <int DK>
-__global__ __launch_bounds__(256) void k_assign_mfma(const float *__restrict__ X, int64_t n, int64_t ld, int d,
-                                                     const __bf16 *__restrict__ rb, const float *__restrict__ rn,
-                                                     int m, int dk, int32_t *cl) {
-  // one wave: AB = 4 blocks of 16 rows against 16 representatives at a time;
-  // the next 16 representatives' operands are loaded while this block's
-  // MFMAs run
-  constexpr int NS = DK / 32, AB = 4;
-  const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
-  const int64_t base = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * AB);
-  if (base >= n) return;
-  const int nst = dk / 32;
+__global__ __launch_bounds__(256, 2) void k_assign_mfma(const float *__restrict__ X, int64_t n, int64_t ld, int d,
+                                                        const float *__restrict__ c0, const __bf16 *__restrict__ rb,
+                                                        const float *__restrict__ rn, int m, int dk, int32_t *cl) {
+  constexpr int NS = DK / 32, AB = DK <= 128 ? 4 : 2, CH = 64, RS = DK + 8, PPR = DK / 8;
+  constexpr int PT = CH * PPR / 256;
+  __shared__ __attribute__((aligned(16))) __bf16 sb[2][CH * RS];
+  __shared__ float sn[2][CH];
+  const int tid = threadIdx.x, lane = tid & 63, g = lane >> 4, c16 = lane & 15;
+  const int64_t base = ((int64_t)blockIdx.x * 4 + (tid >> 6)) * (16 * AB);
+  const bool live = base < n;  // wave-uniform; every wave takes part in the staging
+  const int nblk = (m + 15) >> 4;
+  const int nb = nblk > 1 ? 32 - __builtin_clz((unsigned)(nblk - 1)) : 0;
+  const int lowmask = (1 << nb) - 1, himask = ~lowmask;
+
   as_bf16x8 a[AB][NS];
-#pragma unroll
-  for (int ab = 0; ab < AB; ++ab) {
-    const int64_t p = base + 16 * ab + c16;
+  as_f32x4 cinit[AB];
+  if (live) {
+    float cv[NS][8];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int dim = 32 * s + 8 * g + j;
-        a[ab][s][j] = (__bf16)((s < nst && dim < d && p < n) ? X[p * ld + dim] : 0.f);
+        cv[s][j] = dim < d ? c0[dim] : 0.f;
       }
+#pragma unroll
+    for (int ab = 0; ab < AB; ++ab) {
+      const int64_t p = base + 16 * ab + c16;
+      float part = 0.f;
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int dim = 32 * s + 8 * g + j;
+          const __bf16 v = (__bf16)((dim < d && p < n) ? X[p * ld + dim] - cv[s][j] : 0.f);
+          a[ab][s][j] = v;
+          const float f = (float)v;
+          part = __builtin_fmaf(f, f, part);
+        }
+      // ||x~||^2 of row c16 in all four lanes holding it, then to the
+      // accumulator layout (lane g holds rows 4g .. 4g+3)
+      part += __shfl_xor(part, 16);
+      part += __shfl_xor(part, 32);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) cinit[ab][i] = -0.5f * __shfl(part, 4 * g + i);
+    }
   }
-  float best[AB][4];
-  int bidx[AB][4];
+  int best[AB][4];
 #pragma unroll
   for (int ab = 0; ab < AB; ++ab)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      best[ab][i] = INFINITY;
-      bidx[ab][i] = 0;
+    for (int i = 0; i < 4; ++i) best[ab][i] = 0x7fffffff;
+
+  as_bf16x8 stg[PT];
+  float stn = 0.f;
+  auto gload = [&](int r0) {
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+      const int e = tid + 256 * q, r = e / PPR, t = e % PPR;
+      const int rep = r0 + r;
+      if (rep < m && 8 * t < dk)
+        stg[q] = *(const as_bf16x8 *)(rb + (int64_t)rep * dk + 8 * t);
+      else
+        stg[q] = as_bf16x8{};
     }
-  auto load_b = [&](int r0, as_bf16x8 (&bv)[NS], float &nr) {
-    const int rep = r0 + c16;
-    const bool ok = rep < m;
-    const __bf16 *rr = rb + (int64_t)(ok ? rep : 0) * dk + 8 * g;
-#pragma unroll
-    for (int s = 0; s < NS; ++s)
-      if (s < nst) bv[s] = *(const as_bf16x8 *)(rr + 32 * s);
-    nr = ok ? rn[rep] : INFINITY;
+    if (tid < CH) stn = r0 + tid < m ? rn[r0 + tid] : INFINITY;
   };
-  as_bf16x8 bc[NS], bn[NS];
-  float nrc, nrn = INFINITY;
-  load_b(0, bc, nrc);
-  for (int r0 = 0; r0 < m; r0 += 16) {
-    if (r0 + 16 < m) load_b(r0 + 16, bn, nrn);
-    const int rep = r0 + c16;
+  auto lstore = [&](int buf) {
 #pragma unroll
-    for (int ab = 0; ab < AB; ++ab) {
-      as_f32x4 c = {0.f, 0.f, 0.f, 0.f};
+    for (int q = 0; q < PT; ++q) {
+      const int e = tid + 256 * q, r = e / PPR, t = e % PPR;
+      *(as_bf16x8 *)&sb[buf][r * RS + 8 * t] = stg[q];
+    }
+    if (tid < CH) sn[buf][tid] = stn;
+  };
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  const int nch = (m + CH - 1) / CH;
+  for (int ch = 0; ch < nch; ++ch) {
+    const int buf = ch & 1;
+    if (ch + 1 < nch) gload((ch + 1) * CH);
+    if (live) {
+#pragma unroll 1
+      for (int blk = 0; blk < CH / 16; ++blk) {
+        const int rl = blk * 16 + c16;
+        as_bf16x8 bv[NS];
 #pragma unroll
-      for (int s = 0; s < NS; ++s)
-        if (s < nst) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ab][s], bc[s], c, 0, 0, 0);
+        for (int s = 0; s < NS; ++s) bv[s] = *(const as_bf16x8 *)&sb[buf][rl * RS + 32 * s + 8 * g];
+        const float nrc = sn[buf][rl];
+        const int bno = __builtin_amdgcn_readfirstlane(ch * (CH / 16) + blk);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float v = nrc - 2.f * c[i];
-        if (v < best[ab][i]) {
-          best[ab][i] = v;
-          bidx[ab][i] = rep;
+        for (int ab = 0; ab < AB; ++ab) {
+          as_f32x4 c = cinit[ab];
+#pragma unroll
+          for (int s = 0; s < NS; ++s) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ab][s], bv[s], c, 0, 0, 0);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float dd = __builtin_fmaf(-2.f, c[i], nrc);
+            int key;  // (bits(dd) & himask) | bno in one instruction
+            asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(key) : "v"(dd), "v"(himask), "s"(bno));
+            best[ab][i] = min(best[ab][i], key);
+          }
         }
       }
     }
-#pragma unroll
-    for (int s = 0; s < NS; ++s) bc[s] = bn[s];
-    nrc = nrn;
+    if (ch + 1 < nch) lstore(buf ^ 1);
+    __syncthreads();
   }
+  if (!live) return;
 #pragma unroll
   for (int ab = 0; ab < AB; ++ab)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      float b = best[ab][i];
-      int bi = bidx[ab][i];
+      int kb = best[ab][i];
+      int bi = ((kb & lowmask) << 4) + c16;
 #pragma unroll
       for (int msk = 1; msk < 16; msk <<= 1) {
-        const float ov = __shfl_xor(b, msk);
+        const int ok = __shfl_xor(kb, msk);
         const int oi = __shfl_xor(bi, msk);
-        if (ov < b || (ov == b && oi < bi)) {
-          b = ov;
+        if (ok < kb || (ok == kb && oi < bi)) {
+          kb = ok;
           bi = oi;
         }
       }
@@ -421,7 +476,8 @@ __global__ __launch_bounds__(256) void k_assign_mfma(const float *__restrict__ X
     }
 }
 
-// bf16 copy of the representatives (row width dk) and their FP32 squared norms.
+// bf16 copy of the representatives shifted by the first one (row width dk)
+// and the FP32 squared norms of the rounded rows.
 __global__ void k_reps_bf16(const float *__restrict__ reps, int m, int d, int dpad, int dk, __bf16 *rb,
                             float *rn) {
   const int r = blockIdx.x;
@@ -429,9 +485,10 @@ __global__ void k_reps_bf16(const float *__restrict__ reps, int m, int d, int dp
   const int lane = threadIdx.x;
   float part = 0.f;
   for (int t = lane; t < dk; t += 64) {
-    const float v = t < d ? reps[(int64_t)r * dpad + t] : 0.f;
-    rb[(int64_t)r * dk + t] = (__bf16)v;
-    part = __builtin_fmaf(v, v, part);
+    const __bf16 v = (__bf16)(t < d ? reps[(int64_t)r * dpad + t] - reps[t] : 0.f);
+    rb[(int64_t)r * dk + t] = v;
+    const float f = (float)v;
+    part = __builtin_fmaf(f, f, part);
   }
   part = wave_sum(part);
   if (lane == 0) rn[r] = part;
@@ -496,11 +553,20 @@ void launch_assign(const float *X, int64_t n, int64_t ld, int d, int dpad, const
   const int dk = (d + 31) & ~31;
   if (d >= 16 && dk <= 256 && rb && rn) {
     hipLaunchKernelGGL(k_reps_bf16, dim3(m), dim3(64), 0, s, reps, m, d, dpad, dk, rb, rn);
-    const dim3 g((unsigned)((n + 255) / 256));  // 64 rows per wave, 4 waves
-    if (dk <= 128)
-      hipLaunchKernelGGL(k_assign_mfma<128>, g, dim3(256), 0, s, X, n, ld, d, rb, rn, m, dk, cl);
+    // 4 waves of 64 rows (DK <= 128) or of 32 rows
+    const dim3 g64((unsigned)((n + 255) / 256)), g32((unsigned)((n + 127) / 128));
+    if (dk <= 32)
+      hipLaunchKernelGGL(k_assign_mfma<32>, g64, dim3(256), 0, s, X, n, ld, d, reps, rb, rn, m, dk, cl);
+    else if (dk <= 64)
+      hipLaunchKernelGGL(k_assign_mfma<64>, g64, dim3(256), 0, s, X, n, ld, d, reps, rb, rn, m, dk, cl);
+    else if (dk <= 96)
+      hipLaunchKernelGGL(k_assign_mfma<96>, g64, dim3(256), 0, s, X, n, ld, d, reps, rb, rn, m, dk, cl);
+    else if (dk <= 128)
+      hipLaunchKernelGGL(k_assign_mfma<128>, g64, dim3(256), 0, s, X, n, ld, d, reps, rb, rn, m, dk, cl);
+    else if (dk <= 192)
+      hipLaunchKernelGGL(k_assign_mfma<192>, g32, dim3(256), 0, s, X, n, ld, d, reps, rb, rn, m, dk, cl);
     else
-      hipLaunchKernelGGL(k_assign_mfma<256>, g, dim3(256), 0, s, X, n, ld, d, rb, rn, m, dk, cl);
+      hipLaunchKernelGGL(k_assign_mfma<256>, g32, dim3(256), 0, s, X, n, ld, d, reps, rb, rn, m, dk, cl);
     return;
   }
   const dim3 g((unsigned)((n + 255) / 256));
