@@ -337,7 +337,7 @@ struct Inst {
   // kernel instantiation: joins with equal keys can share one launch
   std::tuple<int, int, int, int, int, int> launch_key() const {
     return std::make_tuple(use_mfma ? (bf16 ? (bf16x1 ? 3 : 2) : 1) : 0, kcap, kernel_exact(), tq,
-                           use_mfma && dk > 128 ? 1 : 0, split);
+                           use_mfma ? dk : 0, split);
   }
 };
 
@@ -425,7 +425,7 @@ knnj_status decide(Inst &I, const u32 *flags) {
   I.bf16x1 = I.bf16 && o.precision == KNNJ_PREC_BF16X1;
   I.exact = I.mfma ? (!I.bf16 && exact_mfma) : I.exact_valu;
   I.dpad = (d + 3) & ~3;
-  I.dk = I.mfma ? (d + 31) & ~31 : 0;
+  I.dk = I.mfma ? mfma_dk(d) : 0;
   if (I.pre) {
     // the index fixed the kernel and the operand planes of the reference rows
     const knnj_index &X = *I.pre;

@@ -159,9 +159,11 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
       return r;
     }
   };
-  const int d = a.d, k = a.k, dk = a.dk, dpQ = a.Q.dpad;
+  // the operand planes are DK wide (zero beyond d): every 32-dimension step
+  // runs, with no per-step branch between the MFMAs
+  constexpr int dk = DK;
+  const int d = a.d, k = a.k, dpQ = a.Q.dpad;
   const float eps = dist_relerr(d);
-  const int nst = dk / 32;
 
   for (int e = lane; e < MF_TQ * KCAP; e += 64) lists[e] = KEY_MAX;
 
@@ -206,7 +208,8 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     }
   };
   float qn = 0.f;
-  for (int s = 0; s < nst; ++s) {
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
     float v[8];
     q_slice(s, v);
 #pragma unroll
@@ -260,7 +263,8 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   auto center_dots = [&](const int bj) {
     f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
     const float *cr = a.reps32 + (int64_t)bj * dk + 8 * g;
-    for (int s = 0; s < nst; ++s) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
       float qv[8];
       q_slice(s, qv);
       const f32x4_t c0 = *(const f32x4_t *)(cr + 32 * s);
@@ -341,32 +345,20 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
           float part = 0.f;
   #pragma unroll
           for (int s = 0; s < NS; ++s) {
-            if (s < nst) {
-              const f32x4_t c0 = *(const f32x4_t *)(rep + 32 * s);
-              const f32x4_t c1 = *(const f32x4_t *)(rep + 32 * s + 4);
-              float qv[8];
-              q_slice(s, qv);
+            const f32x4_t c0 = *(const f32x4_t *)(rep + 32 * s);
+            const f32x4_t c1 = *(const f32x4_t *)(rep + 32 * s + 4);
+            float qv[8];
+            q_slice(s, qv);
   #pragma unroll
-              for (int j = 0; j < 8; ++j) {
-                const float v = alive ? qv[j] - (j < 4 ? c0[j] : c1[j - 4]) : 0.f;
-                part = __builtin_fmaf(v, v, part);
-                if constexpr (BF16) {
-                  const __bf16 h = (__bf16)v;
-                  av[s][j] = h;
-                  if constexpr (SPLIT) avl[s][j] = (__bf16)(v - (float)h);
-                } else {
-                  av[s][j] = v;
-                }
-              }
-            } else {
-  #pragma unroll
-              for (int j = 0; j < 8; ++j) {
-                if constexpr (BF16) {
-                  av[s][j] = (__bf16)0.f;
-                  if constexpr (SPLIT) avl[s][j] = (__bf16)0.f;
-                } else {
-                  av[s][j] = 0.f;
-                }
+            for (int j = 0; j < 8; ++j) {
+              const float v = alive ? qv[j] - (j < 4 ? c0[j] : c1[j - 4]) : 0.f;
+              part = __builtin_fmaf(v, v, part);
+              if constexpr (BF16) {
+                const __bf16 h = (__bf16)v;
+                av[s][j] = h;
+                if constexpr (SPLIT) avl[s][j] = (__bf16)(v - (float)h);
+              } else {
+                av[s][j] = v;
               }
             }
           }
@@ -430,20 +422,19 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
                 G.validv[nb] = p < pend;
                 const int pp = G.validv[nb] ? p : g0;
   #pragma unroll
-                for (int s = 0; s < NS; ++s)
-                  if (s < nst) {
-                    if constexpr (SPLIT) {
-                      const __bf16 *rr = (const __bf16 *)a.rc + (int64_t)pp * 2 * dk + 8 * g + 32 * s;
-                      G.bo[nb][s].h = *(const bf16x8_t *)rr;
-                      G.bo[nb][s].l = *(const bf16x8_t *)(rr + dk);
-                    } else if constexpr (BF16) {
-                      G.bo[nb][s].h = *(const bf16x8_t *)((const __bf16 *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s);
-                    } else {
-                      const float *rr = (const float *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s;
-                      G.bo[nb][s].f0 = *(const f32x4_t *)rr;
-                      G.bo[nb][s].f1 = *(const f32x4_t *)(rr + 4);
-                    }
+                for (int s = 0; s < NS; ++s) {
+                  if constexpr (SPLIT) {
+                    const __bf16 *rr = (const __bf16 *)a.rc + (int64_t)pp * 2 * dk + 8 * g + 32 * s;
+                    G.bo[nb][s].h = *(const bf16x8_t *)rr;
+                    G.bo[nb][s].l = *(const bf16x8_t *)(rr + dk);
+                  } else if constexpr (BF16) {
+                    G.bo[nb][s].h = *(const bf16x8_t *)((const __bf16 *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s);
+                  } else {
+                    const float *rr = (const float *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s;
+                    G.bo[nb][s].f0 = *(const f32x4_t *)rr;
+                    G.bo[nb][s].f1 = *(const f32x4_t *)(rr + 4);
                   }
+                }
                 G.nrv[nb] = a.nr[pp];
                 G.rowv[nb] = EXACT ? (u32)a.R.perm[pp] : (u32)pp;
               }
@@ -461,30 +452,43 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
                   ecd[i] = a.ecode ? (ecode_encode(Ei[i], ec) << ec.rowbits) : 0u;
                 }
               }
-              // compute stage
+              // MFMA stage: the NB blocks are independent accumulation
+              // chains, issued interleaved (blocks past the group's end hold
+              // clamped in-bounds rows; their results are not used)
+              f32x4_t accv[NB];
+  #pragma unroll
+              for (int nb = 0; nb < NB; ++nb) accv[nb] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  #pragma unroll
+              for (int s = 0; s < NS; ++s) {
+                if constexpr (SPLIT) {
+  #pragma unroll
+                  for (int nb = 0; nb < NB; ++nb)
+                    accv[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(avl[s], G.bo[nb][s].h, accv[nb], 0, 0, 0);
+  #pragma unroll
+                  for (int nb = 0; nb < NB; ++nb)
+                    accv[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].l, accv[nb], 0, 0, 0);
+  #pragma unroll
+                  for (int nb = 0; nb < NB; ++nb)
+                    accv[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].h, accv[nb], 0, 0, 0);
+                } else if constexpr (BF16) {
+  #pragma unroll
+                  for (int nb = 0; nb < NB; ++nb)
+                    accv[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].h, accv[nb], 0, 0, 0);
+                } else {
+  #pragma unroll
+                  for (int j = 0; j < 8; ++j)
+  #pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                      accv[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                          av[s][j], j < 4 ? G.bo[nb][s].f0[j] : G.bo[nb][s].f1[j - 4], accv[nb], 0, 0, 0);
+                }
+              }
+              // epilogue per block
   #pragma unroll
               for (int nb = 0; nb < NB; ++nb) {
                 if (g0 + 16 * nb >= pend) break;
                 const bool valid = G.validv[nb];
-                f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-  #pragma unroll
-                for (int s = 0; s < NS; ++s)
-                  if (s < nst) {
-                    if constexpr (SPLIT) {
-                      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(avl[s], G.bo[nb][s].h, acc, 0, 0, 0);
-                      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].l, acc, 0, 0, 0);
-                      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].h, acc, 0, 0, 0);
-                    } else if constexpr (BF16) {
-                      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].h, acc, 0, 0, 0);
-                    } else {
-  #pragma unroll
-                      for (int j = 0; j < 4; ++j)
-                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][j], G.bo[nb][s].f0[j], acc, 0, 0, 0);
-  #pragma unroll
-                      for (int j = 0; j < 4; ++j)
-                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s][4 + j], G.bo[nb][s].f1[j], acc, 0, 0, 0);
-                    }
-                  }
+                const f32x4_t acc = accv[nb];
                 const float nrb = G.nrv[nb];
                 const u32 row = G.rowv[nb];
                 bool pass[4];
@@ -758,46 +762,43 @@ inline size_t mfma_lds(int kcap, int dkmax, int waves) {
   return waves * mfma_wave_words(kcap, dkmax) * sizeof(u64) + 1024;
 }
 
-template <int PREC>
-hipError_t launch_mfma_any(const JoinArgs &a, int64_t grid, hipStream_t s, const JoinArgs *argv = nullptr,
-                           const Tile *tiles = nullptr) {
-  const bool big = a.dk > 128;  // dk = roundup(d, 32)
-  const int waves = mfma_launch_waves(a.kcap, big ? 256 : 128, a.waves);
-  const size_t lds = mfma_lds(a.kcap, big ? 256 : 128, waves);
-#define KJ_LAUNCH_W(KC, EX, DKV, WV)                                                                            \
+// Launch of one operand width DK (a.dk == DK): the capacity, exact-mode and
+// waves-per-workgroup instantiations.  Each width is instantiated in its own
+// translation unit (k_join_mfma_d<DK>.hip); k_join_mfma.hip dispatches.
+template <int PREC, int DK>
+hipError_t launch_mfma_dk(const JoinArgs &a, int64_t grid, hipStream_t s, const JoinArgs *argv,
+                          const Tile *tiles) {
+  const int waves = mfma_launch_waves(a.kcap, DK, a.waves);
+  const size_t lds = mfma_lds(a.kcap, DK, waves);
+#define KJ_LAUNCH_W(KC, EX, WV)                                                                                \
   do {                                                                                                          \
     if (argv) {                                                                                                 \
-      hipFuncSetAttribute((const void *)k_join_mfma_multi<KC, EX, PREC, DKV, WV>,                               \
+      hipFuncSetAttribute((const void *)k_join_mfma_multi<KC, EX, PREC, DK, WV>,                                \
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                \
-      hipLaunchKernelGGL((k_join_mfma_multi<KC, EX, PREC, DKV, WV>), dim3((unsigned)grid), dim3(64 * WV), lds, s, \
+      hipLaunchKernelGGL((k_join_mfma_multi<KC, EX, PREC, DK, WV>), dim3((unsigned)grid), dim3(64 * WV), lds, s, \
                          argv, tiles, (int)grid);                                                               \
     } else {                                                                                                    \
-      hipFuncSetAttribute((const void *)k_join_mfma<KC, EX, PREC, DKV, WV>,                                     \
+      hipFuncSetAttribute((const void *)k_join_mfma<KC, EX, PREC, DK, WV>,                                      \
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                \
-      hipLaunchKernelGGL((k_join_mfma<KC, EX, PREC, DKV, WV>), dim3((unsigned)grid), dim3(64 * WV), lds, s, a);  \
+      hipLaunchKernelGGL((k_join_mfma<KC, EX, PREC, DK, WV>), dim3((unsigned)grid), dim3(64 * WV), lds, s, a);   \
     }                                                                                                           \
   } while (0)
-#define KJ_LAUNCH(KC, EX, DKV)                                   \
+#define KJ_LAUNCH(KC, EX)                                        \
   do {                                                           \
-    constexpr int WM = mfma_waves(KC, DKV);                      \
+    constexpr int WM = mfma_waves(KC, DK);                       \
     if constexpr (WM > 1) {                                      \
       if (waves > 1) {                                           \
-        KJ_LAUNCH_W(KC, EX, DKV, WM);                            \
+        KJ_LAUNCH_W(KC, EX, WM);                                 \
         break;                                                   \
       }                                                          \
     }                                                            \
-    KJ_LAUNCH_W(KC, EX, DKV, 1);                                 \
+    KJ_LAUNCH_W(KC, EX, 1);                                      \
   } while (0)
-#define KJ_DK(KC, EX)       \
-  if (big)                  \
-    KJ_LAUNCH(KC, EX, 256); \
-  else                      \
-    KJ_LAUNCH(KC, EX, 128);
-#define KJ_EX(KC)                                  \
-  if (a.exact && PREC == 0) {                      \
-    if constexpr (PREC == 0) { KJ_DK(KC, true) }   \
-  } else {                                         \
-    KJ_DK(KC, false)                               \
+#define KJ_EX(KC)                                    \
+  if (a.exact && PREC == 0) {                        \
+    if constexpr (PREC == 0) { KJ_LAUNCH(KC, true); } \
+  } else {                                           \
+    KJ_LAUNCH(KC, false);                            \
   }
   switch (a.kcap) {
     case 64: KJ_EX(64) break;
@@ -807,8 +808,14 @@ hipError_t launch_mfma_any(const JoinArgs &a, int64_t grid, hipStream_t s, const
     default: return hipErrorInvalidValue;
   }
 #undef KJ_EX
-#undef KJ_DK
 #undef KJ_LAUNCH
 #undef KJ_LAUNCH_W
   return hipGetLastError();
 }
+
+// operand widths with a compiled kernel (dk = mfma_dk(d))
+#define KJ_FOR_WIDTHS(X) X(32) X(64) X(96) X(128) X(256)
+#define KJ_DK_INST(P, DKV) \
+  template hipError_t launch_mfma_dk<P, DKV>(const JoinArgs &, int64_t, hipStream_t, const JoinArgs *, const Tile *);
+#define KJ_DK_EXTERN(P, DKV) \
+  extern template hipError_t launch_mfma_dk<P, DKV>(const JoinArgs &, int64_t, hipStream_t, const JoinArgs *, const Tile *);

@@ -1,0 +1,8 @@
+// k_join_mfma_d256.hip -- the MFMA join kernels for operand width 256
+// (plain FP32, bf16 and bf16 hi/lo filters), one translation unit per width
+// so the widths compile in parallel.
+#include "k_join_mfma.h"
+
+KJ_DK_INST(0, 256)
+KJ_DK_INST(1, 256)
+KJ_DK_INST(3, 256)

@@ -88,7 +88,7 @@ hipError_t launch_join(const JoinArgs &a, int tq, int64_t max_tiles, hipStream_t
 void launch_center(const SideIndex &Ri, int d, int dk, int planes, void *rc, float *nr, float *reps32, float *rn32,
                    hipStream_t s);
 
-// k_join_mfma_*.hip
+// k_join_mfma.hip (kernels per operand width in k_join_mfma_d*.hip)
 hipError_t launch_join_mfma_f32(const JoinArgs &a, int64_t max_tiles, hipStream_t s);
 // terms: 1 single bf16 product, 3 compensated hi/lo products
 hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles, hipStream_t s);
@@ -97,6 +97,9 @@ hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles
 int join_mfma_tile_q(int kcap, int dk, int want_waves);
 // waves of the MFMA join's split mode for (kcap, dk): 1 if it does not fit
 int join_mfma_split_waves(int kcap, int dk);
+// Operand width of the MFMA join for dimension d: the planes (centred rows,
+// representatives, query rows) are zero padded to one of the compiled widths.
+inline int mfma_dk(int d) { return d <= 32 ? 32 : d <= 64 ? 64 : d <= 96 ? 96 : d <= 128 ? 128 : 256; }
 
 // Fused launches over the tiles of many instances: slot b runs tile tiles[b]
 // of instance tiles[b].pad with arguments argv[tiles[b].pad]; proto carries
