@@ -13,6 +13,13 @@
 //     never computes r_b), list sorted by d(a,b) ascending.
 // All bound arithmetic is binary64; reference-member distances stored as
 // binary32 are scaled up by (1 + 2^-22) before entering an upper bound.
+//
+// k_bounds (64 KiB of LDS, two workgroups per CU) computes the distances with
+// 4 lanes per representative and 16-byte loads, takes the clusters of step 2
+// one at a time by a workgroup min-reduction (a few clusters usually hold 2k
+// points), and sorts only the surviving list.  When BD_SEL clusters still
+// hold fewer than k points it marks the query cluster (pcnt = -1) and
+// k_bounds_full, which sorts every representative, redoes it.
 #include "knnj_common.h"
 #include "knnj_kernels.h"
 
@@ -38,8 +45,8 @@ __device__ __forceinline__ void block_sort_u64(u64 *a, int p2) {
   }
 }
 
-__global__ __launch_bounds__(BD_THREADS) void k_bounds(SideIndex Qi, SideIndex Ri, int d, int k,
-                                                       double *theta, int32_t *plist, int32_t *pcnt) {
+__global__ __launch_bounds__(BD_THREADS) void k_bounds_full(SideIndex Qi, SideIndex Ri, int d, int k,
+                                                            double *theta, int32_t *plist, int32_t *pcnt) {
   __shared__ double dab[KNNJ_MAX_REPS];
   __shared__ u64 keys[KNNJ_MAX_REPS];
   __shared__ int32_t pre[KNNJ_MAX_REPS];
@@ -47,15 +54,8 @@ __global__ __launch_bounds__(BD_THREADS) void k_bounds(SideIndex Qi, SideIndex R
   __shared__ int s_T, s_cnt;
 
   const int A = blockIdx.x;
+  if (pcnt[A] >= 0) return;  // done by k_bounds
   const int mR = Ri.m;
-  const int na = Qi.off[A + 1] - Qi.off[A];
-  if (na == 0) {
-    if (threadIdx.x == 0) {
-      pcnt[A] = 0;
-      theta[A] = 0.0;
-    }
-    return;
-  }
   const double rA = Qi.rad[A];
   const float *ra = Qi.reps + (int64_t)A * Qi.dpad;
   int P = 1;
@@ -168,7 +168,146 @@ __global__ __launch_bounds__(BD_THREADS) void k_bounds(SideIndex Qi, SideIndex R
   }
 }
 
+#define BD_SEL 256  // clusters gathered one at a time before the fallback
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float bd_f32x4;
+
+__device__ __forceinline__ u64 wave_min_u64(u64 v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const u64 o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(BD_THREADS, 2) void k_bounds(SideIndex Qi, SideIndex Ri, int d, int k, double *theta,
+                                                          int32_t *plist, int32_t *pcnt) {
+  __shared__ double dab[KNNJ_MAX_REPS];
+  __shared__ u64 keys[KNNJ_MAX_REPS];  // cluster keys, then candidate bounds, then the surviving list
+  __shared__ u64 wmin[BD_THREADS / 64];
+  __shared__ int32_t sel[BD_SEL], spre[BD_SEL];
+  __shared__ int s_cnt;
+  const int A = blockIdx.x, mR = Ri.m, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int na = Qi.off[A + 1] - Qi.off[A];
+  if (na == 0) {
+    if (tid == 0) {
+      pcnt[A] = 0;
+      theta[A] = 0.0;
+    }
+    return;
+  }
+  const double rA = Qi.rad[A];
+
+  // 1. d(a, b) in binary64: 4 lanes per representative, 16-byte loads, the
+  // lanes' partial sums added at the end (rows are dpad = roundup(d, 4) floats)
+  {
+    const bd_f32x4 *ra4 = (const bd_f32x4 *)(Qi.reps + (int64_t)A * Qi.dpad);
+    const int c = tid & 3, dp4 = Qi.dpad >> 2;
+    for (int b = tid >> 2; b < mR; b += BD_THREADS / 4) {
+      const bd_f32x4 *rb4 = (const bd_f32x4 *)(Ri.reps + (int64_t)b * Ri.dpad);
+      double acc = 0.0;
+      for (int j = c; j < dp4; j += 4) {
+        const bd_f32x4 x = ra4[j], y = rb4[j];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const double t = (double)x[e] - (double)y[e];
+          if (4 * j + e < d) acc = __builtin_fma(t, t, acc);
+        }
+      }
+      acc += __shfl_xor(acc, 1);
+      acc += __shfl_xor(acc, 2);
+      if (c == 0) {
+        const int o = Ri.off[b], nb = Ri.off[b + 1] - o;
+        const double dd = sqrt(acc);
+        dab[b] = nb > 0 ? dd : INFINITY;
+        keys[b] = nb > 0 ? (((u64)__float_as_uint((float)(rA + dd + (double)Ri.pdist[o])) << 32) | (u32)b) : KEY_MAX;
+      }
+    }
+  }
+  __syncthreads();
+
+  // 2. clusters by smallest r_A + d(a,b) + min_s d(s,b), one per round (thread
+  // b mod BD_THREADS owns b), until they hold 2k points
+  u64 lmin = KEY_MAX;
+  for (int b = tid; b < mR; b += BD_THREADS) lmin = lmin < keys[b] ? lmin : keys[b];
+  int T = 0, C = 0;
+  for (;;) {
+    const u64 w = wave_min_u64(lmin);
+    if (lane == 0) wmin[wid] = w;
+    __syncthreads();
+    u64 best = wmin[0];
+#pragma unroll
+    for (int x = 1; x < BD_THREADS / 64; ++x) best = wmin[x] < best ? wmin[x] : best;
+    __syncthreads();
+    if (best == KEY_MAX) break;  // every cluster taken
+    const int b = (int)(u32)best;
+    C += min(k, Ri.off[b + 1] - Ri.off[b]);
+    if (tid == 0) {
+      sel[T] = b;
+      spre[T] = C;
+    }
+    ++T;
+    if ((b & (BD_THREADS - 1)) == tid) {
+      keys[b] = KEY_MAX;
+      lmin = KEY_MAX;
+      for (int bb = tid; bb < mR; bb += BD_THREADS) lmin = lmin < keys[bb] ? lmin : keys[bb];
+    }
+    if (C >= 2 * k || T == BD_SEL) break;
+  }
+  if (C < k) {  // too many tiny clusters: k_bounds_full takes this one
+    if (tid == 0) pcnt[A] = -1;
+    return;
+  }
+  __syncthreads();
+
+  // 3. the k-th smallest of the C candidate upper bounds
+  int P2 = 1;
+  while (P2 < C) P2 <<= 1;
+  for (int f = tid; f < P2; f += BD_THREADS) {
+    u64 v = KEY_MAX;
+    if (f < C) {
+      int t = 0;
+      while (spre[t] <= f) ++t;  // T <= BD_SEL entries
+      const int b = sel[t];
+      const int j = f - (t > 0 ? spre[t - 1] : 0);
+      const double ub = rA + dab[b] + (double)Ri.pdist[Ri.off[b] + j] * (1.0 + 0x1p-22);
+      v = (u64)__double_as_longlong(ub);
+    }
+    keys[f] = v;
+  }
+  __syncthreads();
+  block_sort_u64(keys, P2);
+  const double th = __longlong_as_double((long long)keys[k - 1]) * (1.0 + 1e-9);
+  if (tid == 0) s_cnt = 0;
+  __syncthreads();
+
+  // 4. the surviving clusters, compacted, then sorted by d(a,b)
+  for (int b = tid; b < mR; b += BD_THREADS) {
+    if (Ri.off[b + 1] > Ri.off[b]) {
+      const double dd = dab[b], rb = Ri.rad[b];
+      const double lb = dd - rA - rb;
+      if (!(lb - th > 1e-12 * (dd + rA + rb + th))) {
+        const int pos = atomicAdd(&s_cnt, 1);
+        keys[pos] = ((u64)__float_as_uint((float)dd) << 32) | (u32)b;
+      }
+    }
+  }
+  __syncthreads();
+  const int cnt = s_cnt;
+  int P3 = 1;
+  while (P3 < cnt) P3 <<= 1;
+  for (int f = cnt + tid; f < P3; f += BD_THREADS) keys[f] = KEY_MAX;
+  __syncthreads();
+  block_sort_u64(keys, P3);
+  for (int j = tid; j < cnt; j += BD_THREADS) plist[(int64_t)A * mR + j] = (int32_t)(u32)keys[j];
+  if (tid == 0) {
+    pcnt[A] = cnt;
+    theta[A] = th;
+  }
+}
+
 void launch_bounds(const SideIndex &Qi, const SideIndex &Ri, int d, int k, double *theta, int32_t *plist,
                    int32_t *pcnt, hipStream_t s) {
   hipLaunchKernelGGL(k_bounds, dim3(Qi.m), dim3(BD_THREADS), 0, s, Qi, Ri, d, k, theta, plist, pcnt);
+  hipLaunchKernelGGL(k_bounds_full, dim3(Qi.m), dim3(BD_THREADS), 0, s, Qi, Ri, d, k, theta, plist, pcnt);
 }

@@ -232,3 +232,16 @@ def test_autotune_gives_the_same_answer(ctx, d, k):
     assert np.array_equal(a_idx, b_idx) and np.array_equal(a_dist, b_dist)
     assert_matches_oracle(R, Q, k, a_idx, a_dist)
     tctx.close()
+
+
+@pytest.mark.parametrize("d", [3, 64])
+def test_many_tiny_reference_clusters(ctx, d):
+    """About 1.5 rows per reference cluster and k = 500: gathering k bound
+    candidates needs several hundred clusters, more than the one-at-a-time
+    pass takes, so the full-sort bounds pass answers these query clusters."""
+    mix = datagen.mixture64_model(seed=d + 90, d=d, comps=20)
+    R = mix.sample(datagen.rng(d + 91), 3000)
+    Q = mix.sample(datagen.rng(d + 92), 300)
+    idx, dist, st = ctx.join(R, Q, 500, m_r=2000)
+    assert st["m_r"] == 2000
+    assert_matches_oracle(R, Q, 500, idx, dist)
