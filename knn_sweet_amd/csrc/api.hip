@@ -327,6 +327,7 @@ struct Inst {
   float *nr = nullptr, *reps32 = nullptr, *rn32 = nullptr;
   double *theta = nullptr;
   int32_t *plist = nullptr, *pcnt = nullptr;
+  float *plb = nullptr;
   Tile *tiles = nullptr, *ovf_tiles = nullptr, *ovf2_tiles = nullptr;
   int32_t *counters = nullptr;  // [0] slots, [1] overflow, [2] overflow 2, [3] max err bits, [4] overflow 3
   u64 *evals = nullptr;         // [0] evaluations, [1] merges, [2] appended
@@ -518,6 +519,7 @@ knnj_status alloc_inst(Alloc &A, Inst &I, Tile *tiles) {
   }
   TRY(A.get("theta", (size_t)I.m_q, &I.theta));
   TRY(A.get("plist", (size_t)I.m_q * I.m_r, &I.plist));
+  TRY(A.get("plb", (size_t)I.m_q * I.m_r, &I.plb));
   TRY(A.get("pcnt", (size_t)I.m_q, &I.pcnt));
   if (tiles)
     I.tiles = tiles;
@@ -610,7 +612,7 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
     HIPCHK(hipGetLastError());
   }
   if (ev_index) HIPCHK(hipEventRecord(ev_index, s));
-  launch_bounds(I.Qi, I.Ri, I.d, I.k, I.theta, I.plist, I.pcnt, s);
+  launch_bounds(I.Qi, I.Ri, I.d, I.k, I.theta, I.plist, I.plb, I.pcnt, s);
   HIPCHK(hipGetLastError());
   if (ev_bounds) HIPCHK(hipEventRecord(ev_bounds, s));
   launch_zero(I.counters, 8 * sizeof(int32_t), s);
@@ -633,6 +635,8 @@ void fill_args(Inst &I) {
   a.waves = I.waves;
   a.split = I.split;
   a.lockstep = std::getenv("KNNJ_NOLOCK") ? 0 : 1;
+  a.seed_blocks = 0;
+  if (const char *sb = std::getenv("KNNJ_SEED_BLOCKS")) a.seed_blocks = std::atoi(sb);
   a.exact = I.kernel_exact();
   const double delta = I.exact ? 0.0 : (double)(d + 4) * 0x1p-23 + 0x1p-40;
   a.delta = delta;
@@ -641,6 +645,7 @@ void fill_args(Inst &I) {
   a.ntiles = I.counters;
   a.theta = I.theta;
   a.plist = I.plist;
+  a.plb = I.plb;
   a.pcnt = I.pcnt;
   a.out_idx = I.out_idx;
   a.out_dist = I.out_dist;
@@ -762,18 +767,28 @@ void plan_stats(const Inst &I, knnj_stats &st) {
 // Diagnostic print of the per-tile record (KNNJ_TILE_PROFILE set).
 knnj_status print_tile_profile(const Inst &I, int32_t nslots, int32_t novf) {
   if (nslots <= 0) return KNNJ_OK;
-  std::vector<u64> tp((size_t)nslots * 4);
+  std::vector<u64> tp((size_t)nslots * 8);
   HIPCHK(hipMemcpy(tp.data(), I.args.tile_prof, tp.size() * sizeof(u64), hipMemcpyDeviceToHost));
   std::vector<double> dur;
-  double listed = 0, visited = 0;
+  double listed = 0, visited = 0, seedb = 0, sweepb = 0;
+  double ph[7] = {0, 0, 0, 0, 0, 0, 0};  // ticks: seed, sweep, final merges, emit, seed search; groups; merge ticks
   u64 t0 = ~0ull, t1 = 0;
   for (int i = 0; i < nslots; ++i) {
-    if (tp[4 * i + 1] == 0) continue;  // empty slot
-    dur.push_back((double)(tp[4 * i + 1] - tp[4 * i]));
-    listed += (double)(tp[4 * i + 3] & 0xFFFFFFFFull);
-    visited += (double)(tp[4 * i + 3] >> 32);
-    t0 = std::min(t0, tp[4 * i]);
-    t1 = std::max(t1, tp[4 * i + 1]);
+    if (tp[8 * i + 1] == 0) continue;  // empty slot
+    dur.push_back((double)(tp[8 * i + 1] - tp[8 * i]));
+    listed += (double)(tp[8 * i + 3] & 0xFFFF);
+    sweepb += (double)((tp[8 * i + 3] >> 16) & 0xFFFF);
+    seedb += (double)((tp[8 * i + 3] >> 32) & 0xFFFF);
+    visited += (double)(tp[8 * i + 3] >> 48);
+    ph[0] += tp[8 * i + 4] ? (double)(tp[8 * i + 4] - tp[8 * i]) : 0.0;
+    ph[4] += tp[8 * i + 7] ? (double)(tp[8 * i + 7] - tp[8 * i]) : 0.0;
+    ph[5] += (double)(tp[8 * i + 2] >> 32);
+    ph[6] += (double)(tp[8 * i + 2] & 0xFFFFFFFFull);
+    ph[1] += tp[8 * i + 5] ? (double)(tp[8 * i + 5] - (tp[8 * i + 4] ? tp[8 * i + 4] : tp[8 * i])) : 0.0;
+    ph[2] += tp[8 * i + 6] ? (double)(tp[8 * i + 6] - tp[8 * i + 5]) : 0.0;
+    ph[3] += tp[8 * i + 6] ? (double)(tp[8 * i + 1] - tp[8 * i + 6]) : 0.0;
+    t0 = std::min(t0, tp[8 * i]);
+    t1 = std::max(t1, tp[8 * i + 1]);
   }
   if (const char *dump = std::getenv("KNNJ_TILE_DUMP")) {
     if (FILE *f = fopen(dump, "wb")) {
@@ -788,9 +803,14 @@ knnj_status print_tile_profile(const Inst &I, int32_t nslots, int32_t novf) {
   if (nt > 0)
     fprintf(stderr,
             "[knnj tile profile] tiles=%d span=%.0f ticks sum=%.0f mean=%.1f p50=%.0f p99=%.0f max=%.0f "
-            "listed/tile=%.1f visited/tile=%.1f\n",
+            "listed/tile=%.1f seed-blocks/tile=%.1f sweep-blocks/tile=%.1f visited/tile=%.1f\n",
             nt, (double)(t1 - t0), sum, sum / nt, dur[nt / 2], dur[(size_t)(0.99 * (nt - 1))], dur.back(),
-            listed / nt, visited / nt);
+            listed / nt, seedb / nt, sweepb / nt, visited / nt);
+  if (nt > 0 && ph[0] + ph[1] > 0)
+    fprintf(stderr,
+            "[knnj tile profile] phase ticks per tile: seed=%.0f (search %.0f) sweep=%.0f merges=%.0f emit=%.0f; "
+            "B groups %.1f, ticks in visit merges %.0f\n",
+            ph[0] / nt, ph[4] / nt, ph[1] / nt, ph[2] / nt, ph[3] / nt, ph[5] / nt, ph[6] / nt);
   if (novf > 0 && I.args.ovf_diag) {
     float dg[256];
     HIPCHK(hipMemcpy(dg, I.args.ovf_diag, sizeof(dg), hipMemcpyDeviceToHost));
@@ -830,7 +850,7 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
   if (tile_prof) {
     u64 *tp;
     float *dg;
-    TRY(ws(c, "tile_prof", (size_t)I.max_slots * 4, &tp));
+    TRY(ws(c, "tile_prof", (size_t)I.max_slots * 8, &tp));
     TRY(ws(c, "ovf_diag", 256, &dg));
     HIPCHK(hipMemsetAsync(tp, 0, sizeof(u64) * 4 * I.max_slots, s));
     HIPCHK(hipMemsetAsync(dg, 0, sizeof(float) * 256, s));

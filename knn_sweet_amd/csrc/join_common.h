@@ -19,7 +19,9 @@
 //  - otherwise: entries with key distance <= cutoff; an overflow is a last
 //    list entry within the cutoff.
 // An overflowed query is appended to the overflow list and nothing is written.
-template <int KCAP, bool EXACT>
+// NV > 0: rows have at most NV 16-byte steps (compile time), so the rerank
+// is straight-line code whose row loads all issue before the arithmetic.
+template <int KCAP, bool EXACT, int NV = 0>
 __device__ __forceinline__ void emit_query(const JoinArgs &a, const u64 *L, int qpos, int cluster,
                                            float cutoff, const float *qrow, float theta = 0.f,
                                            float evl = INFINITY) {
@@ -70,18 +72,69 @@ __device__ __forceinline__ void emit_query(const JoinArgs &a, const u64 *L, int 
   u32 rs[E];
   float err = 0.f;
   const int d = a.d, dpR = a.R.dpad;
+  // The E entries of this lane are re-scored together: 16-byte row loads
+  // (rows are dpad = roundup(d, 4) floats, 16-byte aligned) and E independent
+  // binary64 chains, each in the oracle's order (sq64_pair: t_j = (q_j -
+  // r_j)^2, s = s + t_j for j = 0 .. d-1, no contraction).  A lane without an
+  // entry reads the query row itself (one broadcast line) and discards it.
+  typedef __attribute__((__vector_size__(4 * sizeof(float)))) float ef32x4;
+  bool okv[E];
+  int pv[E];
+  const ef32x4 *r4[E];
+  double s64v[E];
 #pragma unroll
   for (int u = 0; u < E; ++u) {
     const u64 key = L[lane + 64 * u];
-    const bool ok = key != KEY_MAX && (a.ecode ? key_lower(key, a.ec) <= thr_l : key_dist(key) <= cutoff);
+    okv[u] = key != KEY_MAX && (a.ecode ? key_lower(key, a.ec) <= thr_l : key_dist(key) <= cutoff);
+    pv[u] = okv[u] ? (int)(a.ecode ? key_row_of(key, a.ec) : key_row(key)) : 0;
+    r4[u] = okv[u] ? (const ef32x4 *)(a.R.px + (int64_t)pv[u] * dpR) : (const ef32x4 *)qrow;
+    s64v[u] = 0.0;
+  }
+  {
+#pragma clang fp contract(off)
+    const ef32x4 *q4 = (const ef32x4 *)qrow;
+    const int nv = (d + 3) >> 2;
+    auto step = [&](const ef32x4 qv, const ef32x4 (&rv)[E], int t) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (4 * t + c < d) {
+#pragma unroll
+          for (int u = 0; u < E; ++u) {
+            const double x = (double)qv[c] - (double)rv[u][c];
+            s64v[u] = s64v[u] + x * x;
+          }
+        }
+      }
+    };
+    if constexpr (NV > 0) {
+#pragma unroll
+      for (int t = 0; t < NV; ++t) {
+        if (t < nv) {
+          ef32x4 rv[E];
+#pragma unroll
+          for (int u = 0; u < E; ++u) rv[u] = r4[u][t];
+          step(q4[t], rv, t);
+        }
+      }
+    } else {
+#pragma unroll 4
+      for (int t = 0; t < nv; ++t) {
+        ef32x4 rv[E];
+#pragma unroll
+        for (int u = 0; u < E; ++u) rv[u] = r4[u][t];
+        step(q4[t], rv, t);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
     hs[u] = KEY_MAX;
     rs[u] = 0xFFFFFFFFu;
-    if (ok) {
-      const int p = (int)(a.ecode ? key_row_of(key, a.ec) : key_row(key));
-      const double s64 = sq64_pair(qrow, a.R.px + (int64_t)p * dpR, d);
-      hs[u] = (u64)__double_as_longlong(s64);
-      rs[u] = (u32)a.R.perm[p];
-      err = fmaxf(err, (float)fabs((double)key_dist(key) - s64));
+    if (okv[u]) {
+      const u64 key = L[lane + 64 * u];
+      hs[u] = (u64)__double_as_longlong(s64v[u]);
+      rs[u] = (u32)a.R.perm[pv[u]];
+      err = fmaxf(err, (float)fabs((double)key_dist(key) - s64v[u]));
     }
   }
   wave_sort_pair<E>(hs, rs);

@@ -10,7 +10,13 @@
 //     distinct points bound the k-th nearest distance, so a subset gives a
 //     valid bound; the reference keeps a K-array that drops tied entries);
 //  3. keep b unless d(a,b) - r_A - r_b > theta_A (both radii; the reference
-//     never computes r_b), list sorted by d(a,b) ascending.
+//     never computes r_b).  The list is sorted by lb_b = max(d(a,b) - r_b, 0),
+//     a lower bound of d(a, s) for every s in b, stored beside it (plb,
+//     binary32 rounded down): a query q of A has d(q, s) >= lb_b - d(q, a), so
+//     a join that walks the list in order stops at the first lb_b that exceeds
+//     every query's d(q, a) + search radius.  (Sorted by d(a,b) instead, the
+//     walk could not stop before the list's end: measured 2.4x more screened
+//     clusters on the d=128 mixture.)
 // All bound arithmetic is binary64; reference-member distances stored as
 // binary32 are scaled up by (1 + 2^-22) before entering an upper bound.
 //
@@ -45,8 +51,23 @@ __device__ __forceinline__ void block_sort_u64(u64 *a, int p2) {
   }
 }
 
+// lb_b = max(d(a,b) - r_b, 0) as binary32, rounded below the binary64 value
+__device__ __forceinline__ float list_lb(double dd, double rb) {
+  const double lb = dd - rb;
+  return lb > 0.0 ? (float)(lb * (1.0 - 0x1p-20)) : 0.f;
+}
+
+// Writes the sorted list (keys[j] = (lb_b bits, b), j < cnt) of query cluster A.
+__device__ __forceinline__ void write_list(const u64 *keys, int cnt, int mR, int A, int32_t *plist, float *plb) {
+  for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+    plist[(int64_t)A * mR + j] = (int32_t)(u32)keys[j];
+    plb[(int64_t)A * mR + j] = __uint_as_float((u32)(keys[j] >> 32));
+  }
+}
+
 __global__ __launch_bounds__(BD_THREADS) void k_bounds_full(SideIndex Qi, SideIndex Ri, int d, int k,
-                                                            double *theta, int32_t *plist, int32_t *pcnt) {
+                                                            double *theta, int32_t *plist, float *plb,
+                                                            int32_t *pcnt) {
   __shared__ double dab[KNNJ_MAX_REPS];
   __shared__ u64 keys[KNNJ_MAX_REPS];
   __shared__ int32_t pre[KNNJ_MAX_REPS];
@@ -151,7 +172,7 @@ __global__ __launch_bounds__(BD_THREADS) void k_bounds_full(SideIndex Qi, SideIn
       const double dd = dab[b], rb = Ri.rad[b];
       const double lb = dd - rA - rb;
       if (!(lb - th > 1e-12 * (dd + rA + rb + th))) {
-        key = ((u64)__float_as_uint((float)dd) << 32) | (u32)b;
+        key = ((u64)__float_as_uint(list_lb(dd, rb)) << 32) | (u32)b;
         ++mine;
       }
     }
@@ -161,7 +182,7 @@ __global__ __launch_bounds__(BD_THREADS) void k_bounds_full(SideIndex Qi, SideIn
   __syncthreads();
   block_sort_u64(keys, P);
   const int cnt = s_cnt;
-  for (int j = threadIdx.x; j < cnt; j += blockDim.x) plist[(int64_t)A * mR + j] = (int32_t)(u32)keys[j];
+  write_list(keys, cnt, mR, A, plist, plb);
   if (threadIdx.x == 0) {
     pcnt[A] = cnt;
     theta[A] = th;
@@ -181,7 +202,7 @@ __device__ __forceinline__ u64 wave_min_u64(u64 v) {
 }
 
 __global__ __launch_bounds__(BD_THREADS, 2) void k_bounds(SideIndex Qi, SideIndex Ri, int d, int k, double *theta,
-                                                          int32_t *plist, int32_t *pcnt) {
+                                                          int32_t *plist, float *plb, int32_t *pcnt) {
   __shared__ double dab[KNNJ_MAX_REPS];
   __shared__ u64 keys[KNNJ_MAX_REPS];  // cluster keys, then candidate bounds, then the surviving list
   __shared__ u64 wmin[BD_THREADS / 64];
@@ -281,14 +302,14 @@ __global__ __launch_bounds__(BD_THREADS, 2) void k_bounds(SideIndex Qi, SideInde
   if (tid == 0) s_cnt = 0;
   __syncthreads();
 
-  // 4. the surviving clusters, compacted, then sorted by d(a,b)
+  // 4. the surviving clusters, compacted, then sorted by lb_b
   for (int b = tid; b < mR; b += BD_THREADS) {
     if (Ri.off[b + 1] > Ri.off[b]) {
       const double dd = dab[b], rb = Ri.rad[b];
       const double lb = dd - rA - rb;
       if (!(lb - th > 1e-12 * (dd + rA + rb + th))) {
         const int pos = atomicAdd(&s_cnt, 1);
-        keys[pos] = ((u64)__float_as_uint((float)dd) << 32) | (u32)b;
+        keys[pos] = ((u64)__float_as_uint(list_lb(dd, rb)) << 32) | (u32)b;
       }
     }
   }
@@ -299,7 +320,7 @@ __global__ __launch_bounds__(BD_THREADS, 2) void k_bounds(SideIndex Qi, SideInde
   for (int f = cnt + tid; f < P3; f += BD_THREADS) keys[f] = KEY_MAX;
   __syncthreads();
   block_sort_u64(keys, P3);
-  for (int j = tid; j < cnt; j += BD_THREADS) plist[(int64_t)A * mR + j] = (int32_t)(u32)keys[j];
+  write_list(keys, cnt, mR, A, plist, plb);
   if (tid == 0) {
     pcnt[A] = cnt;
     theta[A] = th;
@@ -307,7 +328,7 @@ __global__ __launch_bounds__(BD_THREADS, 2) void k_bounds(SideIndex Qi, SideInde
 }
 
 void launch_bounds(const SideIndex &Qi, const SideIndex &Ri, int d, int k, double *theta, int32_t *plist,
-                   int32_t *pcnt, hipStream_t s) {
-  hipLaunchKernelGGL(k_bounds, dim3(Qi.m), dim3(BD_THREADS), 0, s, Qi, Ri, d, k, theta, plist, pcnt);
-  hipLaunchKernelGGL(k_bounds_full, dim3(Qi.m), dim3(BD_THREADS), 0, s, Qi, Ri, d, k, theta, plist, pcnt);
+                   float *plb, int32_t *pcnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_bounds, dim3(Qi.m), dim3(BD_THREADS), 0, s, Qi, Ri, d, k, theta, plist, plb, pcnt);
+  hipLaunchKernelGGL(k_bounds_full, dim3(Qi.m), dim3(BD_THREADS), 0, s, Qi, Ri, d, k, theta, plist, plb, pcnt);
 }

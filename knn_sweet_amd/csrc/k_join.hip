@@ -38,7 +38,7 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
   u64 *tmp = bufs + TQ * KNNJ_BCAP;        // [KCAP]
   const int lane = threadIdx.x;
   const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
-  u64 nvisit = 0;
+  u64 nvisit = 0, nsweepb = 0;  // diagnostics: visits, screened blocks
   const int A = tile.cluster;
   const int nq = tile.count;
   const int d = a.d, dpR = a.R.dpad, dpQ = a.Q.dpad, k = a.k;
@@ -82,7 +82,21 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
 
   const int np = a.pcnt[A];
   const int32_t *pl = a.plist + (int64_t)A * a.R.m;
+  const float *pll = a.plb + (int64_t)A * a.R.m;
+  // d(q, a) of each query: with the list ascending by lb_b <= d(a, s), every
+  // s of a later cluster has d(q, s) >= lb_b - d(q, a)
+  float dqa[TQ];
+#pragma unroll
+  for (int i = 0; i < TQ; ++i) dqa[i] = i < nq ? a.Q.pdist[tile.start + i] * (1.f + 2.f * eps) : 0.f;
   for (int pb = 0; pb < np; pb += 64) {
+    ++nsweepb;
+    {
+      float reach = 0.f;
+#pragma unroll
+      for (int i = 0; i < TQ; ++i)
+        if (i < nq) reach = fmaxf(reach, sqrtf(key_dist(thr[i])) * (1.f + eps) + dqa[i]);
+      if (pll[pb] * (1.f - 2.f * eps) > reach) break;  // this and every later cluster is out of reach
+    }
     // Screening: lane j takes listed cluster pb + j, computes the distance of
     // every query of the tile to its representative, and keeps the cluster
     // if the lower bound d(q,b) - r_b does not exceed some query's threshold.
@@ -235,11 +249,12 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
                             a.Q.px + (int64_t)(tile.start + i) * dpQ);
   }
   if (lane == 0 && a.tile_prof) {
-    u64 *tp = a.tile_prof + 4 * (size_t)blockIdx.x;
+    u64 *tp = a.tile_prof + 8 * (size_t)blockIdx.x;
     tp[0] = clk0;
     tp[1] = wall_clock64();
     tp[2] = 0;
-    tp[3] = (nvisit << 32) | (u64)np;
+    tp[4] = tp[5] = tp[6] = tp[7] = 0;
+    tp[3] = (min(nvisit, (u64)0xFFFF) << 48) | (min(nsweepb, (u64)0xFFFF) << 16) | (u64)min(np, 0xFFFF);
   }
   if (lane == 0) {
     atomicAdd(a.n_evals, evals);

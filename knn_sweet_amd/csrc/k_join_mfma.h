@@ -131,7 +131,8 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   float *qsh = dqs + 256;
   const int lane = threadIdx.x & 63, g = lane >> 4, c16 = lane & 15;
   const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
-  u64 nvisit = 0;
+  u64 nvisit = 0, nseedb = 0, nsweepb = 0;  // diagnostics: visits, screened blocks
+  u64 ngroups = 0, mticks = 0;              // diagnostics: B groups computed, ticks in sweep merges
   // Split mode (a.split > 1, W > 1): every wave serves the same (up to) 16
   // queries and takes every W-th block of 16 listed clusters, lowering its
   // thresholds from the others' published k-th keys (each is an upper bound
@@ -258,40 +259,86 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
 
   const int np = a.pcnt[A];
   const int32_t *pl = a.plist + (int64_t)A * a.R.m;
-  // q.c for this lane's four rows and the representative of lane c16's
-  // cluster bj (columns), on v_mfma_f32_16x16x4_f32
-  auto center_dots = [&](const int bj) {
-    f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+  // The list ascends by lb_b <= d(a, s) for every s of cluster b, so a row
+  // with d(q, a) has d(q, s) >= lb_b - d(q, a) there: once lb_b exceeds every
+  // live row's d(q, a) + search radius, no later cluster can hold a candidate.
+  const float *pll = a.plb + (int64_t)A * a.R.m;
+  float dqa[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) dqa[i] = live[i] ? a.Q.pdist[q0 + 4 * g + i] * (1.f + 2.f * eps) : 0.f;
+  // wave-uniform: true when listed cluster pb and all after it are out of reach
+  auto out_of_reach = [&](const int pb) {
+    float reach = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (live[i]) reach = fmaxf(reach, sqrtf(key_dist(thr[i])) * (1.f + eps) + dqa[i]);
+    reach = wave_max(reach);
+    return pll[pb] * (1.f - 2.f * eps) > reach;
+  };
+  // What the screening of one block needs from memory, for lane c16's
+  // cluster: its representative's slice (dimensions 32 s + 8 g + j), squared
+  // norm, member range and radius.  The seed search and the sweep load the
+  // next block's while the current one computes (one list entry and one
+  // representative row per lane: a dependent pair of loads whose latency a
+  // single wave per SIMD cannot otherwise hide).
+  struct Pre {
+    f32x4_t c[2 * NS];
+    float cn, Rb;
+    int s0, s1;
+  };
+  auto load_pre = [&](const int bj, Pre &P) {
     const float *cr = a.reps32 + (int64_t)bj * dk + 8 * g;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      P.c[2 * s] = *(const f32x4_t *)(cr + 32 * s);
+      P.c[2 * s + 1] = *(const f32x4_t *)(cr + 32 * s + 4);
+    }
+    P.cn = a.rn32[bj];
+    P.s0 = a.R.off[bj];
+    P.s1 = a.R.off[bj + 1];
+    P.Rb = (float)(a.R.rad[bj] * (1.0 + 1e-6));
+  };
+  // list entry pj, clamped into the list (0 for an empty list: never used)
+  auto list_at = [&](const int pj) { return np > 0 ? pl[min(pj, np - 1)] : 0; };
+  // q.c for this lane's four rows and the representative of lane c16's
+  // cluster (columns), on v_mfma_f32_16x16x4_f32; two accumulation chains
+  // (even and odd k-steps) summed at the end
+  auto center_dots = [&](const Pre &P) {
+    f32x4_t acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       float qv[8];
       q_slice(s, qv);
-      const f32x4_t c0 = *(const f32x4_t *)(cr + 32 * s);
-      const f32x4_t c1 = *(const f32x4_t *)(cr + 32 * s + 4);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[j], c0[j], acc, 0, 0, 0);
+      for (int j = 0; j < 4; j += 2) {
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[j], P.c[2 * s][j], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[j + 1], P.c[2 * s][j + 1], acc1, 0, 0, 0);
+      }
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[4 + j], c1[j], acc, 0, 0, 0);
+      for (int j = 0; j < 4; j += 2) {
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[4 + j], P.c[2 * s + 1][j], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[5 + j], P.c[2 * s + 1][j + 1], acc1, 0, 0, 0);
+      }
     }
-    return acc;
+    return f32x4_t{acc0[0] + acc1[0], acc0[1] + acc1[1], acc0[2] + acc1[2], acc0[3] + acc1[3]};
   };
 
   // Screening and visits of one block of up to 16 clusters: lane c16's
   // cluster bj, taking part if lv.  lock: the multi-wave lockstep applies.
   // cslot: the seed slot of lane c16's cluster (-1: not a seed).  The seed
   // block (seedmode) may serve any row; the sweep skips the seeds (served).
-  auto process_block = [&](const int bj, const bool lv, const bool lock, const int cslot, const bool seedmode) {
+  auto process_block = [&](const int bj, const bool lv, const bool lock, const int cslot, const bool seedmode,
+                           const Pre &P) {
       // ---- screening of 16 listed clusters: ||q - c||^2 = ||q||^2 + ||c||^2 - 2 q.c
       // on v_mfma_f32_16x16x4_f32 (rows = queries, columns = representatives),
-      // error <= gE (||q|| + ||c||)^2; a cluster is kept if d(q,c) - r_b can be
-      // <= theta_q for some query.
-      // this lane's cluster metadata, read once here and broadcast per visit
-      const int s0j = a.R.off[bj], s1j = a.R.off[bj + 1];
-      const float Rbj = (float)(a.R.rad[bj] * (1.0 + 1e-6));
+      // error <= gE (||q|| + ||c||)^2 (any summation order of the dot is
+      // covered); a cluster is kept if d(q,c) - r_b can be <= theta_q for some
+      // query.  This lane's cluster metadata (P) is broadcast per visit.
+      const int s0j = P.s0, s1j = P.s1;
+      const float Rbj = P.Rb;
       {
-        const f32x4_t acc = center_dots(bj);
-        const float cn = a.rn32[bj];
+        const f32x4_t acc = center_dots(P);
+        const float cn = P.cn;
         const float rbj = Rbj;
         bool need = false;
   #pragma unroll
@@ -440,6 +487,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
               }
             };
             auto compute_group = [&](int g0, Group &G) {
+              ++ngroups;
               if constexpr (!EXACT) {
                 // rows are sorted by distance to the representative, so the
                 // group's last row bounds ||r - c|| for all of it: a tighter
@@ -524,6 +572,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
                   constexpr int WM = MF_BC - 16;
                   const bool hot = cnt[0] >= WM || cnt[1] >= WM || cnt[2] >= WM || cnt[3] >= WM;
                   if (wave_ballot(hot) != 0) {
+                    const u64 cm0 = a.tile_prof ? wall_clock64() : 0;
                     wave_sync();
                     for (int r = 0; r < MF_TQ; ++r) {
                       const int gi = r >> 2, ii = r & 3;
@@ -533,6 +582,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
                         if (j == ii) mine = cnt[j];
                       if (__shfl(mine, 16 * gi) >= WM) merge(r);
                     }
+                    if (a.tile_prof) mticks += wall_clock64() - cm0;
                   }
                 }
               }
@@ -565,8 +615,11 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   // ---- seed: each query's nearest listed cluster (by centre distance) is
   // visited first, so thresholds are tight before the sweep; a query cluster
   // mixing several dense regions would otherwise start from its loose
-  // cluster bound and visit everything its sweep meets first.
+  // cluster bound and visit everything its sweep meets first.  The search
+  // walks the whole list, or its first seed_blocks x 16 entries when
+  // seed_blocks > 0 (measured: a capped search loses most of the seed's gain).
   int seedv = -1;  // lane c16: seed cluster of row c16, -1 if none or a duplicate
+  u64 clk_search = 0;
   {
     float bestd[4];
     int bestj[4];
@@ -575,19 +628,31 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
       bestd[i] = INFINITY;
       bestj[i] = -1;
     }
-    for (int pb = 0; pb < np; pb += 16) {
-      const int pj = pb + c16;
-      const bool lv = pj < np;
-      const int bj = lv ? pl[pj] : pl[pb];
-      const f32x4_t acc = center_dots(bj);
-      const float cn = a.rn32[bj];
+    const int np_seed = a.seed_blocks > 0 ? min(np, 16 * a.seed_blocks) : np;
+    if (np_seed > 0) {
+      int bjc = list_at(c16), bjn = list_at(16 + c16);
+      Pre cur;
+      load_pre(bjc, cur);
+      for (int pb = 0; pb < np_seed; pb += 16) {
+        ++nseedb;
+        // next block's loads in flight while this one computes
+        Pre nxt;
+        load_pre(bjn, nxt);
+        const int bjnn = list_at(pb + 32 + c16);
+        const bool lv = pb + c16 < np;
+        const f32x4_t acc = center_dots(cur);
+        const float cn = cur.cn;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float sqc = (qnr[i] + cn) - 2.f * acc[i];
-        if (lv && sqc < bestd[i]) {
-          bestd[i] = sqc;
-          bestj[i] = bj;
+        for (int i = 0; i < 4; ++i) {
+          const float sqc = (qnr[i] + cn) - 2.f * acc[i];
+          if (lv && sqc < bestd[i]) {
+            bestd[i] = sqc;
+            bestj[i] = bjc;
+          }
         }
+        bjc = bjn;
+        bjn = bjnn;
+        cur = nxt;
       }
     }
 #pragma unroll
@@ -600,6 +665,27 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
           bestj[i] = oj;
         }
       }
+    // Each row's threshold from its nearest centre c_b: the k members of b
+    // nearest to c_b lie within d(q, c_b) + pdist[off_b + k - 1] of q (rows
+    // of a cluster ascend by pdist), so that bounds the k-th neighbour
+    // distance -- usually far below the query cluster's theta_A, which cuts
+    // the candidates the seed visit appends before its list holds k keys.
+    // d(q, c_b) is taken from above through the screening's error bound.
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int b = bestj[i];
+      if (live[i] && b >= 0) {
+        const int o = a.R.off[b];
+        if (a.R.off[b + 1] - o >= k) {
+          const float pk = a.R.pdist[o + k - 1] * (1.f + 1e-6f);
+          const float nsum = sqrtf(qnr[i]) + sqrtf(a.rn32[b]);
+          const float dhi = sqrtf(fmaxf(bestd[i] + gE * nsum * nsum, 0.f)) * (1.f + eps);
+          const float ub = (dhi + pk) * (1.f + eps);
+          const u64 t = ((u64)__float_as_uint(ub * ub * (1.f + 2.f * eps)) << 32) | 0xFFFFFFFFull;
+          if (t < thr[i]) thr[i] = t;
+        }
+      }
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int v = __shfl(bestj[i], 16 * (c16 >> 2));
@@ -615,8 +701,12 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     if (first != c16) seedv = -1;
     // split mode: wave 0 visits the seeds and publishes its thresholds
     // (cluster 0 stands in for lanes without a seed: never visited)
+    clk_search = a.tile_prof ? wall_clock64() : 0;
     if ((!splitm || wv == 0) && np > 0) {
-      process_block(seedv >= 0 ? seedv : 0, seedv >= 0, false, c16, true);
+      const int sb = seedv >= 0 ? seedv : 0;
+      Pre sp;
+      load_pre(sb, sp);
+      process_block(sb, seedv >= 0, false, c16, true, sp);
       if (splitm) {
         wave_sync();
         for (int r = 0; r < MF_TQ; ++r) {
@@ -631,6 +721,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     }
     if (splitm) __syncthreads();
   }
+  const u64 clk_seed = a.tile_prof ? wall_clock64() : 0;
   // seed slot of cluster bj, -1 if it is no row's seed
   auto seed_slot = [&](const int bj) {
     int cs = -1;
@@ -639,19 +730,44 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     return cs;
   };
 
-  for (int pb = splitm ? 16 * wv : 0; pb < np; pb += splitm ? 16 * W : 16) {
-    if (splitm) {
-      // thresholds the other waves have reached
+  {
+    const int pstep = splitm ? 16 * W : 16;
+    const int pb0 = splitm ? 16 * wv : 0;
+    int bjc = list_at(pb0 + c16), bjn = list_at(pb0 + pstep + c16);
+    Pre cur;
+    if (pb0 < np) load_pre(bjc, cur);
+    for (int pb = pb0; pb < np; pb += pstep) {
+      if (splitm) {
+        // thresholds the other waves have reached
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const u64 t = __atomic_load_n(sthr + 4 * g + i, __ATOMIC_RELAXED);
-        if (t < thr[i]) thr[i] = t;
+        for (int i = 0; i < 4; ++i) {
+          const u64 t = __atomic_load_n(sthr + 4 * g + i, __ATOMIC_RELAXED);
+          if (t < thr[i]) thr[i] = t;
+        }
       }
+      // the waves of a lockstep workgroup leave together (the visits hold
+      // workgroup barriers); split-mode and single waves leave on their own
+      if constexpr (W > 1) {
+        if (a.lockstep && !splitm) {
+          if (wg_or(out_of_reach(pb) ? 0u : 1u) == 0u) break;
+        } else if (out_of_reach(pb)) {
+          break;
+        }
+      } else {
+        if (out_of_reach(pb)) break;
+      }
+      ++nsweepb;
+      // next block's loads in flight while this one is screened and visited
+      Pre nxt;
+      if (pb + pstep < np) load_pre(bjn, nxt);
+      const int bjnn = list_at(pb + 2 * pstep + c16);
+      process_block(bjc, pb + c16 < np, true, seed_slot(bjc), false, cur);
+      bjc = bjn;
+      bjn = bjnn;
+      cur = nxt;
     }
-    const int pj = pb + c16;
-    const int bj = pj < np ? pl[pj] : pl[pb];
-    process_block(bj, pj < np, true, seed_slot(bj), false);
   }
+  const u64 clk_sweep = a.tile_prof ? wall_clock64() : 0;
   wave_sync();
   for (int r = 0; r < MF_TQ; ++r) {
     const int gi = r >> 2, ii = r & 3;
@@ -691,6 +807,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   }
   // in split mode only wave 0 holds the merged lists
   const int nemit = (splitm && wv != 0) ? 0 : nq;
+  const u64 clk_merge = a.tile_prof ? wall_clock64() : 0;
 
   for (int r = 0; r < nemit; ++r) {
     const int gi = r >> 2, ii = r & 3;
@@ -707,25 +824,23 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     const float Er = __shfl(em, 16 * gi);
     const float Vr = __shfl(ev, 16 * gi);
     const float cutoff = EXACT ? Tr : (Tr + 2.f * Er) * (1.f + 2.4e-7f);
-    emit_query<KCAP, EXACT>(a, lists + r * KCAP, q0 + r, A, cutoff, a.Q.px + (int64_t)(q0 + r) * dpQ, Tr, Vr);
+    emit_query<KCAP, EXACT, DK / 4>(a, lists + r * KCAP, q0 + r, A, cutoff, a.Q.px + (int64_t)(q0 + r) * dpQ, Tr,
+                                    Vr);
   }
   if (a.tile_prof && wv == 0) {
-    // diagnostic: the largest and smallest final threshold of the tile's rows
-    float tmax = 0.f, tmin = INFINITY;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (live[i]) {
-        tmax = fmaxf(tmax, key_dist(thr[i]));
-        tmin = fminf(tmin, key_dist(thr[i]));
-      }
-    tmax = wave_max(tmax);
-    tmin = wave_min(tmin);
+    // diagnostic record: clocks at the phase boundaries, B groups and ticks
+    // spent in merges during visits, visits and screened blocks
     if (threadIdx.x == 0) {
-      u64 *tp = a.tile_prof + 4 * (size_t)blockIdx.x;
+      u64 *tp = a.tile_prof + 8 * (size_t)blockIdx.x;
       tp[0] = clk0;
       tp[1] = wall_clock64();
-      tp[2] = ((u64)__float_as_uint(tmax) << 32) | (u64)__float_as_uint(tmin);
-      tp[3] = (nvisit << 32) | (u64)np;
+      tp[4] = clk_seed;
+      tp[5] = clk_sweep;
+      tp[6] = clk_merge;
+      tp[7] = clk_search;
+      tp[2] = (ngroups << 32) | min(mticks, (u64)0xFFFFFFFFull);
+      tp[3] = (min(nvisit, (u64)0xFFFF) << 48) | (min(nseedb, (u64)0xFFFF) << 32) | (min(nsweepb, (u64)0xFFFF) << 16) |
+              (u64)min(np, 0xFFFF);
     }
   }
   if (lane == 0) {

@@ -37,8 +37,10 @@ void launch_make_tiles(const int32_t *off, int m, const double *theta, int tq, i
                        int32_t *nslots, int max_slots, int nxcd, hipStream_t s);
 
 // k_bounds.hip
+// plist[A][j]: surviving reference clusters of query cluster A, ascending by
+// plb[A][j] = max(d(a,b) - r_b, 0) rounded down (a lower bound of d(a, s), s in b)
 void launch_bounds(const SideIndex &Qi, const SideIndex &Ri, int d, int k, double *theta, int32_t *plist,
-                   int32_t *pcnt, hipStream_t s);
+                   float *plb, int32_t *pcnt, hipStream_t s);
 
 // k_join.hip
 struct JoinArgs {
@@ -53,6 +55,7 @@ struct JoinArgs {
   const int32_t *ntiles;
   const double *theta;   // [mQ]
   const int32_t *plist;  // [mQ][mR]
+  const float *plb;      // [mQ][mR] list order key: lower bound of d(a, s), s in the cluster
   const int32_t *pcnt;   // [mQ]
   int32_t *out_idx;      // [nQ][k] or IndexDist [nQ][k] when out_mode == 2
   float *out_dist;
@@ -76,6 +79,7 @@ struct JoinArgs {
   int waves;             // MFMA join: > 1 asks for the multi-wave workgroup (tiles of 16 x waves queries)
   int lockstep;          // multi-wave workgroup: 1 = waves walk the union of their cluster lists together
   int split;             // multi-wave workgroup: > 1 = its waves split one 16-query tile's cluster list
+  int seed_blocks;       // MFMA join: the seed searches the first seed_blocks x 16 listed clusters (0: all)
 };
 
 // Picks the wave tile size for a candidate capacity; returns the dynamic LDS bytes.

@@ -98,10 +98,77 @@ struct NoDrop {
 // three arrays private to the calling wave (other waves of its workgroup may
 // be elsewhere: only wave-level ordering is used).  Returns the wave-wide minimum of
 // drop(key) over the real keys that fell off the end (INFINITY if none).
+//
+// A 64-key buffer into a list of up to 256 is merged by counting: every lane reads the
+// buffer keys as LDS broadcasts and counts, for its own buffer and list keys,
+// the buffer keys below them; a buffer key's list rank is a binary search.
+// Each key's new position is then its index among the smaller keys of both
+// sequences (keys are distinct), with no cross-lane dependency chain.  Larger
+// buffers sort the buffer with the bitonic network and rank-merge.
 template <int KCAP, int BCAP = KNNJ_BCAP, class Drop = NoDrop>
 __device__ __forceinline__ float wave_merge(u64 *list, u64 *buf, int cnt, u64 *tmp, Drop drop = Drop()) {
   constexpr int EB = BCAP / 64;
   const int lane = threadIdx.x & 63;
+  if constexpr (BCAP <= 64 && KCAP <= 256) {
+    constexpr int EL = KCAP / 64;
+    u64 v[EB], y[EL];
+    int rv[EB], ry[EL];
+#pragma unroll
+    for (int u = 0; u < EB; ++u) {
+      v[u] = lane + 64 * u < cnt ? buf[lane + 64 * u] : KEY_MAX;
+      rv[u] = 0;
+    }
+#pragma unroll
+    for (int u = 0; u < EL; ++u) {
+      y[u] = list[lane + 64 * u];
+      ry[u] = 0;
+    }
+    int i = 0;
+    for (; i + 4 <= cnt; i += 4) {
+      u64 b[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) b[t] = buf[i + t];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int u = 0; u < EB; ++u) rv[u] += b[t] < v[u] ? 1 : 0;
+#pragma unroll
+        for (int u = 0; u < EL; ++u) ry[u] += b[t] < y[u] ? 1 : 0;
+      }
+    }
+    for (; i < cnt; ++i) {
+      const u64 b = buf[i];
+#pragma unroll
+      for (int u = 0; u < EB; ++u) rv[u] += b < v[u] ? 1 : 0;
+#pragma unroll
+      for (int u = 0; u < EL; ++u) ry[u] += b < y[u] ? 1 : 0;
+    }
+    float dmin = INFINITY;
+#pragma unroll
+    for (int u = 0; u < EL; ++u) {
+      const int pos = lane + 64 * u + ry[u];
+      if (pos < KCAP)
+        tmp[pos] = y[u];
+      else if (y[u] != KEY_MAX)
+        dmin = fminf(dmin, drop(y[u]));
+    }
+#pragma unroll
+    for (int u = 0; u < EB; ++u) {
+      const int pos = rv[u] + count_below<false>(list, KCAP, KCAP, v[u]);
+      if (pos < KCAP)
+        tmp[pos] = v[u];
+      else if (v[u] != KEY_MAX)
+        dmin = fminf(dmin, drop(v[u]));
+    }
+    wave_sync();
+#pragma unroll
+    for (int u = 0; u < EL; ++u) list[lane + 64 * u] = tmp[lane + 64 * u];
+    wave_sync();
+    if constexpr (Drop::tracks)
+      return wave_min(dmin);
+    else
+      return INFINITY;
+  }
   u64 v[EB];
 #pragma unroll
   for (int u = 0; u < EB; ++u) v[u] = lane + 64 * u < cnt ? buf[lane + 64 * u] : KEY_MAX;
