@@ -487,7 +487,7 @@ knnj_status alloc_side(Alloc &A, const std::string &p, int64_t n, int m, int dpa
   TRY(A.get(p + "cand", (size_t)m * trials, &b.cand));
   TRY(A.get(p + "reprows", (size_t)m, &b.rep_rows));
   b.nblk = rep_score_blocks(m);
-  TRY(A.get(p + "partial", (size_t)b.nblk * trials, &b.partial));
+  TRY(A.get(p + "partial", (size_t)b.nblk * trials + 1, &b.partial));  // + the winning trial's slot
   const int dk = (d + 31) & ~31;
   if (d >= 16 && dk <= 256) {  // the MFMA assignment's bf16 representatives
     TRY(A.get(p + "repsbf16", (size_t)m * dk, &b.rb));
@@ -1185,7 +1185,7 @@ knnj_status knnj_index_build(knnj_ctx *c, knnj_matrix R, const knnj_opts *opts, 
   OWN(A.get("ix_cand", (size_t)m * I.trials, &b.cand));
   OWN(A.get("ix_reprows", (size_t)m, &b.rep_rows));
   b.nblk = rep_score_blocks(m);
-  OWN(A.get("ix_partial", (size_t)b.nblk * I.trials, &b.partial));
+  OWN(A.get("ix_partial", (size_t)b.nblk * I.trials + 1, &b.partial));  // + the winning trial's slot
   if (I.d >= 16 && ((I.d + 31) & ~31) <= 256) {
     OWN(A.get("ix_repsbf16", (size_t)m * ((I.d + 31) & ~31), &b.rb));
     OWN(A.get("ix_repsnrm", (size_t)m, &b.rn));

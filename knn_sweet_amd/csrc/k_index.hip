@@ -207,11 +207,10 @@ __global__ __launch_bounds__(256) void k_rep_score(const float *__restrict__ X, 
 
 // Picks the best trial (first maximum, as the reference's strict compare,
 // knnjoin.cu:135-148) and gathers its rows into reps[m][dpad] (zero padded).
-__global__ void k_rep_pick(const double *__restrict__ partial, int nblk, int trials,
-                           const int32_t *__restrict__ cand, int m, const float *__restrict__ X,
-                           int64_t ld, int d, int dpad, float *reps, int32_t *rep_rows) {
+// The trial with the largest score (first on ties); its index goes to
+// partial[trials * nblk] (one slot past the partial sums).
+__global__ void k_rep_pick(double *__restrict__ partial, int nblk, int trials) {
   __shared__ double tot[64];
-  __shared__ int best;
   // one wave per trial slot: lane-strided sums, then a fixed shuffle tree
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int t = w; t < trials; t += blockDim.x / 64) {
@@ -229,11 +228,18 @@ __global__ void k_rep_pick(const double *__restrict__ partial, int nblk, int tri
         bs = tot[t];
         bt = t;
       }
-    best = bt;
+    *(int32_t *)(partial + (int64_t)trials * nblk) = bt;
   }
-  __syncthreads();
+}
+
+// Gathers the winning trial's rows into reps[m][dpad] (zero padded).
+__global__ void k_rep_gather(const double *__restrict__ partial, int nblk, int trials,
+                             const int32_t *__restrict__ cand, int m, const float *__restrict__ X, int64_t ld,
+                             int d, int dpad, float *reps, int32_t *rep_rows) {
+  const int best = *(const int32_t *)(partial + (int64_t)trials * nblk);
   const int32_t *c = cand + (int64_t)best * m;
-  for (int64_t e = threadIdx.x; e < (int64_t)m * dpad; e += blockDim.x) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)m * dpad;
+       e += (int64_t)gridDim.x * blockDim.x) {
     const int r = (int)(e / dpad), q = (int)(e - (int64_t)r * dpad);
     reps[e] = q < d ? X[(int64_t)c[r] * ld + q] : 0.f;
     if (q == 0 && rep_rows) rep_rows[r] = c[r];
@@ -249,7 +255,10 @@ void launch_rep_select(const float *X, int64_t ld, int d, int dpad, const int32_
                        int trials, double *partial, int nblk, float *reps, int32_t *rep_rows,
                        hipStream_t s) {
   hipLaunchKernelGGL(k_rep_score, dim3(nblk, trials), dim3(256), 0, s, X, ld, d, cand, m, partial);
-  hipLaunchKernelGGL(k_rep_pick, dim3(1), dim3(1024), 0, s, partial, nblk, trials, cand, m, X, ld, d, dpad, reps,
+  hipLaunchKernelGGL(k_rep_pick, dim3(1), dim3(1024), 0, s, partial, nblk, trials);
+  const int64_t tot = (int64_t)m * dpad;
+  const unsigned gb = (unsigned)std::min<int64_t>(1024, (tot + 255) / 256);
+  hipLaunchKernelGGL(k_rep_gather, dim3(gb), dim3(256), 0, s, partial, nblk, trials, cand, m, X, ld, d, dpad, reps,
                      rep_rows);
 }
 
@@ -349,7 +358,9 @@ template <int DK>
 __global__ __launch_bounds__(256, 2) void k_assign_mfma(const float *__restrict__ X, int64_t n, int64_t ld, int d,
                                                         const float *__restrict__ c0, const __bf16 *__restrict__ rb,
                                                         const float *__restrict__ rn, int m, int dk, int32_t *cl) {
-  constexpr int NS = DK / 32, AB = DK <= 128 ? 4 : 2, CH = 64, RS = DK + 8, PPR = DK / 8;
+  // CH representatives per LDS chunk: 128 for DK <= 128, so a chunk's MFMA
+  // work covers the global loads of the next chunk (64 measured stalls)
+  constexpr int NS = DK / 32, AB = DK <= 128 ? 4 : 2, CH = DK <= 128 ? 128 : 64, RS = DK + 8, PPR = DK / 8;
   constexpr int PT = CH * PPR / 256;
   __shared__ __attribute__((aligned(16))) __bf16 sb[2][CH * RS];
   __shared__ float sn[2][CH];
@@ -429,27 +440,40 @@ __global__ __launch_bounds__(256, 2) void k_assign_mfma(const float *__restrict_
     const int buf = ch & 1;
     if (ch + 1 < nch) gload((ch + 1) * CH);
     if (live) {
-#pragma unroll 1
-      for (int blk = 0; blk < CH / 16; ++blk) {
+      // the chunk's four 16-representative blocks, unrolled: the next block's
+      // LDS reads are issued before this block's MFMAs, and the AB
+      // accumulation chains are interleaved step by step
+      as_bf16x8 bv[2][NS];
+      float nrc[2];
+      auto lds_b = [&](int blk, int slot) {
         const int rl = blk * 16 + c16;
-        as_bf16x8 bv[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) bv[s] = *(const as_bf16x8 *)&sb[buf][rl * RS + 32 * s + 8 * g];
-        const float nrc = sn[buf][rl];
+        for (int s = 0; s < NS; ++s) bv[slot][s] = *(const as_bf16x8 *)&sb[buf][rl * RS + 32 * s + 8 * g];
+        nrc[slot] = sn[buf][rl];
+      };
+      lds_b(0, 0);
+#pragma unroll
+      for (int blk = 0; blk < CH / 16; ++blk) {
+        const int cur = blk & 1;
+        if (blk + 1 < CH / 16) lds_b(blk + 1, cur ^ 1);
         const int bno = __builtin_amdgcn_readfirstlane(ch * (CH / 16) + blk);
+        as_f32x4 c[AB];
 #pragma unroll
-        for (int ab = 0; ab < AB; ++ab) {
-          as_f32x4 c = cinit[ab];
+        for (int ab = 0; ab < AB; ++ab) c[ab] = cinit[ab];
 #pragma unroll
-          for (int s = 0; s < NS; ++s) c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ab][s], bv[s], c, 0, 0, 0);
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+          for (int ab = 0; ab < AB; ++ab)
+            c[ab] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ab][s], bv[cur][s], c[ab], 0, 0, 0);
+#pragma unroll
+        for (int ab = 0; ab < AB; ++ab)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const float dd = __builtin_fmaf(-2.f, c[i], nrc);
+            const float dd = __builtin_fmaf(-2.f, c[ab][i], nrc[cur]);
             int key;  // (bits(dd) & himask) | bno in one instruction
             asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(key) : "v"(dd), "v"(himask), "s"(bno));
             best[ab][i] = min(best[ab][i], key);
           }
-        }
       }
     }
     if (ch + 1 < nch) lstore(buf ^ 1);

@@ -11,6 +11,7 @@ void launch_zero(void *p, size_t bytes, hipStream_t s);  // bytes: a multiple of
 void launch_draw(int32_t *cand, int count, uint64_t seed, int64_t n, hipStream_t s);
 void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags, hipStream_t s);
 int rep_score_blocks(int m);  // partial sums per trial of launch_rep_select
+// partial: trials * nblk partial sums + one slot for the winning trial
 void launch_rep_select(const float *X, int64_t ld, int d, int dpad, const int32_t *cand, int m, int trials,
                        double *partial, int nblk, float *reps, int32_t *rep_rows, hipStream_t s);
 // nearest representative (MFMA bf16 for 16 <= d <= 256 when rb/rn are given, VALU otherwise)
