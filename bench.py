@@ -10,7 +10,9 @@ Workloads (seeded synthetic data, knn_sweet_amd.datagen):
   c2  |R|=1,000,000, |Q|=100,000, d=64, k=32, FP32 Gaussian mixture
   c3  |R|=2,000,000, |Q|=200,000, d=128, k=100 (or 10), FP32 Gaussian mixture:
       the BF16-filter join timed as the headline, the plain FP32 join of the
-      same inputs timed beside it (`plain_fp32`, `bf16_vs_fp32`)
+      same inputs timed beside it on the matrix cores (`plain_fp32`,
+      `bf16_vs_fp32`) and on the vector ALUs (`plain_fp32_valu`,
+      `bf16_vs_fp32_valu`)
   c4  512 independent joins, |R_i| log-uniform in [1e3, 2e5], |Q_i| = |R_i|/4,
       d_i in {16, 32, 96}, k=16: one batched call per step, the per-instance
       loop timed beside it
@@ -193,6 +195,15 @@ def bench_single(args, kj, cfg):
                              "p99_ms": float(np.percentile(ftimes, 99)), "join_ms": ft_join,
                              "roofline_frac": froof["frac"], "achieved_tflops": froof["achieved"]}
         out["bf16_vs_fp32"] = value / fvalue
+        # and the plain FP32 join on the vector ALUs (direct-form distances,
+        # no matrix cores), the same index and bounds
+        vtotal, vtimes, vstats = time_joins(ctx, Rd, Qd, k, fsteps, 1,
+                                            dict(opts, precision=kj.PREC_FP32, kernel=kj.KERNEL_VALU))
+        vvalue = nq * fsteps / vtotal
+        _, vt_join, _ = roofline(kj, vstats, nq, nr, d, k)
+        out["plain_fp32_valu"] = {"value": vvalue, "steps": fsteps, "ms_per_step": vtotal * 1e3 / fsteps,
+                                  "p99_ms": float(np.percentile(vtimes, 99)), "join_ms": vt_join}
+        out["bf16_vs_fp32_valu"] = value / vvalue
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(R, Q, k)
     return out

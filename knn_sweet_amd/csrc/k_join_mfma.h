@@ -251,7 +251,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
 #pragma unroll
     for (int j = 1; j < 4; ++j)
       if (j == ii) mine = cnt[j];
-    const int cr = __shfl(mine, 16 * gi);
+    const int cr = __builtin_amdgcn_readlane(mine, 16 * gi);  // r is wave-uniform
     const float dropped = wave_merge<KCAP, MF_BC>(lists + r * KCAP, bufs + r * MF_BC, cr, tmp, make_drop<DropL>(ec));
     ++nmerge;
     absorb(r, dropped);
@@ -449,7 +449,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
             const bool needc = (c < nch) && (cmin <= hi) && (nmin >= lo);
             const u64 m = wave_ballot(needc);
             if (m == 0) {
-              if (__shfl(cmin, 0) > hi) break;
+              if (__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, cmin), 0)) > hi) break;
               continue;
             }
             const int pbeg = s0 + 64 * (cg + __builtin_ctzll(m));
@@ -574,14 +574,15 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
                   if (wave_ballot(hot) != 0) {
                     const u64 cm0 = a.tile_prof ? wall_clock64() : 0;
                     wave_sync();
-                    for (int r = 0; r < MF_TQ; ++r) {
-                      const int gi = r >> 2, ii = r & 3;
-                      int mine = cnt[0];
+                    // rows 4g + i with a full buffer: lanes 16 g hold their counts
+                    u32 rows = 0;
   #pragma unroll
-                      for (int j = 1; j < 4; ++j)
-                        if (j == ii) mine = cnt[j];
-                      if (__shfl(mine, 16 * gi) >= WM) merge(r);
+                    for (int i = 0; i < 4; ++i) {
+                      const u64 hb = wave_ballot(cnt[i] >= WM);
+  #pragma unroll
+                      for (int gg = 0; gg < 4; ++gg) rows |= (u32)((hb >> (16 * gg)) & 1u) << (4 * gg + i);
                     }
+                    for (; rows; rows &= rows - 1) merge(__builtin_ctz(rows));
                     if (a.tile_prof) mticks += wall_clock64() - cm0;
                   }
                 }
@@ -695,7 +696,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     // one slot per distinct seed: its first row's lane
     int first = c16;
     for (int j = 0; j < 15; ++j) {
-      const int vj = __shfl(seedv, j);
+      const int vj = __builtin_amdgcn_readlane(seedv, j);
       if (j < first && vj == seedv) first = j;
     }
     if (first != c16) seedv = -1;
@@ -715,7 +716,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
 #pragma unroll
           for (int j = 1; j < 4; ++j)
             if (j == ii) mine = cnt[j];
-          if (__shfl(mine, 16 * gi) > 0) merge(r);
+          if (__builtin_amdgcn_readlane(mine, 16 * gi) > 0) merge(r);
         }
       }
     }
@@ -775,7 +776,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
 #pragma unroll
     for (int j = 1; j < 4; ++j)
       if (j == ii) mine = cnt[j];
-    if (__shfl(mine, 16 * gi) > 0) merge(r);
+    if (__builtin_amdgcn_readlane(mine, 16 * gi) > 0) merge(r);
   }
   wave_sync();
   if (splitm) {
@@ -820,9 +821,9 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
         em = emax[j];
         ev = evl[j];
       }
-    const float Tr = __shfl(key_dist(t), 16 * gi);
-    const float Er = __shfl(em, 16 * gi);
-    const float Vr = __shfl(ev, 16 * gi);
+    const float Tr = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, key_dist(t)), 16 * gi));
+    const float Er = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, em), 16 * gi));
+    const float Vr = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, ev), 16 * gi));
     const float cutoff = EXACT ? Tr : (Tr + 2.f * Er) * (1.f + 2.4e-7f);
     emit_query<KCAP, EXACT, DK / 4>(a, lists + r * KCAP, q0 + r, A, cutoff, a.Q.px + (int64_t)(q0 + r) * dpQ, Tr,
                                     Vr);

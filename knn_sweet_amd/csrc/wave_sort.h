@@ -123,25 +123,23 @@ __device__ __forceinline__ float wave_merge(u64 *list, u64 *buf, int cnt, u64 *t
       y[u] = list[lane + 64 * u];
       ry[u] = 0;
     }
-    int i = 0;
-    for (; i + 4 <= cnt; i += 4) {
-      u64 b[4];
+    // pad the buffer to a multiple of 16 keys with KEY_MAX (below no key),
+    // then count in batches of 16 broadcast reads
 #pragma unroll
-      for (int t = 0; t < 4; ++t) b[t] = buf[i + t];
+    for (int u = 0; u < EB; ++u)
+      if (lane + 64 * u >= cnt) buf[lane + 64 * u] = KEY_MAX;
+    wave_sync();
+    for (int i = 0; i < cnt; i += 16) {
+      u64 b[16];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < 16; ++t) b[t] = buf[i + t];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
 #pragma unroll
         for (int u = 0; u < EB; ++u) rv[u] += b[t] < v[u] ? 1 : 0;
 #pragma unroll
         for (int u = 0; u < EL; ++u) ry[u] += b[t] < y[u] ? 1 : 0;
       }
-    }
-    for (; i < cnt; ++i) {
-      const u64 b = buf[i];
-#pragma unroll
-      for (int u = 0; u < EB; ++u) rv[u] += b < v[u] ? 1 : 0;
-#pragma unroll
-      for (int u = 0; u < EL; ++u) ry[u] += b < y[u] ? 1 : 0;
     }
     float dmin = INFINITY;
 #pragma unroll
