@@ -726,9 +726,63 @@ void launch_count(const int32_t *cl, int64_t n, int32_t *cnt, hipStream_t s) {
   hipLaunchKernelGGL(k_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cl, n, cnt);
 }
 
+// The same mean for 16-byte rows (d % 4 == 0, ld % 4 == 0, X 16-byte
+// aligned, d <= 256): thread (r, t) of RP = 256 / (d/4) row slots sums
+// float4 t of members r, r + RP, ... (four rows in flight per step) in
+// binary64; the row slots' partial sums are then added in slot order, so
+// the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_centroid4(const float4 *__restrict__ X4, int64_t ld4, int d, int dpad,
+                                                   const int32_t *__restrict__ off,
+                                                   const int32_t *__restrict__ perm, int integral, float *reps) {
+  __shared__ double part[256][4];
+  const int b = blockIdx.x;
+  const int s0 = off[b], s1 = off[b + 1];
+  if (s1 <= s0) return;
+  const int q4 = d >> 2, RP = 256 / q4;
+  const int tid = threadIdx.x, r = tid / q4, t = tid - r * q4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (r < RP) {
+    for (int p = s0 + r; p < s1; p += 4 * RP) {
+      float4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int pu = p + u * RP;
+        v[u] = pu < s1 ? X4[(int64_t)perm[pu] * ld4 + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        acc[0] += (double)v[u].x;
+        acc[1] += (double)v[u].y;
+        acc[2] += (double)v[u].z;
+        acc[3] += (double)v[u].w;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) part[tid][c] = acc[c];
+  __syncthreads();
+  if (tid < q4) {
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int rr = 0; rr < RP; ++rr)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) sum[c] += part[rr * q4 + tid][c];
+    const double cntd = (double)(s1 - s0);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float v = (float)(sum[c] / cntd);
+      if (integral) v = rintf(v);
+      reps[(int64_t)b * dpad + 4 * tid + c] = v;
+    }
+  }
+}
+
 void launch_centroid(const float *X, int64_t ld, int d, int dpad, int m, const int32_t *off, const int32_t *perm,
                      int integral, float *reps, hipStream_t s) {
-  hipLaunchKernelGGL(k_centroid, dim3(m), dim3(256), 0, s, X, ld, d, dpad, off, perm, integral, reps);
+  if (d % 4 == 0 && ld % 4 == 0 && ((uintptr_t)X & 15) == 0 && d <= 256)
+    hipLaunchKernelGGL(k_centroid4, dim3(m), dim3(256), 0, s, (const float4 *)X, ld / 4, d, dpad, off, perm, integral,
+                       reps);
+  else
+    hipLaunchKernelGGL(k_centroid, dim3(m), dim3(256), 0, s, X, ld, d, dpad, off, perm, integral, reps);
 }
 
 // ---------------------------------------------------------------------------
@@ -826,6 +880,16 @@ __global__ void k_permute(const float *__restrict__ X, int64_t ld, int d, int dp
   px[e] = t < d ? X[(int64_t)perm[p] * ld + t] : 0.f;
 }
 
+// The permuted copy for 16-byte rows (d % 4 == 0, ld % 4 == 0, X 16-byte
+// aligned; then dpad == d): one float4 per thread, 32-bit index arithmetic.
+__global__ void k_permute4(const float4 *__restrict__ X4, int64_t ld4, int q4, const int32_t *__restrict__ perm,
+                           uint32_t total, float4 *px4) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const uint32_t p = e / (uint32_t)q4, t = e - p * (uint32_t)q4;
+  px4[e] = X4[(int64_t)perm[p] * ld4 + t];
+}
+
 void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, int dpad, int m,
                                 const int32_t *cl, const int32_t *cnt, int32_t *off, int32_t *fill,
                                 int32_t *perm, const float *d2r, float *pdist, u64 *gkeys, float *px,
@@ -836,8 +900,14 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
                      perm);
   hipLaunchKernelGGL(k_segsort, dim3(m), dim3(1024), 0, s, off, perm, d2r, pdist, gkeys);
   const int64_t tot = n * dpad;
-  hipLaunchKernelGGL(k_permute, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, X, ld, d, dpad,
-                     perm, n, px);
+  if (d % 4 == 0 && ld % 4 == 0 && ((uintptr_t)X & 15) == 0 && tot / 4 < ((int64_t)1 << 32)) {
+    const int64_t t4 = tot / 4;
+    hipLaunchKernelGGL(k_permute4, dim3((unsigned)((t4 + 255) / 256)), dim3(256), 0, s, (const float4 *)X, ld / 4,
+                       d / 4, perm, (uint32_t)t4, (float4 *)px);
+  } else {
+    hipLaunchKernelGGL(k_permute, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, X, ld, d, dpad,
+                       perm, n, px);
+  }
 }
 
 // ---------------------------------------------------------------------------
