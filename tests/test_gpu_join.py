@@ -245,3 +245,25 @@ def test_many_tiny_reference_clusters(ctx, d):
     idx, dist, st = ctx.join(R, Q, 500, m_r=2000)
     assert st["m_r"] == 2000
     assert_matches_oracle(R, Q, 500, idx, dist)
+
+
+@pytest.mark.parametrize("d,extra,shift", [(64, 4, 0), (64, 3, 0), (32, 4, 1), (96, 0, 0)])
+def test_strided_device_inputs(ctx, d, extra, shift):
+    """Device row views with ld > d and unaligned starts: the 16-byte index
+    kernels (ld % 4 == 0, 16-byte aligned rows) and the scalar ones give the
+    oracle's answer."""
+    import torch
+    mix = datagen.mixture64_model(seed=d + 70 + extra, d=d, comps=12)
+    R = mix.sample(datagen.rng(d + 71), 5000)
+    Q = mix.sample(datagen.rng(d + 72), 700)
+
+    def view(X):
+        W = torch.zeros((X.shape[0], d + extra + shift), dtype=torch.float32, device="cuda")
+        W[:, shift:shift + d] = torch.from_numpy(X).cuda()
+        return W[:, shift:shift + d]
+
+    Rv, Qv = view(R), view(Q)
+    assert Rv.stride(0) == d + extra + shift
+    idx, dist, _ = ctx.join(Rv, Qv, 16)
+    torch.cuda.synchronize()
+    assert_matches_oracle(R, Q, 16, idx.cpu().numpy(), dist.cpu().numpy())
