@@ -578,14 +578,19 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
     mc += step;
   }
   // Lloyd refinement: members of each sampled representative (ascending row
-  // order), their mean becomes the representative
+  // order), their mean becomes the representative.  The means are taken over
+  // every ls-th row (a view with ls times the row stride) while that leaves
+  // about 32 or more rows per representative: an estimate of the mean is all
+  // the step needs, and the assignment it runs costs ls times less.
+  const int ls = n >= 128 * (int64_t)m ? 4 : n >= 64 * (int64_t)m ? 2 : 1;
+  const int64_t ns = (n + ls - 1) / ls;
   for (int it = 0; it < I.refine; ++it) {
-    launch_assign(X, n, ld, d, dpad, b.reps, m, b.cl, b.rb, b.rn, s);
+    launch_assign(X, ns, ld * ls, d, dpad, b.reps, m, b.cl, b.rb, b.rn, s);
     launch_zero(b.cnt, sizeof(int32_t) * m, s);
     launch_zero(b.fill, sizeof(int32_t) * m, s);
-    launch_count(b.cl, n, b.cnt, s);
-    launch_bucket_rows(b.cl, n, m, b.cnt, b.off, b.fill, b.perm, b.gkeys, s);
-    launch_centroid(X, ld, d, dpad, m, b.off, b.perm, I.integral ? 1 : 0, b.reps, s);
+    launch_count(b.cl, ns, b.cnt, s);
+    launch_bucket_rows(b.cl, ns, m, b.cnt, b.off, b.fill, b.perm, b.gkeys, s);
+    launch_centroid(X, ld * ls, d, dpad, m, b.off, b.perm, I.integral ? 1 : 0, b.reps, s);
   }
   launch_assign(X, n, ld, d, dpad, b.reps, m, b.cl, b.rb, b.rn, s);
   launch_zero(b.radbits, sizeof(u64) * m, s);
