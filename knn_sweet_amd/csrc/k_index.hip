@@ -138,14 +138,19 @@ void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags,
 // int(distance) over ordered pairs, knnjoin.cu:125-133; the argmax is the
 // same up to that truncation).  The m x m pair matrix of a trial is cut into
 // 64 x 64 blocks (upper triangle); a workgroup stages the two blocks' rows in
-// LDS 32 dimensions at a time and each thread accumulates a 4 x 4 register
-// tile.  Block partial sums are reduced in a fixed order (k_rep_pick), so the
-// choice is deterministic.
+// LDS 32 dimensions at a time.  The distances take the GEMM form
+// sqrt(max(|x_i|^2 + |x_j|^2 - 2 x_i.x_j, 0)) with the dots on
+// v_mfma_f32_16x16x4_f32: wave w owns rows 16 w .. 16 w + 15 of the block
+// against its 64 columns (four 16 x 16 accumulators); the score only ranks
+// the trials, so the FP32 GEMM form is precise enough.  Block partial sums
+// are reduced in a fixed order (k_rep_pick), so the choice is deterministic.
 #define RS_B 64
 #define RS_DC 32
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_rep_score(const float *__restrict__ X, int64_t ld, int d,
                                                    const int32_t *__restrict__ cand, int m, double *partial) {
   __shared__ float si[RS_DC][RS_B + 1], sj[RS_DC][RS_B + 1];
+  __shared__ float ni[RS_B], nj[RS_B];
   __shared__ double red[256];
   const int t = blockIdx.y;
   const int nb = (m + RS_B - 1) / RS_B;
@@ -157,58 +162,89 @@ __global__ __launch_bounds__(256) void k_rep_score(const float *__restrict__ X, 
   }
   const int bj = bi + rem;
   const int32_t *c = cand + (int64_t)t * m;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // 16 x 16 threads, 4 x 4 pairs each
-  float acc[4][4];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63, g = lane >> 4, c16 = lane & 15;
+  typedef __attribute__((__vector_size__(4 * sizeof(float)))) float rs_f32x4;
+  rs_f32x4 acc[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+  for (int b = 0; b < 4; ++b) acc[b] = rs_f32x4{0.f, 0.f, 0.f, 0.f};
+  float nacc = 0.f;  // threads 0..63: |x_i|^2 of row tid; 64..127: |x_j|^2 of row tid - 64
   for (int t0 = 0; t0 < d; t0 += RS_DC) {
     __syncthreads();
-    for (int e = threadIdx.x; e < RS_B * RS_DC; e += blockDim.x) {
-      const int r = e / RS_DC, q = e % RS_DC;
-      const int gi = bi * RS_B + r, gj = bj * RS_B + r;
-      si[q][r] = (gi < m && t0 + q < d) ? X[(int64_t)c[gi] * ld + t0 + q] : 0.f;
-      sj[q][r] = (gj < m && t0 + q < d) ? X[(int64_t)c[gj] * ld + t0 + q] : 0.f;
+    if constexpr (VEC) {
+      // 16-byte loads (rows 16-byte aligned, d % 4 == 0): thread e stages
+      // dimensions 4 q4 .. 4 q4 + 3 of row r
+      for (int e = tid; e < RS_B * RS_DC / 4; e += blockDim.x) {
+        const int r = e / (RS_DC / 4), q4 = e % (RS_DC / 4), q = 4 * q4;
+        const int gi = bi * RS_B + r, gj = bj * RS_B + r;
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 vi = (gi < m && t0 + q < d) ? *(const float4 *)(X + (int64_t)c[gi] * ld + t0 + q) : z;
+        const float4 vj = (gj < m && t0 + q < d) ? *(const float4 *)(X + (int64_t)c[gj] * ld + t0 + q) : z;
+        si[q][r] = vi.x;
+        si[q + 1][r] = vi.y;
+        si[q + 2][r] = vi.z;
+        si[q + 3][r] = vi.w;
+        sj[q][r] = vj.x;
+        sj[q + 1][r] = vj.y;
+        sj[q + 2][r] = vj.z;
+        sj[q + 3][r] = vj.w;
+      }
+    } else {
+      for (int e = tid; e < RS_B * RS_DC; e += blockDim.x) {
+        const int r = e / RS_DC, q = e % RS_DC;
+        const int gi = bi * RS_B + r, gj = bj * RS_B + r;
+        si[q][r] = (gi < m && t0 + q < d) ? X[(int64_t)c[gi] * ld + t0 + q] : 0.f;
+        sj[q][r] = (gj < m && t0 + q < d) ? X[(int64_t)c[gj] * ld + t0 + q] : 0.f;
+      }
     }
     __syncthreads();
-    for (int q = 0; q < RS_DC; ++q) {
-      float vi[4], vj[4];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        vi[a] = si[q][ty * 4 + a];
-        vj[a] = sj[q][tx * 4 + a];
+    if (tid < 2 * RS_B) {
+      const int r = tid & (RS_B - 1);
+#pragma unroll 8
+      for (int q = 0; q < RS_DC; ++q) {
+        const float v = tid < RS_B ? si[q][r] : sj[q][r];
+        nacc = __builtin_fmaf(v, v, nacc);
       }
+    }
+    // k-step q4: dimensions 4 q4 .. 4 q4 + 3; lane (c16, g) supplies A[c16][g]
+    // = x_i(row 16 w + c16, dim 4 q4 + g) and B[g][c16] = x_j(col 16 b + c16, same dim)
+#pragma unroll 4
+    for (int q4 = 0; q4 < RS_DC / 4; ++q4) {
+      const float av = si[4 * q4 + g][16 * w + c16];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const float df = vi[a] - vj[b];
-          acc[a][b] = __builtin_fmaf(df, df, acc[a][b]);
-        }
+      for (int b = 0; b < 4; ++b)
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sj[4 * q4 + g][16 * b + c16], acc[b], 0, 0, 0);
     }
   }
+  if (tid < RS_B)
+    ni[tid] = nacc;
+  else if (tid < 2 * RS_B)
+    nj[tid - RS_B] = nacc;
+  __syncthreads();
+  // acc[b][i]: pair (row 16 w + 4 g + i, column 16 b + c16)
   double sum = 0.0;
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int b = 0; b < 4; ++b)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int gi = bi * RS_B + ty * 4 + a, gj = bj * RS_B + tx * 4 + b;
-      if (gi < m && gj < m && gi < gj) sum += (double)sqrtf(acc[a][b]);
+    for (int i = 0; i < 4; ++i) {
+      const int ri = 16 * w + 4 * g + i, cj = 16 * b + c16;
+      const int gi = bi * RS_B + ri, gj = bj * RS_B + cj;
+      if (gi < m && gj < m && gi < gj) {
+        const float sq = (ni[ri] + nj[cj]) - 2.f * acc[b][i];
+        sum += (double)sqrtf(fmaxf(sq, 0.f));
+      }
     }
-  red[threadIdx.x] = sum;
+  red[tid] = sum;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if (tid < s2) red[tid] += red[tid + s2];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[(int64_t)t * gridDim.x + blockIdx.x] = red[0];
+  if (tid == 0) partial[(int64_t)t * gridDim.x + blockIdx.x] = red[0];
 }
 
-// Picks the best trial (first maximum, as the reference's strict compare,
-// knnjoin.cu:135-148) and gathers its rows into reps[m][dpad] (zero padded).
-// The trial with the largest score (first on ties); its index goes to
-// partial[trials * nblk] (one slot past the partial sums).
+// The trial with the largest score (first maximum, as the reference's strict
+// compare, knnjoin.cu:135-148); its index goes to partial[trials * nblk] (one
+// slot past the partial sums).
 __global__ void k_rep_pick(double *__restrict__ partial, int nblk, int trials) {
   __shared__ double tot[64];
   // one wave per trial slot: lane-strided sums, then a fixed shuffle tree
@@ -254,7 +290,10 @@ int rep_score_blocks(int m) {
 void launch_rep_select(const float *X, int64_t ld, int d, int dpad, const int32_t *cand, int m,
                        int trials, double *partial, int nblk, float *reps, int32_t *rep_rows,
                        hipStream_t s) {
-  hipLaunchKernelGGL(k_rep_score, dim3(nblk, trials), dim3(256), 0, s, X, ld, d, cand, m, partial);
+  if (d % 4 == 0 && ld % 4 == 0 && ((uintptr_t)X & 15) == 0)
+    hipLaunchKernelGGL(k_rep_score<true>, dim3(nblk, trials), dim3(256), 0, s, X, ld, d, cand, m, partial);
+  else
+    hipLaunchKernelGGL(k_rep_score<false>, dim3(nblk, trials), dim3(256), 0, s, X, ld, d, cand, m, partial);
   hipLaunchKernelGGL(k_rep_pick, dim3(1), dim3(1024), 0, s, partial, nblk, trials);
   const int64_t tot = (int64_t)m * dpad;
   const unsigned gb = (unsigned)std::min<int64_t>(1024, (tot + 255) / 256);
