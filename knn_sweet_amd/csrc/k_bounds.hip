@@ -220,29 +220,50 @@ __global__ __launch_bounds__(BD_THREADS, 2) void k_bounds(SideIndex Qi, SideInde
   const double rA = Qi.rad[A];
 
   // 1. d(a, b) in binary64: 4 lanes per representative, 16-byte loads, the
-  // lanes' partial sums added at the end (rows are dpad = roundup(d, 4) floats)
+  // lanes' partial sums added at the end (rows are dpad = roundup(d, 4)
+  // floats); two representatives per lane group in flight
   {
     const bd_f32x4 *ra4 = (const bd_f32x4 *)(Qi.reps + (int64_t)A * Qi.dpad);
     const int c = tid & 3, dp4 = Qi.dpad >> 2;
-    for (int b = tid >> 2; b < mR; b += BD_THREADS / 4) {
-      const bd_f32x4 *rb4 = (const bd_f32x4 *)(Ri.reps + (int64_t)b * Ri.dpad);
-      double acc = 0.0;
+    constexpr int BS = BD_THREADS / 4;
+    for (int b0 = tid >> 2; b0 < mR; b0 += 2 * BS) {
+      const int b1 = b0 + BS;
+      const bool has1 = b1 < mR;
+      const bd_f32x4 *rb4 = (const bd_f32x4 *)(Ri.reps + (int64_t)b0 * Ri.dpad);
+      const bd_f32x4 *rc4 = (const bd_f32x4 *)(Ri.reps + (int64_t)(has1 ? b1 : b0) * Ri.dpad);
+      double acc0 = 0.0, acc1 = 0.0;
       for (int j = c; j < dp4; j += 4) {
-        const bd_f32x4 x = ra4[j], y = rb4[j];
+        const bd_f32x4 x = ra4[j], y = rb4[j], z = rc4[j];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const double t = (double)x[e] - (double)y[e];
-          if (4 * j + e < d) acc = __builtin_fma(t, t, acc);
+          if (4 * j + e < d) {
+            const double t0 = (double)x[e] - (double)y[e];
+            const double t1 = (double)x[e] - (double)z[e];
+            acc0 = __builtin_fma(t0, t0, acc0);
+            acc1 = __builtin_fma(t1, t1, acc1);
+          }
         }
       }
-      acc += __shfl_xor(acc, 1);
-      acc += __shfl_xor(acc, 2);
+      acc0 += __shfl_xor(acc0, 1);
+      acc0 += __shfl_xor(acc0, 2);
+      acc1 += __shfl_xor(acc1, 1);
+      acc1 += __shfl_xor(acc1, 2);
       if (c == 0) {
-        const int o = Ri.off[b], nb = Ri.off[b + 1] - o;
-        const double dd = sqrt(acc);
-        dab[b] = nb > 0 ? dd : INFINITY;
-        keys[b] = nb > 0 ? (((u64)__float_as_uint((float)(rA + dd + (double)Ri.pdist[o])) << 32) | (u32)b) : KEY_MAX;
+        dab[b0] = sqrt(acc0);
+        if (has1) dab[b1] = sqrt(acc1);
       }
+    }
+  }
+  __syncthreads();
+  // the clusters' keys r_A + d(a,b) + min_s d(s,b) (empty clusters: never taken)
+  for (int b = tid; b < mR; b += BD_THREADS) {
+    const int o = Ri.off[b], nb = Ri.off[b + 1] - o;
+    const double dd = dab[b];
+    if (nb > 0) {
+      keys[b] = ((u64)__float_as_uint((float)(rA + dd + (double)Ri.pdist[o])) << 32) | (u32)b;
+    } else {
+      dab[b] = INFINITY;
+      keys[b] = KEY_MAX;
     }
   }
   __syncthreads();
