@@ -17,7 +17,7 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 kc_bf16x8;
 // Workgroup per cluster.  Wave 0 first writes the representative (padded to
 // dk) and its squared norm.  Rows: a group of G lanes (G = the power of two
 // >= dk/8) takes one row, lane c dimensions 8c .. 8c+7 with 16-byte loads
-// and stores; 64/G rows per wave step.
+// and stores; 2 x 64/G rows per wave step.
 template <int G>
 __global__ __launch_bounds__(256) void k_center(SideIndex Ri, int d, int dk, int planes, void *rc, float *nr,
                                                 float *reps32, float *rn32) {
@@ -42,35 +42,24 @@ __global__ __launch_bounds__(256) void k_center(SideIndex Ri, int d, int dk, int
   float rv[8];
 #pragma unroll
   for (int u = 0; u < 8; ++u) rv[u] = (8 * c + u < d) ? rep[8 * c + u] : 0.f;
-  for (int p0 = s0 + wave * RPW; p0 < s1; p0 += 4 * RPW) {
-    const int p = p0 + sub;
-    const bool ok = p < s1 && act;
-    float v[8];
-    float part = 0.f;
-    if (ok) {
-      const float *x = Ri.px + (int64_t)p * Ri.dpad + 8 * c;
-      float xv[8];
-      if (8 * c < Ri.dpad) {
-        const float4 x0 = *(const float4 *)x;
-        xv[0] = x0.x; xv[1] = x0.y; xv[2] = x0.z; xv[3] = x0.w;
-      } else {
-        xv[0] = xv[1] = xv[2] = xv[3] = 0.f;
-      }
-      if (8 * c + 4 < Ri.dpad) {
-        const float4 x1 = *(const float4 *)(x + 4);
-        xv[4] = x1.x; xv[5] = x1.y; xv[6] = x1.z; xv[7] = x1.w;
-      } else {
-        xv[4] = xv[5] = xv[6] = xv[7] = 0.f;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        v[u] = (8 * c + u < d) ? xv[u] - rv[u] : 0.f;
-        part = __builtin_fmaf(v[u], v[u], part);
-      }
+  // two rows per lane group per step (a workgroup step covers 8 RPW rows):
+  // both rows' loads are in flight before either is centred
+  auto load_row = [&](int p, float (&xv)[8]) {
+    const float *x = Ri.px + (int64_t)p * Ri.dpad + 8 * c;
+    if (8 * c < Ri.dpad) {
+      const float4 x0 = *(const float4 *)x;
+      xv[0] = x0.x; xv[1] = x0.y; xv[2] = x0.z; xv[3] = x0.w;
+    } else {
+      xv[0] = xv[1] = xv[2] = xv[3] = 0.f;
     }
-#pragma unroll
-    for (int msk = 1; msk < G; msk <<= 1) part += __shfl_xor(part, msk);
-    if (!ok) continue;
+    if (8 * c + 4 < Ri.dpad) {
+      const float4 x1 = *(const float4 *)(x + 4);
+      xv[4] = x1.x; xv[5] = x1.y; xv[6] = x1.z; xv[7] = x1.w;
+    } else {
+      xv[4] = xv[5] = xv[6] = xv[7] = 0.f;
+    }
+  };
+  auto store_row = [&](int p, const float (&v)[8], float part) {
     if (planes == 2) {
       kc_bf16x8 h, l;
 #pragma unroll
@@ -91,6 +80,29 @@ __global__ __launch_bounds__(256) void k_center(SideIndex Ri, int d, int dk, int
       *(float4 *)(o + 4) = make_float4(v[4], v[5], v[6], v[7]);
     }
     if (c == 0) nr[p] = part;
+  };
+  for (int p0 = s0 + wave * 2 * RPW; p0 < s1; p0 += 8 * RPW) {
+    const int pa = p0 + sub, pb = p0 + RPW + sub;
+    const bool oka = pa < s1 && act, okb = pb < s1 && act;
+    float xa[8], xb[8];
+    if (oka) load_row(pa, xa);
+    if (okb) load_row(pb, xb);
+    float va[8], vb[8];
+    float parta = 0.f, partb = 0.f;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      va[u] = (oka && 8 * c + u < d) ? xa[u] - rv[u] : 0.f;
+      vb[u] = (okb && 8 * c + u < d) ? xb[u] - rv[u] : 0.f;
+      parta = __builtin_fmaf(va[u], va[u], parta);
+      partb = __builtin_fmaf(vb[u], vb[u], partb);
+    }
+#pragma unroll
+    for (int msk = 1; msk < G; msk <<= 1) {
+      parta += __shfl_xor(parta, msk);
+      partb += __shfl_xor(partb, msk);
+    }
+    if (oka) store_row(pa, va, parta);
+    if (okb) store_row(pb, vb, partb);
   }
 }
 

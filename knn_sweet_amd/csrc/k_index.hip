@@ -649,31 +649,49 @@ __global__ __launch_bounds__(256) void k_assign_finish4(const float *__restrict_
                                                         int dpad, const float *__restrict__ reps,
                                                         const int32_t *__restrict__ cl, float *d2r, u64 *radbits,
                                                         int32_t *cnt) {
-  constexpr int RPW = 64 / L;  // rows per wave step
+  // a wave step takes 2 RPW rows: lane group sub holds rows i and i + RPW,
+  // both rows' loads in flight before the binary64 sums
+  constexpr int RPW = 64 / L;  // rows per lane-group pass
   const int lane = threadIdx.x & 63, sub = lane / L, j = lane % L;
   const int64_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int64_t i = wave * RPW + sub;
-  const bool ok = i < n;
-  const int b = ok ? cl[i] : 0;
-  double acc = 0.0;
-  for (int c = j; 4 * c < d; c += L) {
-    const float4 x = ok ? *(const float4 *)(X + i * ld + 4 * c) : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 r = *(const float4 *)(reps + (int64_t)b * dpad + 4 * c);
-    const float xa[4] = {x.x, x.y, x.z, x.w}, ra[4] = {r.x, r.y, r.z, r.w};
+  int64_t ir[2];
+  bool ok[2];
+  int b[2];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (4 * c + u < d) {
-        const double t = (double)xa[u] - (double)ra[u];
-        acc += t * t;
-      }
+  for (int h = 0; h < 2; ++h) {
+    ir[h] = wave * 2 * RPW + h * RPW + sub;
+    ok[h] = ir[h] < n;
+    b[h] = ok[h] ? cl[ir[h]] : 0;
+  }
+  double acc[2] = {0.0, 0.0};
+  for (int c = j; 4 * c < d; c += L) {
+    float4 x[2], r[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      x[h] = ok[h] ? *(const float4 *)(X + ir[h] * ld + 4 * c) : make_float4(0.f, 0.f, 0.f, 0.f);
+      r[h] = *(const float4 *)(reps + (int64_t)b[h] * dpad + 4 * c);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const float xa[4] = {x[h].x, x[h].y, x[h].z, x[h].w}, ra[4] = {r[h].x, r[h].y, r[h].z, r[h].w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (4 * c + u < d) {
+          const double t = (double)xa[u] - (double)ra[u];
+          acc[h] += t * t;
+        }
+    }
   }
 #pragma unroll
-  for (int msk = 1; msk < L; msk <<= 1) acc += __shfl_xor(acc, msk);
-  if (ok && j == 0) {
-    const double dd = sqrt(acc);
-    d2r[i] = (float)dd;
-    atomicMax(&radbits[b], (u64)__double_as_longlong(dd));
-    atomicAdd(&cnt[b], 1);
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int msk = 1; msk < L; msk <<= 1) acc[h] += __shfl_xor(acc[h], msk);
+    if (ok[h] && j == 0) {
+      const double dd = sqrt(acc[h]);
+      d2r[ir[h]] = (float)dd;
+      atomicMax(&radbits[b[h]], (u64)__double_as_longlong(dd));
+      atomicAdd(&cnt[b[h]], 1);
+    }
   }
 }
 
@@ -684,7 +702,7 @@ void launch_assign_finish(const float *X, int64_t n, int64_t ld, int d, int dpad
   if (vec) {
     int L = 1;
     while (L < dpad / 4) L <<= 1;
-    const int64_t waves = (n + (64 / L) - 1) / (64 / L);
+    const int64_t waves = (n + 2 * (64 / L) - 1) / (2 * (64 / L));  // 2 x 64/L rows per wave
     const dim3 g((unsigned)((waves + 3) / 4));
 #define KJ_FIN(LL) hipLaunchKernelGGL(k_assign_finish4<LL>, g, dim3(256), 0, s, X, n, ld, d, dpad, reps, cl, d2r, radbits, cnt)
     switch (L) {
