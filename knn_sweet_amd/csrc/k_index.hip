@@ -890,13 +890,62 @@ __device__ __forceinline__ void cex(u64 *a, int i, int j) {
   }
 }
 
+// Clusters of at most SEG_WAVE rows: one wave sorts the keys in registers
+// (wave_sort_u64, no workgroup barriers); k_segsort (LDS bitonic network,
+// global memory above SEG_LDS keys) takes the larger ones.
+#define SEG_WAVE 512
+template <int E>
+__device__ __forceinline__ void seg_wave_sort(int32_t *perm, const float *__restrict__ d2r, float *pdist, int s0,
+                                              int n) {
+  const int lane = threadIdx.x & 63;
+  u64 v[E];
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int j = lane + 64 * u;
+    if (j < n) {
+      const int row = perm[s0 + j];
+      v[u] = ((u64)(d2r ? __float_as_uint(d2r[row]) : 0u) << 32) | (u32)row;
+    } else {
+      v[u] = KEY_MAX;
+    }
+  }
+  wave_sort_u64<E>(v);
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int j = lane + 64 * u;
+    if (j < n) {
+      perm[s0 + j] = (int32_t)(u32)v[u];
+      if (pdist) pdist[s0 + j] = __uint_as_float((u32)(v[u] >> 32));
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void k_segsort_wave(const int32_t *__restrict__ off, int32_t *perm,
+                                                     const float *__restrict__ d2r, float *pdist) {
+  const int c = blockIdx.x;
+  const int s0 = off[c], n = off[c + 1] - s0;
+  if (n <= 1 || n > SEG_WAVE) return;
+  if (n <= 64)
+    seg_wave_sort<1>(perm, d2r, pdist, s0, n);
+  else if (n <= 128)
+    seg_wave_sort<2>(perm, d2r, pdist, s0, n);
+  else if (n <= 256)
+    seg_wave_sort<4>(perm, d2r, pdist, s0, n);
+  else
+    seg_wave_sort<8>(perm, d2r, pdist, s0, n);
+}
+
 __global__ __launch_bounds__(1024) void k_segsort(const int32_t *__restrict__ off, int32_t *perm,
                                                   const float *__restrict__ d2r, float *pdist,
                                                   u64 *gkeys) {
   __shared__ u64 sk[SEG_LDS];
   const int c = blockIdx.x;
   const int s0 = off[c], n = off[c + 1] - s0;
-  if (n <= 0) return;
+  if (n <= SEG_WAVE) {
+    // sorted by k_segsort_wave; a single row still gets its distance
+    if (n == 1 && threadIdx.x == 0 && pdist) pdist[s0] = d2r ? d2r[perm[s0]] : 0.f;
+    return;
+  }
   int p2 = 1;
   while (p2 < n) p2 <<= 1;
   const bool in_lds = p2 <= SEG_LDS;
@@ -955,6 +1004,7 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
   if (n <= 0) return;
   hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cl, n, off, fill,
                      perm);
+  hipLaunchKernelGGL(k_segsort_wave, dim3(m), dim3(64), 0, s, off, perm, d2r, pdist);
   hipLaunchKernelGGL(k_segsort, dim3(m), dim3(1024), 0, s, off, perm, d2r, pdist, gkeys);
   const int64_t tot = n * dpad;
   if (d % 4 == 0 && ld % 4 == 0 && ((uintptr_t)X & 15) == 0 && tot / 4 < ((int64_t)1 << 32)) {
@@ -1103,6 +1153,7 @@ void launch_bucket_rows(const int32_t *cl, int64_t n, int m, const int32_t *cnt,
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, cnt, m, off);
   if (n <= 0) return;
   hipLaunchKernelGGL(k_scatter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cl, n, off, fill, perm);
+  hipLaunchKernelGGL(k_segsort_wave, dim3(m), dim3(64), 0, s, off, perm, (const float *)nullptr, (float *)nullptr);
   hipLaunchKernelGGL(k_segsort, dim3(m), dim3(1024), 0, s, off, perm, (const float *)nullptr, (float *)nullptr,
                      gkeys);
 }
