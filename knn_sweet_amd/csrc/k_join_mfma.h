@@ -631,53 +631,17 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     }
     const int np_seed = a.seed_blocks > 0 ? min(np, 16 * a.seed_blocks) : np;
     if (np_seed > 0) {
-      // The seed only orders the visits, so its centre distances need no
-      // error bound: q.c runs as v_mfma_f32_16x16x32_bf16 on bf16-rounded
-      // operands (NS products per block instead of 8 NS FP32 ones), with the
-      // exact FP32 norms.  Lane (c16, g) holds dimensions 32 s + 8 g + j.
-      bf16x8_t qb[NS];
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        float v[8];
-        q_slice(s, v);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) qb[s][j] = (__bf16)v[j];
-      }
-      // representative slice and squared norm of lane c16's cluster
-      struct SeedPre {
-        f32x4_t c[2 * NS];
-        float cn;
-      };
-      auto load_seed = [&](const int bj, SeedPre &P) {
-        const float *cr = a.reps32 + (int64_t)bj * dk + 8 * g;
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-          P.c[2 * s] = *(const f32x4_t *)(cr + 32 * s);
-          P.c[2 * s + 1] = *(const f32x4_t *)(cr + 32 * s + 4);
-        }
-        P.cn = a.rn32[bj];
-      };
       int bjc = list_at(c16), bjn = list_at(16 + c16);
-      SeedPre cur;
-      load_seed(bjc, cur);
+      Pre cur;
+      load_pre(bjc, cur);
       for (int pb = 0; pb < np_seed; pb += 16) {
         ++nseedb;
         // next block's loads in flight while this one computes
-        SeedPre nxt;
-        load_seed(bjn, nxt);
+        Pre nxt;
+        load_pre(bjn, nxt);
         const int bjnn = list_at(pb + 32 + c16);
         const bool lv = pb + c16 < np;
-        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-          bf16x8_t cb;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            cb[j] = (__bf16)cur.c[2 * s][j];
-            cb[4 + j] = (__bf16)cur.c[2 * s + 1][j];
-          }
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qb[s], cb, acc, 0, 0, 0);
-        }
+        const f32x4_t acc = center_dots(cur);
         const float cn = cur.cn;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -702,6 +666,27 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
           bestj[i] = oj;
         }
       }
+    // Each row's threshold from its nearest centre c_b: the k members of b
+    // nearest to c_b lie within d(q, c_b) + pdist[off_b + k - 1] of q (rows
+    // of a cluster ascend by pdist), so that bounds the k-th neighbour
+    // distance -- usually far below the query cluster's theta_A, which cuts
+    // the candidates the seed visit appends before its list holds k keys.
+    // d(q, c_b) is taken from above through the screening's error bound.
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int b = bestj[i];
+      if (live[i] && b >= 0) {
+        const int o = a.R.off[b];
+        if (a.R.off[b + 1] - o >= k) {
+          const float pk = a.R.pdist[o + k - 1] * (1.f + 1e-6f);
+          const float nsum = sqrtf(qnr[i]) + sqrtf(a.rn32[b]);
+          const float dhi = sqrtf(fmaxf(bestd[i] + gE * nsum * nsum, 0.f)) * (1.f + eps);
+          const float ub = (dhi + pk) * (1.f + eps);
+          const u64 t = ((u64)__float_as_uint(ub * ub * (1.f + 2.f * eps)) << 32) | 0xFFFFFFFFull;
+          if (t < thr[i]) thr[i] = t;
+        }
+      }
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int v = __shfl(bestj[i], 16 * (c16 >> 2));
