@@ -317,6 +317,7 @@ struct Inst {
   bool mfma = false, bf16 = false, bf16x1 = false, exact = false, exact_valu = false, integral = false;
   bool use_mfma = false;
   int split_rounds = 0;  // split rounds in the representative selection
+  bool resident = false;  // building a resident index (knnj_index_build): full Lloyd means
   double span = 0.0;
   uint64_t seed = 2015;
   int64_t max_slots = 0;
@@ -578,11 +579,13 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
     mc += step;
   }
   // Lloyd refinement: members of each sampled representative (ascending row
-  // order), their mean becomes the representative.  The means are taken over
-  // every ls-th row (a view with ls times the row stride) while that leaves
-  // about 32 or more rows per representative: an estimate of the mean is all
-  // the step needs, and the assignment it runs costs ls times less.
-  const int ls = n >= 128 * (int64_t)m ? 4 : n >= 64 * (int64_t)m ? 2 : 1;
+  // order), their mean becomes the representative.  In a join the means are
+  // taken over every ls-th row (a view with ls times the row stride) while
+  // that leaves about 32 or more rows per representative: an estimate of the
+  // mean is all the step needs, and the assignment it runs costs ls times
+  // less.  A resident index, built once and queried many times, uses every row.
+  int ls = I.resident ? 1 : n >= 128 * (int64_t)m ? 4 : n >= 64 * (int64_t)m ? 2 : 1;
+  if (const char *e = std::getenv("KNNJ_LLOYD_STRIDE")) ls = std::max(1, std::atoi(e));  // diagnostic override
   const int64_t ns = (n + ls - 1) / ls;
   for (int it = 0; it < I.refine; ++it) {
     launch_assign(X, ns, ld * ls, d, dpad, b.reps, m, b.cl, b.rb, b.rn, s);
@@ -1120,10 +1123,12 @@ knnj_status knnj_index_build(knnj_ctx *c, knnj_matrix R, const knnj_opts *opts, 
   Inst I;
   I.d = R.d;
   I.k = 1;
-  // a resident index is built once: more split rounds and Lloyd steps by default
+  // a resident index is built once: more split rounds and Lloyd steps by
+  // default, its Lloyd means over every row
   if (o.split_rounds == 0) o.split_rounds = 16;
   if (o.refine_iters == 0) o.refine_iters = 2;
   I.o = o;
+  I.resident = true;
   I.nR = R.n;
   I.ldR = R.ld ? R.ld : R.d;
   (R.on_device ? I.dR : I.hR) = R.data;
