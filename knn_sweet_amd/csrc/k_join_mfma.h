@@ -810,7 +810,9 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   const int nemit = (splitm && wv != 0) ? 0 : nq;
   const u64 clk_merge = a.tile_prof ? wall_clock64() : 0;
 
-  for (int r = 0; r < nemit; ++r) {
+  // row r's final threshold, error maximum and smallest evicted lower bound
+  // (wave-uniform r), and the cutoff of its rerank
+  auto row_final = [&](const int r, float &Tr, float &Vr, float &cutoff) {
     const int gi = r >> 2, ii = r & 3;
     u64 t = thr[0];
     float em = emax[0], ev = evl[0];
@@ -821,10 +823,32 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
         em = emax[j];
         ev = evl[j];
       }
-    const float Tr = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, key_dist(t)), 16 * gi));
+    Tr = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, key_dist(t)), 16 * gi));
     const float Er = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, em), 16 * gi));
-    const float Vr = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, ev), 16 * gi));
-    const float cutoff = EXACT ? Tr : (Tr + 2.f * Er) * (1.f + 2.4e-7f);
+    Vr = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, ev), 16 * gi));
+    cutoff = EXACT ? Tr : (Tr + 2.f * Er) * (1.f + 2.4e-7f);
+  };
+  // two rows at a time (their rerank loads in flight together) up to
+  // capacity 128, one at a time above
+  constexpr int NQE = KCAP <= 128 ? 2 : 1;
+  int r = 0;
+  for (; r + NQE <= nemit; r += NQE) {
+    const u64 *Ls[NQE];
+    int qp[NQE];
+    float cu[NQE], th[NQE], ev[NQE];
+    const float *qr[NQE];
+#pragma unroll
+    for (int n = 0; n < NQE; ++n) {
+      row_final(r + n, th[n], ev[n], cu[n]);
+      Ls[n] = lists + (r + n) * KCAP;
+      qp[n] = q0 + r + n;
+      qr[n] = a.Q.px + (int64_t)(q0 + r + n) * dpQ;
+    }
+    emit_queries<KCAP, EXACT, DK / 4, NQE>(a, Ls, qp, A, cu, qr, th, ev);
+  }
+  for (; r < nemit; ++r) {
+    float Tr, Vr, cutoff;
+    row_final(r, Tr, Vr, cutoff);
     emit_query<KCAP, EXACT, DK / 4>(a, lists + r * KCAP, q0 + r, A, cutoff, a.Q.px + (int64_t)(q0 + r) * dpQ, Tr,
                                     Vr);
   }
