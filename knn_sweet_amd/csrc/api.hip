@@ -1017,8 +1017,9 @@ std::vector<knnj_opts> tune_candidates(const knnj_opts &o, const knnj_matrix &R,
   const bool valu = o.kernel == KNNJ_KERNEL_VALU ||
                     (o.kernel == KNNJ_KERNEL_AUTO && !(d >= 32 && d <= 256) && o.precision != KNNJ_PREC_BF16 &&
                      o.precision != KNNJ_PREC_BF16X1);
-  const int refines[2] = {o.refine_iters ? o.refine_iters : 1, -1};
-  const int nref = o.refine_iters ? 1 : 2;
+  // Lloyd steps: 1, none, or 2 (cheap with the strided means; C3 takes 2)
+  const int refines[3] = {o.refine_iters ? o.refine_iters : 1, -1, 2};
+  const int nref = o.refine_iters ? 1 : 3;
   const int scales[2] = {1, 2};
   const int nsc = (o.m_r || o.m_q || o.brute_force) ? 1 : 2;
   const int tqs[2] = {o.tile_q ? o.tile_q : 2, 4};
