@@ -73,6 +73,15 @@ __device__ __forceinline__ D make_drop(const ECode &ec) {
 #define MF_TQ 16
 #define MF_BC 64  // per-query candidate buffer
 
+// Element ii (wave-uniform) of a lane's four per-row values.  The values pass
+// through an empty asm statement so the compiler cannot fold the selects into
+// one indexed load, which would move the per-row arrays to scratch memory.
+template <class T>
+__device__ __forceinline__ T pick4(T a0, T a1, T a2, T a3, int ii) {
+  asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3));
+  return ii == 0 ? a0 : ii == 1 ? a1 : ii == 2 ? a2 : a3;
+}
+
 // LDS of one wave (u64 words): lists, buffers, merge scratch, then floats:
 // row norms [16], screening distances [16][16], query rows [16][DK + 4]
 __host__ __device__ constexpr size_t mfma_wave_words(int kcap, int dk) {
@@ -234,23 +243,19 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     const u64 kth = lists[r * KCAP + k - 1];
     u64 t = KEY_MAX;
     if (kth != KEY_MAX) t = EXACT ? kth - 1 : (((u64)__float_as_uint(key_dist(kth)) << 32) | 0xFFFFFFFFull);
-    if (g == gi) {
+    // every element written with a static index (no indexed store)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (j == ii) {
-          cnt[j] = 0;
-          if (t < thr[j]) thr[j] = t;
-          evl[j] = fminf(evl[j], dropped);
-        }
+    for (int j = 0; j < 4; ++j) {
+      const bool me = g == gi && j == ii;
+      cnt[j] = me ? 0 : cnt[j];
+      thr[j] = (me && t < thr[j]) ? t : thr[j];
+      evl[j] = me ? fminf(evl[j], dropped) : evl[j];
     }
     if (splitm && lane == 16 * gi && t != KEY_MAX) atomicMin(sthr + r, t);
   };
   auto merge = [&](int r) {
     const int gi = r >> 2, ii = r & 3;
-    int mine = cnt[0];
-#pragma unroll
-    for (int j = 1; j < 4; ++j)
-      if (j == ii) mine = cnt[j];
+    const int mine = pick4(cnt[0], cnt[1], cnt[2], cnt[3], ii);
     const int cr = __builtin_amdgcn_readlane(mine, 16 * gi);  // r is wave-uniform
     const float dropped = wave_merge<KCAP, MF_BC>(lists + r * KCAP, bufs + r * MF_BC, cr, tmp, make_drop<DropL>(ec));
     ++nmerge;
@@ -712,10 +717,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
         wave_sync();
         for (int r = 0; r < MF_TQ; ++r) {
           const int gi = r >> 2, ii = r & 3;
-          int mine = cnt[0];
-#pragma unroll
-          for (int j = 1; j < 4; ++j)
-            if (j == ii) mine = cnt[j];
+          const int mine = pick4(cnt[0], cnt[1], cnt[2], cnt[3], ii);
           if (__builtin_amdgcn_readlane(mine, 16 * gi) > 0) merge(r);
         }
       }
@@ -772,10 +774,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   wave_sync();
   for (int r = 0; r < MF_TQ; ++r) {
     const int gi = r >> 2, ii = r & 3;
-    int mine = cnt[0];
-#pragma unroll
-    for (int j = 1; j < 4; ++j)
-      if (j == ii) mine = cnt[j];
+    const int mine = pick4(cnt[0], cnt[1], cnt[2], cnt[3], ii);
     if (__builtin_amdgcn_readlane(mine, 16 * gi) > 0) merge(r);
   }
   wave_sync();
@@ -814,15 +813,9 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   // (wave-uniform r), and the cutoff of its rerank
   auto row_final = [&](const int r, float &Tr, float &Vr, float &cutoff) {
     const int gi = r >> 2, ii = r & 3;
-    u64 t = thr[0];
-    float em = emax[0], ev = evl[0];
-#pragma unroll
-    for (int j = 1; j < 4; ++j)
-      if (j == ii) {
-        t = thr[j];
-        em = emax[j];
-        ev = evl[j];
-      }
+    const u64 t = pick4(thr[0], thr[1], thr[2], thr[3], ii);
+    const float em = pick4(emax[0], emax[1], emax[2], emax[3], ii);
+    const float ev = pick4(evl[0], evl[1], evl[2], evl[3], ii);
     Tr = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, key_dist(t)), 16 * gi));
     const float Er = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, em), 16 * gi));
     Vr = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, ev), 16 * gi));
