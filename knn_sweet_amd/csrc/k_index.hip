@@ -176,9 +176,11 @@ __global__ __launch_bounds__(256) void k_rep_score(const float *__restrict__ X, 
       for (int e = tid; e < RS_B * RS_DC / 4; e += blockDim.x) {
         const int r = e / (RS_DC / 4), q4 = e % (RS_DC / 4), q = 4 * q4;
         const int gi = bi * RS_B + r, gj = bj * RS_B + r;
-        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-        const float4 vi = (gi < m && t0 + q < d) ? *(const float4 *)(X + (int64_t)c[gi] * ld + t0 + q) : z;
-        const float4 vj = (gj < m && t0 + q < d) ? *(const float4 *)(X + (int64_t)c[gj] * ld + t0 + q) : z;
+        // guarded loads (a select between a load and a constant can become a
+        // load through a selected pointer, with the constant in scratch memory)
+        float4 vi = make_float4(0.f, 0.f, 0.f, 0.f), vj = vi;
+        if (gi < m && t0 + q < d) vi = *(const float4 *)(X + (int64_t)c[gi] * ld + t0 + q);
+        if (gj < m && t0 + q < d) vj = *(const float4 *)(X + (int64_t)c[gj] * ld + t0 + q);
         si[q][r] = vi.x;
         si[q + 1][r] = vi.y;
         si[q + 2][r] = vi.z;
