@@ -7,7 +7,10 @@ configuration's query set against its reference set, inputs already
 resident in HBM.  `value` is queries per second over the K timed steps.
 Workloads (seeded synthetic data, knn_sweet_amd.datagen):
   c1  self-join of 245,057 integer points, d=3, k=200 (plain FP32, exact keys)
-  c2  |R|=1,000,000, |Q|=100,000, d=64, k=32, FP32 Gaussian mixture
+  c2  |R|=1,000,000, |Q|=100,000, d=64, k=32, FP32 Gaussian mixture: the
+      plain FP32 join on the matrix cores (v_mfma_f32_16x16x4_f32 tiles) timed
+      as the headline, the plain FP32 join on the vector ALUs beside it
+      (`plain_fp32_valu`, `mfma_vs_valu`)
   c3  |R|=2,000,000, |Q|=200,000, d=128, k=100 (or 10), FP32 Gaussian mixture:
       the BF16-filter join timed as the headline, the plain FP32 join of the
       same inputs timed beside it on the matrix cores (`plain_fp32`,
@@ -23,8 +26,12 @@ The roofline object describes the join kernel (the dominant one); its
 duration is the hipEvent time the library records around its launch on its
 own stream.  `traffic` is the HBM bytes per launch recorded from rocprofv3
 FETCH_SIZE + WRITE_SIZE runs in profiles/hbm_traffic.json (null if that
-workload has no record).  cpu_baseline times the FP64 brute-force oracle on
-a bounded sample of the same queries on this host's cores.
+workload has no record).  cpu_baseline times, on this host's cores and a
+bounded sample of the same queries, the FP64 brute-force oracle and the CPU
+triangle-inequality join (the reference's algorithm restated in C with its
+defects fixed); `parity` compares the GPU answer of the last timed step
+with the oracle's on that sample (indices bit-identical, distances within
+1e-5 relative) and the bench fails on any mismatch.
 """
 from __future__ import annotations
 
@@ -70,23 +77,73 @@ def load_config(name: str, k: int | None):
     raise SystemExit(f"unknown config {name}")
 
 
-def cpu_baseline(R, Q, k, target_s: float = 12.0):
-    """FP64 brute-force oracle on a bounded query sample: the sample grows
-    until one timed pass takes at least half of target_s (or covers Q)."""
+def host_threads() -> int:
+    """Threads the CPU baselines use: OMP_NUM_THREADS if set, else every core."""
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+
+
+def cpu_baseline(R, Q, k, target_s: float = 12.0, ti_target_s: float = 8.0, rows=None):
+    """The CPU baselines on a bounded query sample of the same workload.
+
+    FP64 brute force (the oracle): the sample grows until one timed pass
+    takes at least half of target_s (or covers Q); the rows it computes are
+    kept as the parity reference of the GPU answer.
+    Triangle-inequality join: its index build covers the whole R whatever
+    the sample, so it is timed once on a query sample sized from the brute
+    force's rate, and the full-workload rate is extrapolated from its two
+    phases (index and bounds once, scan per query).
+    rows: query indices to sample (default: the first ones)."""
     import oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    threads = min(threads, 16)
-    oracle.bruteforce(R, Q[:min(8, Q.shape[0])], k, nthreads=threads)  # warm-up: threads, pages
-    n = min(32, Q.shape[0])
+    threads = host_threads()
+    nq = Q.shape[0]
+    order = np.arange(nq) if rows is None else np.asarray(rows)
+    oracle.bruteforce(R, Q[order[:min(8, nq)]], k, nthreads=threads)  # warm-up: threads, pages
+    n = min(32, nq)
     while True:
+        sel = order[:n]
+        Qs = np.ascontiguousarray(Q[sel])
         t0 = time.perf_counter()
-        oracle.bruteforce(R, Q[:n], k, nthreads=threads)
+        ref = oracle.bruteforce(R, Qs, k, nthreads=threads)
         dt = time.perf_counter() - t0
-        if dt >= 0.5 * target_s or n >= Q.shape[0]:
+        if dt >= 0.5 * target_s or n >= nq:
             break
-        n = int(min(Q.shape[0], max(2 * n, n * target_s / max(dt, 1e-3))))
-    return {"value": n / dt, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} queries of the same workload, FP64 brute force (oracle), {dt:.1f} s"}
+        n = int(min(nq, max(2 * n, n * target_s / max(dt, 1e-3))))
+    bf_rate = n / dt
+    # TI join: a sample whose scan takes about ti_target_s at the brute force's rate
+    # times the pruning the TI join typically reaches (it is re-measured below)
+    nt = int(min(nq, max(64, 20 * ti_target_s * bf_rate)))
+    tsel = np.ascontiguousarray(Q[order[:nt]])
+    t0 = time.perf_counter()
+    _, _, _, ev, (t_idx, t_scan) = oracle.ti_join(R, tsel, k, nthreads=threads, phases=True)
+    ti_dt = time.perf_counter() - t0
+    ti_full = nq / (t_idx + nq * t_scan / nt)
+    out = {"value": bf_rate, "unit": "queries/s", "cores": threads, "kind": "port",
+           "host_cpus": os.cpu_count(),
+           "sample": f"{n} queries of the same workload ({'first rows' if rows is None else 'seeded sample'}), "
+                     f"FP64 brute force (the oracle), {dt:.1f} s",
+           "ti_join": {"value_full_workload": ti_full, "unit": "queries/s", "sample_queries": nt,
+                       "sample_s": ti_dt, "index_and_bounds_s": t_idx, "scan_s": t_scan,
+                       "scan_queries_per_s": nt / t_scan if t_scan > 0 else None,
+                       "dist_evals_per_query": ev / nt,
+                       "note": "CPU triangle-inequality join (reference algorithm, defects fixed); "
+                               "full-workload rate = |Q| / (index+bounds + |Q| x scan time per query)"}}
+    return out, (sel, ref)
+
+
+def parity_check(sel, ref, idx, dist):
+    """GPU rows `sel` of (idx, dist) against the oracle's (ref_idx, ref_dist, _):
+    indices bit-identical, distances within 1e-5 relative.  Raises on a mismatch."""
+    ref_idx, ref_dist, _ = ref
+    gi = np.asarray(idx)[sel]
+    gd = np.asarray(dist)[sel]
+    bad = int((gi != ref_idx).any(axis=1).sum())
+    rel = float(np.max(np.abs(gd - ref_dist) / np.maximum(np.abs(ref_dist), 1e-30))) if gd.size else 0.0
+    out = {"rows_checked": int(len(sel)), "mismatches": bad, "max_dist_rel_err": rel,
+           "dist_bit_identical_frac": float((gd == ref_dist).mean()) if gd.size else 1.0}
+    if bad or rel > 1e-5:
+        print(json.dumps({"parity_failure": out}), file=sys.stderr)
+        raise SystemExit(f"parity failure: {bad} of {len(sel)} rows differ from the oracle (max rel {rel:.3g})")
+    return out
 
 
 def traffic_record(workload: str, precision: str):
@@ -111,19 +168,23 @@ def copy_probe(nbytes: int = 1 << 30, reps: int = 3):
 
 
 def time_joins(ctx, Rd, Qd, k, steps, warmup, opts):
+    """Times `steps` joins after `warmup`; returns (seconds, per-step ms,
+    stats, the last step's (idx, dist) on the device)."""
     import torch
     for _ in range(warmup):
         ctx.join(Rd, Qd, k, **opts)
     torch.cuda.synchronize()
     times, stats = [], []
+    last = None
     t0 = time.perf_counter()
     for _ in range(steps):
         s0 = time.perf_counter()
-        _, _, st = ctx.join(Rd, Qd, k, **opts)
+        idx, dist, st = ctx.join(Rd, Qd, k, **opts)
         times.append((time.perf_counter() - s0) * 1e3)
         stats.append(st)
+        last = (idx, dist)
     torch.cuda.synchronize()
-    return time.perf_counter() - t0, times, stats
+    return time.perf_counter() - t0, times, stats, last
 
 
 def roofline(kj, stats, nq, nr, d, k):
@@ -155,13 +216,16 @@ def bench_single(args, kj, cfg):
     Qd = Rd if cfg["self_join"] else torch.from_numpy(Q).cuda()
     torch.cuda.synchronize()
     ctx = kj.Context(0)
+    if args.config == "c2" and args.precision == "auto" and args.kernel == "auto":
+        # the plain FP32 join (no BF16 filter): matrix cores headline, vector ALUs beside it
+        args.precision, args.kernel = "fp32", "mfma"
     opts = dict(brute_force=args.brute_force, tile_waves=args.tile_waves, xcd_tiles=args.xcd_tiles,
                 tile_split=args.tile_split, split_rounds=args.split_rounds,
                 refine_iters=args.refine_iters, tile_q=args.tile_q, autotune=not args.no_autotune,
                 kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel],
                 precision={"auto": kj.PREC_AUTO, "fp32": kj.PREC_FP32, "bf16": kj.PREC_BF16,
                            "bf16x1": kj.PREC_BF16X1}[args.precision])
-    total, times, stats = time_joins(ctx, Rd, Qd, k, args.steps, args.warmup, opts)
+    total, times, stats, last = time_joins(ctx, Rd, Qd, k, args.steps, args.warmup, opts)
     nq, nr, d = Q.shape[0], R.shape[0], R.shape[1]
     value = nq * args.steps / total
     roof, t_join, evals = roofline(kj, stats, nq, nr, d, k)
@@ -184,29 +248,64 @@ def bench_single(args, kj, cfg):
         "kcap": st["kcap"], "tile_q": st["tile_q"], "n_overflow": st["n_overflow"],
         "exact_mode": st["exact_mode"], "merges": st["n_merges"], "appended": st["n_appended"],
         "filter_max_abs_err": st["filter_max_abs_err"], "autotune_choice": st["tuned"],
+        "filter_error_vs_fp64": filter_error(st),
+        "rerun_stages": st["rerun_stages"],
     }
     if args.config == "c3" and args.precision == "auto" and not args.no_compare:
         # the plain FP32 join of the same inputs, timed the same way
         fsteps = max(2, args.steps // 2)
-        ftotal, ftimes, fstats = time_joins(ctx, Rd, Qd, k, fsteps, 1, dict(opts, precision=kj.PREC_FP32))
+        ftotal, ftimes, fstats, _ = time_joins(ctx, Rd, Qd, k, fsteps, 1, dict(opts, precision=kj.PREC_FP32))
         fvalue = nq * fsteps / ftotal
         froof, ft_join, _ = roofline(kj, fstats, nq, nr, d, k)
         out["plain_fp32"] = {"value": fvalue, "steps": fsteps, "ms_per_step": ftotal * 1e3 / fsteps,
                              "p99_ms": float(np.percentile(ftimes, 99)), "join_ms": ft_join,
-                             "roofline_frac": froof["frac"], "achieved_tflops": froof["achieved"]}
+                             "roofline_frac": froof["frac"], "achieved_tflops": froof["achieved"],
+                             "filter_error_vs_fp64": filter_error(fstats[-1])}
         out["bf16_vs_fp32"] = value / fvalue
         # and the plain FP32 join on the vector ALUs (direct-form distances,
         # no matrix cores), the same index and bounds
-        vtotal, vtimes, vstats = time_joins(ctx, Rd, Qd, k, fsteps, 1,
-                                            dict(opts, precision=kj.PREC_FP32, kernel=kj.KERNEL_VALU))
+        vtotal, vtimes, vstats, _ = time_joins(ctx, Rd, Qd, k, fsteps, 1,
+                                               dict(opts, precision=kj.PREC_FP32, kernel=kj.KERNEL_VALU))
         vvalue = nq * fsteps / vtotal
         _, vt_join, _ = roofline(kj, vstats, nq, nr, d, k)
         out["plain_fp32_valu"] = {"value": vvalue, "steps": fsteps, "ms_per_step": vtotal * 1e3 / fsteps,
-                                  "p99_ms": float(np.percentile(vtimes, 99)), "join_ms": vt_join}
+                                  "p99_ms": float(np.percentile(vtimes, 99)), "join_ms": vt_join,
+                                  "filter_error_vs_fp64": filter_error(vstats[-1])}
         out["bf16_vs_fp32_valu"] = value / vvalue
+    if args.config == "c2" and not args.no_compare:
+        # the same plain FP32 join on the vector ALUs (direct-form tiles)
+        fsteps = max(2, args.steps // 2)
+        vopts = dict(opts, precision=kj.PREC_FP32, kernel=kj.KERNEL_VALU)
+        vtotal, vtimes, vstats, _ = time_joins(ctx, Rd, Qd, k, fsteps, 1, vopts)
+        vvalue = nq * fsteps / vtotal
+        vroof, vt_join, _ = roofline(kj, vstats, nq, nr, d, k)
+        out["plain_fp32_valu"] = {"value": vvalue, "steps": fsteps, "ms_per_step": vtotal * 1e3 / fsteps,
+                                  "p99_ms": float(np.percentile(vtimes, 99)), "join_ms": vt_join,
+                                  "achieved_tflops": vroof["achieved"], "roofline_frac": vroof["frac"],
+                                  "pruning_ratio": vstats[-1]["pruning_ratio"],
+                                  "filter_error_vs_fp64": filter_error(vstats[-1])}
+        out["mfma_vs_valu"] = value / vvalue
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(R, Q, k)
+        rows = np.random.default_rng(2015).permutation(Q.shape[0])
+        cb, (sel, ref) = cpu_baseline(R, Q, k, rows=rows)
+        out["cpu_baseline"] = cb
+        idx, dist = last
+        out["parity"] = parity_check(sel, ref, _to_host(idx), _to_host(dist))
     return out
+
+
+def _to_host(t):
+    return t.cpu().numpy() if hasattr(t, "cpu") else np.asarray(t)
+
+
+def filter_error(st) -> dict:
+    """The filter's squared-distance error against binary64 over the
+    re-scored candidates, apart from the bound that widened the thresholds."""
+    if st["exact_mode"]:
+        return {"exact_integer_keys": True, "max_abs_err": 0.0}
+    return {"max_abs_err": st["filter_err_max"], "max_bound": st["filter_bound_max"],
+            "max_err_over_bound": st["filter_err_ratio"],
+            "key_minus_fp64_max": st["filter_max_abs_err"]}
 
 
 def bench_batch(args, kj, cfg):
@@ -260,11 +359,16 @@ def bench_batch(args, kj, cfg):
     lt = time.perf_counter() - t0
     out["per_instance_loop"] = {"value": nq * lsteps / lt, "ms_per_step": lt * 1e3 / lsteps, "steps": lsteps}
     out["batched_vs_loop"] = value / out["per_instance_loop"]["value"]
+    out["filter_error_vs_fp64"] = filter_error(st)
     if not args.no_cpu:
-        R0, Q0 = max(inst, key=lambda p: p[0].shape[0])
-        cb = cpu_baseline(R0, Q0, k, target_s=6.0)
+        big = max(range(len(inst)), key=lambda i: inst[i][0].shape[0])
+        R0, Q0 = inst[big]
+        rows = np.random.default_rng(2019).permutation(Q0.shape[0])
+        cb, (sel, ref) = cpu_baseline(R0, Q0, k, target_s=6.0, ti_target_s=4.0, rows=rows)
         cb["sample"] = "largest instance: " + cb["sample"]
         out["cpu_baseline"] = cb
+        out["parity"] = parity_check(sel, ref, _to_host(outs[big][0]), _to_host(outs[big][1]))
+        out["parity"]["instance"] = int(big)
     return out
 
 
@@ -307,8 +411,11 @@ def bench_serving(args, kj, cfg):
         if dt > 0:
             time.sleep(dt)
         futs.append(pool.submit(srv.wait, srv.submit(q)))
+    answers = []
     for f in futs:
-        _, _, inf = f.result()
+        a_idx, a_dist, inf = f.result()
+        if len(answers) < 8:
+            answers.append((a_idx, a_dist))
         lat.append(inf["latency_ms"])
         qms.append(inf["queue_ms"])
         fused.append(inf["batch_requests"])
@@ -335,8 +442,12 @@ def bench_serving(args, kj, cfg):
     }
     if not args.no_cpu:
         Q = np.concatenate(reqs[:8])
-        cb = cpu_baseline(R, Q, k, target_s=10.0)
+        rows = np.random.default_rng(2020).permutation(Q.shape[0])
+        cb, (sel, ref) = cpu_baseline(R, Q, k, target_s=10.0, rows=rows)
+        cb["sample"] = "the first 8 requests' rows: " + cb["sample"]
         out["cpu_baseline"] = cb
+        out["parity"] = parity_check(sel, ref, np.concatenate([a for a, _ in answers]),
+                                     np.concatenate([b for _, b in answers]))
     ix.close()
     return out
 

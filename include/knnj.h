@@ -107,7 +107,14 @@ typedef struct {
                           HIP graph and replay it while the plan (every pointer, size and
                           decision) is unchanged; the input scan and overflow re-runs stay
                           outside it */
-  int32_t reserved[14];
+  int32_t kcap;        /* candidate capacity per query of the first pass: 0 = auto, else a
+                          power of two in [64, 1024], >= k (queries whose candidate lists
+                          overflow are re-run at larger capacities, so any value gives the
+                          same answer) */
+  int32_t kcap_mfma_max; /* largest capacity of the MFMA kernel's overflow re-runs before the
+                          VALU kernel at 1024 takes the rest: 0 = 512, else a power of two in
+                          [64, 512] */
+  int32_t reserved[12];
 } knnj_opts;
 
 /* Device-side timings (hipEvents on the context's stream) and counters. */
@@ -128,15 +135,23 @@ typedef struct {
   int32_t kcap;             /* candidate capacity per query used */
   int64_t n_tiles;          /* query tiles launched */
   double filter_max_abs_err;/* largest |list key - binary64 distance^2| over re-scored
-                               candidates (VALU: the FP32 filter error; MFMA: the
-                               filter error plus the bound's width) */
+                               candidates (VALU: the FP32 filter error; MFMA: the key is the
+                               upper bound s_f + E, so this is the filter error plus the
+                               bound's width; filter_err_max below is the error alone) */
   int32_t kernel_used;      /* knnj_kernel actually run */
   int32_t n_launches;       /* join kernel launches (knnj_join_batch: fused launches) */
   int64_t n_merges;         /* k-select buffer merges */
   int64_t n_appended;       /* candidates that passed a query's threshold */
   int32_t tuned;            /* autotune: 0 off, else 1 + the candidate the context chose */
   int32_t graph;            /* knnj_join_batch: 0 eager, 1 captured this call, 2 replayed */
-  int32_t reserved[2];
+  int32_t rerun_stages;     /* overflow re-run stages executed (0: none; each stage doubles the
+                               capacity on the MFMA kernel, the last one is the VALU kernel at 1024) */
+  int32_t reserved0;
+  double filter_err_max;    /* the filter value's own error: largest |s_f - s64| over re-scored
+                               candidates, s_f the filter's squared distance and s64 the binary64
+                               one (0 in exact mode) */
+  double filter_bound_max;  /* largest rigorous error bound E attached to a re-scored candidate */
+  double filter_err_ratio;  /* largest |s_f - s64| / E over them: <= 1 when the bound holds */
 } knnj_stats;
 
 /* One independent join of a batch (knnj_join_batch). */
@@ -183,6 +198,48 @@ knnj_status knnj_join(knnj_ctx *ctx, knnj_matrix R, knnj_matrix Q, int32_t k,
  * builds, bounds and tile lists. */
 knnj_status knnj_join_batch(knnj_ctx *ctx, int32_t n_inst, const knnj_instance *inst,
                             const knnj_opts *opts, knnj_stats *stats);
+
+/* ---- Introspection of the last join (tests and diagnostics) -------------
+ * After knnj_join or knnj_index_query returns, the context still holds that
+ * join's cluster index and bounds until its next call: the parts the
+ * reference keeps in R2all_static / R2all_dyn (include/common8.h:50-69) and
+ * in its replists (knnjoin.cu:506-525, 1076-1094).  They let a caller check
+ * the pruning invariants the join's exactness rests on (radius = largest
+ * member distance, theta_A >= every member query's true k-th distance, no
+ * true neighbour in a pruned cluster pair). */
+typedef struct {
+  int64_t n_r, n_q;    /* rows of each side */
+  int32_t d, dpad;     /* dimension; row stride of the representative rows */
+  int32_t m_r, m_q;    /* clusters per side */
+  int32_t k;
+  int32_t self_join;   /* 1: the query side is the reference side's index */
+  int32_t valid;       /* 0: the context has run no single join yet */
+  int32_t reserved[5];
+} knnj_plan_desc;
+
+typedef enum {
+  KNNJ_PART_R_REPS = 0,    /* float [m_r][dpad]: representative rows (zero padded) */
+  KNNJ_PART_R_RADIUS = 1,  /* double [m_r]: largest member distance to the representative */
+  KNNJ_PART_R_OFFSETS = 2, /* int32 [m_r + 1]: cluster b holds positions off[b] .. off[b+1]-1 */
+  KNNJ_PART_R_PERM = 3,    /* int32 [n_r]: position -> original row */
+  KNNJ_PART_R_PDIST = 4,   /* float [n_r]: distance to the representative, ascending per cluster */
+  KNNJ_PART_Q_REPS = 5,    /* the same five parts of the query side */
+  KNNJ_PART_Q_RADIUS = 6,
+  KNNJ_PART_Q_OFFSETS = 7,
+  KNNJ_PART_Q_PERM = 8,
+  KNNJ_PART_Q_PDIST = 9,
+  KNNJ_PART_THETA = 10,     /* double [m_q]: theta_A, an upper bound of the k-th neighbour
+                               distance of every query of cluster A */
+  KNNJ_PART_PAIR_COUNT = 11,/* int32 [m_q]: surviving reference clusters of A */
+  KNNJ_PART_PAIR_LIST = 12, /* int32 [m_q][m_r]: their ids (first count entries of row A) */
+  KNNJ_PART_PAIR_LB = 13    /* float [m_q][m_r]: lower bound of d(a, s) over the members s of
+                               each listed cluster, ascending along the list */
+} knnj_plan_part;
+
+knnj_status knnj_ctx_last_plan(knnj_ctx *ctx, knnj_plan_desc *out);
+/* Copies part `part` of the last join's plan to host memory `dst` of
+ * `bytes` bytes (exactly the part's size, see knnj_plan_part). */
+knnj_status knnj_ctx_copy_plan_part(knnj_ctx *ctx, int32_t part, void *dst, int64_t bytes);
 
 /* ---- Resident reference index (the serving path) ----------------------
  * The reference builds R's clusters inside every run (knnjoin.cu:1043-1068:
@@ -244,6 +301,9 @@ typedef struct {
   int32_t slot;
   double queue_ms;         /* submit -> its batch started */
   double latency_ms;       /* submit -> results written */
+  double t_submit_ms;      /* submit, batch start and batch end on the server's clock */
+  double t_start_ms;       /*   (milliseconds since knnj_server_create), so callers can */
+  double t_end_ms;         /*   see which batches of different slots overlapped */
   char message[160];       /* error text when status != KNNJ_OK */
 } knnj_request_info;
 
@@ -253,7 +313,8 @@ knnj_status knnj_server_create(const knnj_index *index, const knnj_server_config
 knnj_status knnj_server_submit(knnj_server *srv, const float *Q, int64_t n, int64_t ld, int32_t *out_idx,
                                float *out_dist, uint64_t *ticket);
 /* Blocks until the request completes; returns its status.  Each ticket is
- * waited for once. */
+ * waited for once: a second wait on the same ticket (also one that races
+ * the first from another thread) returns KNNJ_EINVAL. */
 knnj_status knnj_server_wait(knnj_server *srv, uint64_t ticket, knnj_request_info *info);
 /* Fused batches run and rows served so far. */
 knnj_status knnj_server_counters(knnj_server *srv, int64_t *batches, int64_t *rows);

@@ -49,7 +49,8 @@ class Opts(ctypes.Structure):
                 ("kernel", ctypes.c_int32), ("refine_iters", ctypes.c_int32),
                 ("xcd_tiles", ctypes.c_int32), ("tile_waves", ctypes.c_int32),
                 ("tile_split", ctypes.c_int32), ("split_rounds", ctypes.c_int32),
-                ("autotune", ctypes.c_int32), ("graph", ctypes.c_int32), ("reserved", ctypes.c_int32 * 14)]
+                ("autotune", ctypes.c_int32), ("graph", ctypes.c_int32), ("kcap", ctypes.c_int32),
+                ("kcap_mfma_max", ctypes.c_int32), ("reserved", ctypes.c_int32 * 12)]
 
 
 class Stats(ctypes.Structure):
@@ -63,7 +64,9 @@ class Stats(ctypes.Structure):
                 ("filter_max_abs_err", ctypes.c_double), ("kernel_used", ctypes.c_int32),
                 ("n_launches", ctypes.c_int32), ("n_merges", ctypes.c_int64),
                 ("n_appended", ctypes.c_int64), ("tuned", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("rerun_stages", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("filter_err_max", ctypes.c_double), ("filter_bound_max", ctypes.c_double),
+                ("filter_err_ratio", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_
@@ -90,13 +93,41 @@ class RequestInfo(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("batch_rows", ctypes.c_int32),
                 ("batch_requests", ctypes.c_int32), ("slot", ctypes.c_int32),
                 ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
-                ("message", ctypes.c_char * 160)]
+                ("t_submit_ms", ctypes.c_double), ("t_start_ms", ctypes.c_double),
+                ("t_end_ms", ctypes.c_double), ("message", ctypes.c_char * 160)]
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [("n_r", ctypes.c_int64), ("n_q", ctypes.c_int64), ("d", ctypes.c_int32),
+                ("dpad", ctypes.c_int32), ("m_r", ctypes.c_int32), ("m_q", ctypes.c_int32),
+                ("k", ctypes.c_int32), ("self_join", ctypes.c_int32), ("valid", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 5)]
+
+
+# parts of the last join's plan (knnj_plan_part): name -> (id, dtype, shape from the desc)
+PLAN_PARTS = {
+    "r_reps": (0, np.float32, lambda p: (p["m_r"], p["dpad"])),
+    "r_radius": (1, np.float64, lambda p: (p["m_r"],)),
+    "r_offsets": (2, np.int32, lambda p: (p["m_r"] + 1,)),
+    "r_perm": (3, np.int32, lambda p: (p["n_r"],)),
+    "r_pdist": (4, np.float32, lambda p: (p["n_r"],)),
+    "q_reps": (5, np.float32, lambda p: (p["m_q"], p["dpad"])),
+    "q_radius": (6, np.float64, lambda p: (p["m_q"],)),
+    "q_offsets": (7, np.int32, lambda p: (p["m_q"] + 1,)),
+    "q_perm": (8, np.int32, lambda p: (p["n_q"],)),
+    "q_pdist": (9, np.float32, lambda p: (p["n_q"],)),
+    "theta": (10, np.float64, lambda p: (p["m_q"],)),
+    "pair_count": (11, np.int32, lambda p: (p["m_q"],)),
+    "pair_list": (12, np.int32, lambda p: (p["m_q"], p["m_r"])),
+    "pair_lb": (13, np.float32, lambda p: (p["m_q"], p["m_r"])),
+}
 
 
 EXPORTS = ("knnj_version", "knnj_last_error", "knnj_ctx_create", "knnj_ctx_destroy",
            "knnj_ctx_set_budget", "knnj_join", "knnj_join_batch", "knnj_index_build",
            "knnj_index_destroy", "knnj_index_info", "knnj_index_query", "knnj_server_create",
-           "knnj_server_submit", "knnj_server_wait", "knnj_server_counters", "knnj_server_destroy")
+           "knnj_server_submit", "knnj_server_wait", "knnj_server_counters", "knnj_server_destroy",
+           "knnj_ctx_last_plan", "knnj_ctx_copy_plan_part")
 
 _lib = None
 
@@ -142,6 +173,10 @@ def load_library(path: str = LIB_PATH):
     lib.knnj_server_counters.restype = ctypes.c_int
     lib.knnj_server_destroy.argtypes = [vp]
     lib.knnj_server_destroy.restype = None
+    lib.knnj_ctx_last_plan.argtypes = [vp, ctypes.POINTER(PlanDesc)]
+    lib.knnj_ctx_last_plan.restype = ctypes.c_int
+    lib.knnj_ctx_copy_plan_part.argtypes = [vp, i32, vp, i64]
+    lib.knnj_ctx_copy_plan_part.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -188,10 +223,11 @@ def _outputs(nq: int, k: int, aos: bool, qdev):
 
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
               brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0, xcd_tiles=False,
-              tile_waves=0, tile_split=0, split_rounds=0, autotune=False, graph=False) -> Opts:
+              tile_waves=0, tile_split=0, split_rounds=0, autotune=False, graph=False, kcap=0,
+              kcap_mfma_max=0) -> Opts:
     return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
                 tile_q, kernel, refine_iters, int(xcd_tiles), tile_waves, tile_split, split_rounds,
-                int(autotune), int(graph))
+                int(autotune), int(graph), kcap, kcap_mfma_max)
 
 
 class Context:
@@ -230,6 +266,19 @@ class Context:
         _check(_lib.knnj_join(self._h, mR, mQ, k, ctypes.byref(o), pi, pd, ctypes.byref(st)))
         del keepR, keepQ
         return idx, dist, st.as_dict()
+
+    def last_plan(self) -> dict:
+        """Sizes of the last single join's cluster index and bounds (knnj_ctx_last_plan)."""
+        v = PlanDesc()
+        _check(_lib.knnj_ctx_last_plan(self._h, ctypes.byref(v)))
+        return {name: getattr(v, name) for name, _ in v._fields_ if name != "reserved"}
+
+    def plan_part(self, name: str) -> np.ndarray:
+        """One part of the last single join's plan (see PLAN_PARTS) as a host array."""
+        pid, dt, shape = PLAN_PARTS[name]
+        out = np.empty(shape(self.last_plan()), dtype=dt)
+        _check(_lib.knnj_ctx_copy_plan_part(self._h, pid, out.ctypes.data, out.nbytes))
+        return out
 
     def build_index(self, R, **opts) -> "Index":
         """A resident index of R on this context's device (see Index)."""

@@ -34,8 +34,31 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def _compile(src: str) -> str:
+def _headers_of(src: str, seen=None) -> set:
+    """Local headers `src` includes, recursively (quoted includes only)."""
+    seen = set() if seen is None else seen
+    with open(src) as f:
+        for line in f:
+            line = line.strip()
+            if line.startswith('#include "'):
+                h = os.path.normpath(os.path.join(os.path.dirname(src), line.split('"')[1]))
+                if os.path.exists(h) and h not in seen:
+                    seen.add(h)
+                    _headers_of(h, seen)
+    return seen
+
+
+def _stale(src: str, obj: str) -> bool:
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(f) > t for f in [src, __file__, *_headers_of(src)])
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o"))
+    if incremental and not _stale(src, obj):
+        return obj
     cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -43,11 +66,15 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compiles every source (incremental: only those whose object is older
+    than the source or a header it includes) and links libknnj.so."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = sources()
+    # the slowest translation units first
+    srcs.sort(key=lambda p: -os.path.getsize(p) - (1 << 20 if "k_join_mfma_d" in p else 0))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda p: _compile(p, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -76,5 +103,5 @@ def build_cli() -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -75,6 +75,17 @@ struct knnj_ctx {
   // of the last plan, and that plan's signature
   hipGraphExec_t batch_exec = nullptr;
   std::vector<unsigned char> batch_sig;
+  // the last single join's index and bounds (knnj_ctx_last_plan): device
+  // pointers into this context's workspace or a resident index
+  struct Plan {
+    bool valid = false;
+    int64_t n_r = 0, n_q = 0;
+    int d = 0, dpad = 0, m_r = 0, m_q = 0, k = 0, self = 0;
+    SideIndex Ri{}, Qi{};
+    const double *theta = nullptr;
+    const int32_t *plist = nullptr, *pcnt = nullptr;
+    const float *plb = nullptr;
+  } plan;
 };
 
 #define HIPCHK(expr)                                                                      \
@@ -256,6 +267,15 @@ knnj_status check_join_args(const knnj_matrix &R, const knnj_matrix &Q, int32_t 
   }
   if ((o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1) && o.kernel == KNNJ_KERNEL_VALU) {
     set_err("the BF16 filter runs on the MFMA kernel only");
+    return KNNJ_EINVAL;
+  }
+  const auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+  if (o.kcap != 0 && (!pow2(o.kcap) || o.kcap < 64 || o.kcap > KNNJ_MAX_K || o.kcap < k)) {
+    set_err("kcap must be 0 or a power of two in [64, %d] and >= k", KNNJ_MAX_K);
+    return KNNJ_EINVAL;
+  }
+  if (o.kcap_mfma_max != 0 && (!pow2(o.kcap_mfma_max) || o.kcap_mfma_max < 64 || o.kcap_mfma_max > 512)) {
+    set_err("kcap_mfma_max must be 0 or a power of two in [64, 512]");
     return KNNJ_EINVAL;
   }
   return KNNJ_OK;
@@ -452,6 +472,7 @@ knnj_status decide(Inst &I, const u32 *flags) {
   int kcap = I.exact    ? pow2_at_least(std::max(k, 64))
              : I.bf16x1 ? pow2_at_least(std::max(2 * k, 64))
                         : pow2_at_least(std::max(k + std::max(8, k / 8), 64));
+  if (o.kcap > 0) kcap = o.kcap;  // the caller's first-pass capacity (>= k, checked)
   kcap = std::min(kcap, I.mfma ? 512 : KNNJ_MAX_K);
   if (kcap < k) kcap = KNNJ_MAX_K;  // only the VALU kernel holds k > 512
   I.kcap = kcap;
@@ -662,6 +683,8 @@ void fill_args(Inst &I) {
   a.ovf_tiles = I.ovf_tiles;
   a.ovf_count = I.counters + 1;
   a.max_err_bits = (u32 *)(I.counters + 3);
+  a.ferr_bits = (u32 *)(I.counters + 5);
+  a.relbound = (!I.use_mfma && !a.exact) ? 1 : 0;
   a.rc = I.rc;
   a.nr = I.nr;
   a.reps32 = I.reps32;
@@ -703,13 +726,15 @@ hipError_t launch_main(const Inst &I, int64_t grid, hipStream_t s) {
 // that capacity's LDS still allows several waves, they split each query's
 // cluster list), then on the VALU kernel at 1024; KNNJ_EOVERFLOW only if
 // that overflows too.  Each stage's overflows feed the next.
-knnj_status rerun_overflow(Inst &I, int32_t novf, hipStream_t s) {
+knnj_status rerun_overflow(Inst &I, int32_t novf, hipStream_t s, int *stages) {
   Tile *in = I.ovf_tiles, *out = I.ovf2_tiles;
   int32_t *in_cnt = I.counters + 1, *out_cnt = I.counters + 2, *spare = I.counters + 4;
   int kc = I.kcap;
   bool mfma = I.use_mfma;
+  const int mfma_max = I.o.kcap_mfma_max > 0 ? I.o.kcap_mfma_max : 512;
   int32_t n = novf;
   while (n > 0) {
+    ++*stages;
     JoinArgs b = I.args;
     b.tiles = in;
     b.ntiles = in_cnt;
@@ -720,8 +745,8 @@ knnj_status rerun_overflow(Inst &I, int32_t novf, hipStream_t s) {
     b.waves = 1;  // one-query tiles
     b.split = 1;
     HIPCHK(hipMemsetAsync(out_cnt, 0, sizeof(int32_t), s));
-    if (mfma && kc < 512) {
-      kc = std::min(512, 2 * kc);
+    if (mfma && kc < mfma_max) {
+      kc = std::min(mfma_max, 2 * kc);
       b.kcap = kc;
       const int sw = join_mfma_split_waves(kc, I.dk);
       if (sw > 1) b.split = b.waves = sw;
@@ -735,6 +760,7 @@ knnj_status rerun_overflow(Inst &I, int32_t novf, hipStream_t s) {
       b.kcap = kc;
       b.exact = I.exact_valu ? 1 : 0;
       b.ecode = 0;
+      b.relbound = b.exact ? 0 : 1;
       HIPCHK(launch_join(b, 1, n, s));
     } else {
       set_err("%d queries overflowed the largest candidate capacity (%d)", n, KNNJ_MAX_K);
@@ -839,6 +865,7 @@ namespace {
 knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
   hipStream_t s = c->stream;
   hipEvent_t *ev = c->ev;
+  c->plan.valid = false;  // set again once this join's plan is complete
   knnj_stats st{};
   plan_stats(I, st);
   if (I.nQ == 0) {
@@ -860,7 +887,7 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
     float *dg;
     TRY(ws(c, "tile_prof", (size_t)I.max_slots * 8, &tp));
     TRY(ws(c, "ovf_diag", 256, &dg));
-    HIPCHK(hipMemsetAsync(tp, 0, sizeof(u64) * 4 * I.max_slots, s));
+    HIPCHK(hipMemsetAsync(tp, 0, sizeof(u64) * 8 * I.max_slots, s));
     HIPCHK(hipMemsetAsync(dg, 0, sizeof(float) * 256, s));
     I.args.tile_prof = tp;
     I.args.ovf_diag = dg;
@@ -877,13 +904,16 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
   st.n_launches = 1;
   if (tile_prof) TRY(print_tile_profile(I, nslots, hc[1]));
   HIPCHK(hipEventRecord(ev[5], s));
-  if (hc[1] > 0) TRY(rerun_overflow(I, hc[1], s));
+  int stages = 0;
+  if (hc[1] > 0) TRY(rerun_overflow(I, hc[1], s, &stages));
+  st.rerun_stages = stages;
   HIPCHK(hipEventRecord(ev[6], s));
   TRY(copy_outputs(I, s));
   u64 hcnt[4] = {0, 0, 0, 0};
-  u32 herr = 0;
+  u32 herr = 0, hferr[3] = {0, 0, 0};
   HIPCHK(hipMemcpyAsync(hcnt, I.evals, sizeof(hcnt), hipMemcpyDeviceToHost, s));
   HIPCHK(hipMemcpyAsync(&herr, I.counters + 3, sizeof(herr), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(hferr, I.counters + 5, sizeof(hferr), hipMemcpyDeviceToHost, s));
   HIPCHK(hipEventRecord(ev[7], s));
   HIPCHK(hipStreamSynchronize(s));
 
@@ -897,9 +927,31 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
   st.n_appended = (int64_t)hcnt[2];
   const double tot = (double)I.nR * (double)I.nQ;
   st.pruning_ratio = tot > 0 ? 1.0 - (double)hcnt[0] / tot : 0.0;
-  float ferr;
+  float ferr, fe3[3];
   std::memcpy(&ferr, &herr, 4);
+  std::memcpy(fe3, hferr, sizeof(fe3));
   st.filter_max_abs_err = ferr;
+  st.filter_err_max = fe3[0];
+  st.filter_bound_max = fe3[1];
+  st.filter_err_ratio = fe3[2];
+  {
+    knnj_ctx::Plan &P = c->plan;
+    P.valid = true;
+    P.n_r = I.nR;
+    P.n_q = I.nQ;
+    P.d = I.d;
+    P.dpad = I.dpad;
+    P.m_r = I.m_r;
+    P.m_q = I.m_q;
+    P.k = I.k;
+    P.self = I.self ? 1 : 0;
+    P.Ri = I.Ri;
+    P.Qi = I.Qi;
+    P.theta = I.theta;
+    P.plist = I.plist;
+    P.pcnt = I.pcnt;
+    P.plb = I.plb;
+  }
   {
     std::vector<int32_t> hp(I.m_q);
     HIPCHK(hipMemcpy(hp.data(), I.pcnt, sizeof(int32_t) * I.m_q, hipMemcpyDeviceToHost));
@@ -1553,10 +1605,10 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   HIPCHK(hipEventRecord(ev[4], s));
   // counters of every join in one copy: the counter and evaluation blocks
   // sit in the arena, gathered here per join
-  std::vector<int32_t> hc((size_t)4 * n_inst, 0);
+  std::vector<int32_t> hc((size_t)8 * n_inst, 0);
   std::vector<u64> hcnt((size_t)4 * n_inst, 0);
   for (int id : order) {
-    HIPCHK(hipMemcpyAsync(&hc[4 * id], I[id].counters, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&hc[8 * id], I[id].counters, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&hcnt[4 * id], I[id].evals, 4 * sizeof(u64), hipMemcpyDeviceToHost, s));
   }
   HIPCHK(hipStreamSynchronize(s));
@@ -1566,22 +1618,29 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   double tot = 0.0, ferr = 0.0;
   for (int id : order) {
     Inst &x = I[id];
-    if (hc[4 * id + 1] > 0) {
-      const knnj_status e = rerun_overflow(x, hc[4 * id + 1], s);
+    if (hc[8 * id + 1] > 0) {
+      int stages = 0;
+      const knnj_status e = rerun_overflow(x, hc[8 * id + 1], s, &stages);
+      st.rerun_stages = std::max(st.rerun_stages, stages);
       if (e != KNNJ_OK) {
         prefix_err("instance %d: ", id);
         return e;
       }
     }
-    novf += hc[4 * id + 1];
+    novf += hc[8 * id + 1];
     TRY(copy_outputs(x, s));
     evals += (int64_t)hcnt[4 * id];
     merges += (int64_t)hcnt[4 * id + 1];
     appended += (int64_t)hcnt[4 * id + 2];
     tot += (double)x.nR * (double)x.nQ;
     float fe;
-    std::memcpy(&fe, &hc[4 * id + 3], 4);
+    std::memcpy(&fe, &hc[8 * id + 3], 4);
     ferr = std::max(ferr, (double)fe);
+    float f3[3];
+    std::memcpy(f3, &hc[8 * id + 5], sizeof(f3));
+    st.filter_err_max = std::max(st.filter_err_max, (double)f3[0]);
+    st.filter_bound_max = std::max(st.filter_bound_max, (double)f3[1]);
+    st.filter_err_ratio = std::max(st.filter_err_ratio, (double)f3[2]);
   }
   HIPCHK(hipEventRecord(ev[6], s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1603,6 +1662,78 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   st.graph = graph_state;
   (void)n_launch;
   if (stats) *stats = st;
+  return KNNJ_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+knnj_status knnj_ctx_last_plan(knnj_ctx *c, knnj_plan_desc *out) {
+  g_err.clear();
+  if (!c || !out) {
+    set_err("context or output pointer is NULL");
+    return KNNJ_EINVAL;
+  }
+  const knnj_ctx::Plan &P = c->plan;
+  knnj_plan_desc v{};
+  v.valid = P.valid ? 1 : 0;
+  v.n_r = P.n_r;
+  v.n_q = P.n_q;
+  v.d = P.d;
+  v.dpad = P.dpad;
+  v.m_r = P.m_r;
+  v.m_q = P.m_q;
+  v.k = P.k;
+  v.self_join = P.self;
+  *out = v;
+  return KNNJ_OK;
+}
+
+knnj_status knnj_ctx_copy_plan_part(knnj_ctx *c, int32_t part, void *dst, int64_t bytes) {
+  g_err.clear();
+  if (!c || !dst) {
+    set_err("context or destination is NULL");
+    return KNNJ_EINVAL;
+  }
+  const knnj_ctx::Plan &P = c->plan;
+  if (!P.valid) {
+    set_err("the context has run no single join");
+    return KNNJ_EINVAL;
+  }
+  const void *src = nullptr;
+  size_t n = 0;
+  const bool q = part >= KNNJ_PART_Q_REPS && part <= KNNJ_PART_Q_PDIST;
+  const SideIndex &S = q ? P.Qi : P.Ri;
+  const int side = q ? part - KNNJ_PART_Q_REPS : part;
+  if (part >= KNNJ_PART_R_REPS && part <= KNNJ_PART_Q_PDIST) {
+    switch (side) {
+      case 0: src = S.reps, n = sizeof(float) * (size_t)S.m * S.dpad; break;
+      case 1: src = S.rad, n = sizeof(double) * (size_t)S.m; break;
+      case 2: src = S.off, n = sizeof(int32_t) * ((size_t)S.m + 1); break;
+      case 3: src = S.perm, n = sizeof(int32_t) * (size_t)S.n; break;
+      case 4: src = S.pdist, n = sizeof(float) * (size_t)S.n; break;
+    }
+  } else if (part == KNNJ_PART_THETA) {
+    src = P.theta, n = sizeof(double) * (size_t)P.m_q;
+  } else if (part == KNNJ_PART_PAIR_COUNT) {
+    src = P.pcnt, n = sizeof(int32_t) * (size_t)P.m_q;
+  } else if (part == KNNJ_PART_PAIR_LIST) {
+    src = P.plist, n = sizeof(int32_t) * (size_t)P.m_q * P.m_r;
+  } else if (part == KNNJ_PART_PAIR_LB) {
+    src = P.plb, n = sizeof(float) * (size_t)P.m_q * P.m_r;
+  } else {
+    set_err("unknown plan part %d", part);
+    return KNNJ_EINVAL;
+  }
+  if ((int64_t)n != bytes) {
+    set_err("plan part %d holds %zu bytes, %lld asked for", part, n, (long long)bytes);
+    return KNNJ_EINVAL;
+  }
+  if (n == 0) return KNNJ_OK;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
   return KNNJ_OK;
 }
 

@@ -41,7 +41,10 @@
 // host.  Integer inputs with 4 d (2 max|x|)^2 <
 // 2^24 make every FP32 step exact (E = 0, exact mode).
 //
-// List keys are (U = s_f + E, row).  theta_q is the k-th smallest U; a
+// List keys are (U = s_f + E, row), where E is the value of the entry's
+// error code (the bound rounded up to the code's grid, so the rerank can
+// recover s_f = U - E and report the filter's own error); without codes
+// E is the bound itself.  theta_q is the k-th smallest U; a
 // candidate passes when L = s_f - E <= theta_q.  E_max(q) is the largest E of
 // any candidate appended for the query (an entry in the list or evicted from
 // it was appended, so its L >= U - 2 E_max); at the end every list entry with
@@ -423,7 +426,9 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
           for (int i = 0; i < 4; ++i) nqr[i] = fsh[4 * g + i];
           wave_sync();
 
-          float Ei[4], dqbv[4];
+          // Ei: the rigorous bound; Ed: the bound the key carries (its code's
+          // decoded value, >= Ei, with per-entry codes; Ei without)
+          float Ei[4], Ed[4], dqbv[4];
           u32 ecd[4];
           float lo = INFINITY, hi = -INFINITY;
   #pragma unroll
@@ -433,7 +438,9 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
             const float thi = sqrtf(key_dist(thr[i])) * (1.f + eps);
             use[i] = use[i] && !(dqb * (1.f - 2.f * eps) - Rb > thi);
             Ei[i] = EXACT ? 0.f : a.c1 * dqb * (1.f + eps) * Rb + a.c2 * (nqr[i] + Rb * Rb);
-            ecd[i] = (!EXACT && a.ecode) ? (ecode_encode(Ei[i], ec) << ec.rowbits) : 0u;
+            const u32 code = (!EXACT && a.ecode) ? ecode_encode(Ei[i], ec) : 0u;
+            ecd[i] = code << ec.rowbits;
+            Ed[i] = (!EXACT && a.ecode) ? ecode_decode(code, ec) : Ei[i];
             if (use[i]) {
               lo = fminf(lo, dqb * (1.f - 2.f * eps) - thi);
               hi = fmaxf(hi, dqb * (1.f + 2.f * eps) + thi);
@@ -502,7 +509,9 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                   Ei[i] = a.c1 * dqbv[i] * (1.f + eps) * Rg + a.c2 * (nqr[i] + Rg * Rg);
-                  ecd[i] = a.ecode ? (ecode_encode(Ei[i], ec) << ec.rowbits) : 0u;
+                  const u32 code = a.ecode ? ecode_encode(Ei[i], ec) : 0u;
+                  ecd[i] = code << ec.rowbits;
+                  Ed[i] = a.ecode ? ecode_decode(code, ec) : Ei[i];
                 }
               }
               // MFMA stage: the NB blocks are independent accumulation
@@ -554,7 +563,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
                     key[i] = make_key(sf, row);
                     pass[i] = valid && use[i] && key[i] <= thr[i];
                   } else {
-                    key[i] = make_key(sf + Ei[i], row | ecd[i]);
+                    key[i] = make_key(sf + Ed[i], row | ecd[i]);
                     pass[i] = valid && use[i] && (sf - Ei[i]) <= key_dist(thr[i]);
                   }
                   anyp |= pass[i];
@@ -570,7 +579,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
                     cnt[i] += __popc(gm);
                     npass += __popcll(mk);
                     // every entry ever appended for this row has E <= emax
-                    if (gm) emax[i] = fmaxf(emax[i], Ei[i]);
+                    if (gm) emax[i] = fmaxf(emax[i], Ed[i]);
                   }
                   // a 16-column block appends at most 16 keys per query, so a
                   // buffer below MF_BC - 16 has room for the next block

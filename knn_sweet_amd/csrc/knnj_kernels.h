@@ -65,6 +65,8 @@ struct JoinArgs {
   Tile *ovf_tiles;       // overflowed queries, as one-query tiles
   int32_t *ovf_count;
   u32 *max_err_bits;     // max |key - s64| seen in rerank (float bits)
+  u32 *ferr_bits;        // [3] rerank: max |s_f - s64|, max E, max |s_f - s64| / E (float bits)
+  int relbound;          // filter keys are the FP32 direct-form values, |s_f - s| <= delta s
   // MFMA paths: reference rows centred on their cluster representative
   const void *rc;        // [nR][dk] bf16 or fp32
   const float *nr;       // [nR] squared norm of the centred row (binary32)

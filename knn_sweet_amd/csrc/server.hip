@@ -53,6 +53,7 @@ struct Request {
   std::string error;
   int32_t batch_rows = 0, batch_requests = 0, slot = -1;
   bool done = false;
+  bool claimed = false;  // a waiter owns the request (it alone frees it)
 };
 
 }  // namespace
@@ -70,6 +71,7 @@ struct knnj_server {
   std::vector<std::thread> workers;
   std::vector<knnj_ctx *> ctxs;
   std::atomic<int64_t> batches{0}, rows{0};
+  Clock::time_point t_created = Clock::now();  // origin of the request timestamps
 };
 
 namespace {
@@ -218,8 +220,10 @@ knnj_status knnj_server_wait(knnj_server *S, uint64_t ticket, knnj_request_info 
   {
     std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->all.find(ticket);
-    if (it == S->all.end()) return KNNJ_EINVAL;
+    // unknown, already waited for, or being waited for by another thread
+    if (it == S->all.end() || it->second->claimed) return KNNJ_EINVAL;
     r = it->second;
+    r->claimed = true;  // later waiters on this ticket are turned away above
     S->cv_done.wait(lk, [&] { return r->done; });
     S->all.erase(ticket);  // by key: inserts while waiting may have rehashed the map
   }
@@ -232,6 +236,9 @@ knnj_status knnj_server_wait(knnj_server *S, uint64_t ticket, knnj_request_info 
     v.slot = r->slot;
     v.queue_ms = ms_between(r->t_submit, r->t_start);
     v.latency_ms = ms_between(r->t_submit, r->t_done);
+    v.t_submit_ms = ms_between(S->t_created, r->t_submit);
+    v.t_start_ms = ms_between(S->t_created, r->t_start);
+    v.t_end_ms = ms_between(S->t_created, r->t_done);
     std::snprintf(v.message, sizeof(v.message), "%s", r->error.c_str());
     *info = v;
   }

@@ -50,7 +50,7 @@ def _load():
     lib.knnj_oracle_ti_join.restype = ctypes.c_int
     lib.knnj_oracle_ti_join.argtypes = [
         _f32p, i64, i64, _f32p, i64, i64, i32, i32, i32, i32, i32, ctypes.c_uint64,
-        _i32p, _f32p, _f64p, ctypes.POINTER(ctypes.c_int64), i32]
+        _i32p, _f32p, _f64p, ctypes.POINTER(ctypes.c_int64), i32, _f64p]
     _lib = lib
     return lib
 
@@ -100,8 +100,9 @@ def pair_sq64(R, Q, qi, ri):
 
 
 def ti_join(R, Q, k: int, m_q: int = 0, m_r: int = 0, trials: int = 10, seed: int = 2015,
-            nthreads: int = 0):
-    """CPU triangle-inequality join: returns (idx, dist, sq, n_distance_evaluations)."""
+            nthreads: int = 0, phases: bool = False):
+    """CPU triangle-inequality join: returns (idx, dist, sq, n_distance_evaluations),
+    plus the seconds of (index build and bounds, per-query scan) with phases=True."""
     lib = _load()
     R, Q = _as_f32(R), _as_f32(Q)
     nR, d = R.shape
@@ -112,8 +113,11 @@ def ti_join(R, Q, k: int, m_q: int = 0, m_r: int = 0, trials: int = 10, seed: in
     dist = np.empty((nQ, k), np.float32)
     sq = np.empty((nQ, k), np.float64)
     ev = ctypes.c_int64(0)
+    tp = np.zeros(2, np.float64)
     rc = lib.knnj_oracle_ti_join(R, nR, d, Q, nQ, d, d, k, m_q, m_r, trials, seed, idx, dist, sq,
-                                 ctypes.byref(ev), nthreads)
+                                 ctypes.byref(ev), nthreads, tp)
     if rc != 0:
         raise ValueError("ti_join rejected the arguments")
+    if phases:
+        return idx, dist, sq, int(ev.value), (float(tp[0]), float(tp[1]))
     return idx, dist, sq, int(ev.value)

@@ -88,23 +88,54 @@ static void pick_reps(const float *X, int64_t n, int64_t ld, int32_t d, int32_t 
   free(score);
 }
 
-/* nearest representative for every row; returns cluster ids and distances */
+/* Nearest representative for every row; returns cluster ids and distances.
+ * Like the reference (cublasSgemm of -2 X C^T, then AddAll and the argmin,
+ * knnjoin.cu:270-342, 64-97) the choice is made on the binary32 GEMM form
+ * ||c||^2 - 2 x.c, lowest index on ties; it only has to be good: every
+ * bound then uses the binary64 distance to the chosen representative.  The
+ * dots run with the representatives transposed, [d][m], so the inner loop
+ * over representatives is independent sums the compiler vectorises. */
 static void assign(const float *X, int64_t n, int64_t ld, int32_t d, const float *reps, int32_t m,
                    int32_t *cl, double *dist) {
-#pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < n; ++i) {
-    double best = INFINITY;
-    int32_t bi = 0;
-    for (int32_t c = 0; c < m; ++c) {
-      double s = knnj_oracle_sq64(X + i * ld, reps + (int64_t)c * d, d);
-      if (s < best) {
-        best = s;
-        bi = c;
-      }
+  float *rt = (float *)malloc(sizeof(float) * (size_t)d * m);
+  float *rn = (float *)malloc(sizeof(float) * (size_t)m);
+  for (int32_t c = 0; c < m; ++c) {
+    float s = 0.f;
+    for (int32_t j = 0; j < d; ++j) {
+      const float v = reps[(int64_t)c * d + j];
+      rt[(int64_t)j * m + c] = v;
+      s += v * v;
     }
-    cl[i] = bi;
-    dist[i] = sqrt(best);
+    rn[c] = s;
   }
+#pragma omp parallel
+  {
+    float *acc = (float *)malloc(sizeof(float) * (size_t)m);
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      const float *x = X + i * ld;
+      for (int32_t c = 0; c < m; ++c) acc[c] = 0.f;
+      for (int32_t j = 0; j < d; ++j) {
+        const float xj = x[j];
+        const float *col = rt + (int64_t)j * m;
+        for (int32_t c = 0; c < m; ++c) acc[c] += xj * col[c];
+      }
+      float best = INFINITY;
+      int32_t bi = 0;
+      for (int32_t c = 0; c < m; ++c) {
+        const float s = rn[c] - 2.f * acc[c];
+        if (s < best) {
+          best = s;
+          bi = c;
+        }
+      }
+      cl[i] = bi;
+      dist[i] = sqrt(knnj_oracle_sq64(x, reps + (int64_t)bi * d, d));
+    }
+    free(acc);
+  }
+  free(rt);
+  free(rn);
 }
 
 /* counting sort of rows by cluster: off[m+1], mem[n] (row ids, ascending within cluster) */
@@ -187,13 +218,15 @@ static void d_up(double *h, int c) {
  * Exact kNN join by clustering + triangle-inequality pruning.
  * m_q, m_r: representative counts (clamped to the set sizes); trials: draws
  * per side (the reference uses 10); seed: SplitMix64 seed for both sides.
- * n_evals (nullable) receives the number of exact distance evaluations.
+ * n_evals (nullable) receives the number of exact distance evaluations;
+ * t_phases (nullable) the wall seconds of [0] the index build and bounds of
+ * both sides and [1] the per-query scan.
  * Returns 0, or -1 on bad arguments.
  */
 int knnj_oracle_ti_join(const float *R, int64_t nR, int64_t ldR, const float *Q, int64_t nQ,
                         int64_t ldQ, int32_t d, int32_t k, int32_t m_q, int32_t m_r, int32_t trials,
                         uint64_t seed, int32_t *out_idx, float *out_dist, double *out_sq,
-                        int64_t *n_evals, int32_t nthreads) {
+                        int64_t *n_evals, int32_t nthreads, double *t_phases) {
   if (!R || !Q || d < 1 || k < 1 || k > nR || nQ < 0 || ldR < d || ldQ < d || m_q < 1 || m_r < 1 ||
       trials < 1)
     return -1;
@@ -204,6 +237,7 @@ int knnj_oracle_ti_join(const float *R, int64_t nR, int64_t ldR, const float *Q,
   }
   if (m_q > nQ) m_q = (int32_t)nQ;
   if (m_r > nR) m_r = (int32_t)nR;
+  const double t0 = omp_get_wtime();
 
   int32_t *rq = (int32_t *)malloc(sizeof(int32_t) * m_q), *rr = (int32_t *)malloc(sizeof(int32_t) * m_r);
   pick_reps(Q, nQ, ldQ, d, m_q, trials, seed, rq);
@@ -311,6 +345,7 @@ int knnj_oracle_ti_join(const float *R, int64_t nR, int64_t ldR, const float *Q,
   }
 
   /* the per-query scan */
+  const double t1 = omp_get_wtime();
   int64_t evals = 0;
 #pragma omp parallel reduction(+ : evals)
   {
@@ -356,6 +391,10 @@ int knnj_oracle_ti_join(const float *R, int64_t nR, int64_t ldR, const float *Q,
     free(h);
   }
   if (n_evals) *n_evals = evals;
+  if (t_phases) {
+    t_phases[0] = t1 - t0;
+    t_phases[1] = omp_get_wtime() - t1;
+  }
 
   free(rq); free(rr); free(repq); free(repr); free(clq); free(clr); free(dq); free(dr);
   free(offq); free(offr); free(memq); free(memr); free(radq); free(radr); free(sm);

@@ -1,8 +1,13 @@
-"""Builds the Skin fixture: the first rows of the reference's own data file
-test/Skin_NonSkin.txt (B, G, R, class label; integers) and the FP64 oracle's
-exact k-NN self-join of them at d=3 (B, G, R) and d=4 (label included, as
-the reference's run_sample.sh reads it), ties broken by (squared distance,
-row).  Run once in a checkout beside the reference:
+"""Builds the Skin fixtures from the reference's own data file
+test/Skin_NonSkin.txt (B, G, R, class label; integers 0..255 and 1..2):
+  - skin_1200.txt: its first 1,200 lines as they are, and skin_1200_k32.npz,
+    the FP64 oracle's exact k-NN self-join of them at d=3 (B, G, R) and d=4
+    (label included, as the reference's run_sample.sh reads it), ties
+    broken by (squared distance, row);
+  - skin_rows_u8.npz: every row of the file as uint8 [245057, 4] (the data
+    only; tests write the text the reference's loader reads from it and
+    compute the expected answers with the oracle).
+Run once in a checkout beside the reference:
 
     python tests/golden/make_skin_fixture.py /path/to/reference/test/Skin_NonSkin.txt
 """
@@ -20,7 +25,13 @@ N, K = 1200, 32
 
 def main(src):
     with open(src, "rb") as f:
-        lines = [f.readline() for _ in range(N)]
+        every = [ln for ln in f if ln.strip()]
+    allrows = np.array([[int(v) for v in ln.split()] for ln in every], dtype=np.int64)
+    assert allrows.min() >= 0 and allrows.max() <= 255
+    np.savez_compressed(os.path.join(HERE, "skin_rows_u8.npz"), rows=allrows.astype(np.uint8))
+    print("skin_rows_u8.npz rows", allrows.shape, "sha256",
+          hashlib.sha256(allrows.astype(np.uint8).tobytes()).hexdigest())
+    lines = every[:N]
     with open(os.path.join(HERE, "skin_1200.txt"), "wb") as f:
         f.writelines(lines)
     X = np.array([[float(v) for v in ln.split()] for ln in lines], dtype=np.float32)

@@ -47,6 +47,32 @@ def test_context_without_gpu_reports_error():
     assert e.value.status == kj.KNNJ_EDEVICE
 
 
+def test_option_checks_need_no_gpu():
+    """Bad capacities are rejected before any device work."""
+    lib = kj.load_library()
+    X = (ctypes.c_float * 30)()
+    m = kj.Matrix(ctypes.cast(X, ctypes.c_void_p), 10, 3, 3, 0)
+    out_i = (ctypes.c_int32 * 30)()
+    out_d = (ctypes.c_float * 30)()
+    fake_ctx = ctypes.c_void_p(1)  # never dereferenced: the checks fail first
+    for kw in ({"kcap": 96}, {"kcap": 2048}, {"kcap_mfma_max": 1024}, {"kcap_mfma_max": 48}):
+        o = kj.make_opts(**kw)
+        rc = lib.knnj_join(fake_ctx, m, m, 3, ctypes.byref(o), out_i, out_d, None)
+        assert rc == kj.KNNJ_EINVAL, kw
+        assert b"kcap" in lib.knnj_last_error()
+
+
+def test_plan_part_table_matches_header():
+    text = open(os.path.join(ROOT, "include", "knnj.h")).read()
+    ids = dict((n.lower(), int(v)) for n, v in re.findall(r"KNNJ_PART_(\w+)\s*=\s*(\d+)", text))
+    assert {n: v[0] for n, v in kj.PLAN_PARTS.items()} == {
+        "r_reps": ids["r_reps"], "r_radius": ids["r_radius"], "r_offsets": ids["r_offsets"],
+        "r_perm": ids["r_perm"], "r_pdist": ids["r_pdist"], "q_reps": ids["q_reps"],
+        "q_radius": ids["q_radius"], "q_offsets": ids["q_offsets"], "q_perm": ids["q_perm"],
+        "q_pdist": ids["q_pdist"], "theta": ids["theta"], "pair_count": ids["pair_count"],
+        "pair_list": ids["pair_list"], "pair_lb": ids["pair_lb"]}
+
+
 def test_null_context_is_rejected():
     lib = kj.load_library()
     m = kj.Matrix(None, 0, 3, 3, 0)
@@ -65,7 +91,8 @@ def test_struct_layouts_match_c_compiler(tmp_path):
         pytest.skip("no C compiler")
     mirrors = {"knnj_matrix": kj.Matrix, "knnj_opts": kj.Opts, "knnj_stats": kj.Stats,
                "knnj_instance": kj.Instance, "knnj_index_desc": kj.IndexDesc,
-               "knnj_server_config": kj.ServerConfig, "knnj_request_info": kj.RequestInfo}
+               "knnj_server_config": kj.ServerConfig, "knnj_request_info": kj.RequestInfo,
+               "knnj_plan_desc": kj.PlanDesc}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "knnj.h"', "int main(void) {"]
     for cname, py in mirrors.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')

@@ -105,10 +105,17 @@ def test_server_alone_equals_in_mix(ctx, served):
         assert np.array_equal(idx, ref_idx), f"request {i} indices differ in the mix"
         assert np.array_equal(dist, ref_dist), f"request {i} distances differ in the mix"
         assert info["latency_ms"] >= info["queue_ms"] >= 0.0
+        assert info["t_submit_ms"] <= info["t_start_ms"] <= info["t_end_ms"]
         fused += info["batch_requests"] > 1
     nb, nrows = srv.counters()
     assert nrows == sum(Q.shape[0] for Q in reqs)
     assert nb <= len(reqs)
+    # a real mix: some batch fused several requests, and batches of two
+    # different slots were in flight at the same time
+    assert fused >= 1, "no request was fused with another"
+    spans = {(inf["slot"], inf["t_start_ms"], inf["t_end_ms"]) for _, _, inf in outs}
+    overlap = any(s1 != s2 and a1 < b2 and a2 < b1 for (s1, a1, b1) in spans for (s2, a2, b2) in spans)
+    assert overlap, f"no two slots' batches overlapped: {sorted(spans)[:6]}"
     srv.close()
     ix.close()
 
