@@ -109,13 +109,20 @@ def cpu_baseline(R, Q, k, target_s: float = 12.0, ti_target_s: float = 8.0, rows
             break
         n = int(min(nq, max(2 * n, n * target_s / max(dt, 1e-3))))
     bf_rate = n / dt
-    # TI join: a sample whose scan takes about ti_target_s at the brute force's rate
-    # times the pruning the TI join typically reaches (it is re-measured below)
-    nt = int(min(nq, max(64, 20 * ti_target_s * bf_rate)))
-    tsel = np.ascontiguousarray(Q[order[:nt]])
+    # TI join: a pilot on a few queries measures its scan rate, then a
+    # sample sized for about ti_target_s of scan (the pilot if that is small)
+    nt = min(nq, 128)
     t0 = time.perf_counter()
-    _, _, _, ev, (t_idx, t_scan) = oracle.ti_join(R, tsel, k, nthreads=threads, phases=True)
+    _, _, _, ev, (t_idx, t_scan) = oracle.ti_join(R, np.ascontiguousarray(Q[order[:nt]]), k, nthreads=threads,
+                                                  phases=True)
     ti_dt = time.perf_counter() - t0
+    want = int(min(nq, nt / max(t_scan, 1e-6) * ti_target_s))
+    if want > 2 * nt:
+        nt = want
+        t0 = time.perf_counter()
+        _, _, _, ev, (t_idx, t_scan) = oracle.ti_join(R, np.ascontiguousarray(Q[order[:nt]]), k,
+                                                      nthreads=threads, phases=True)
+        ti_dt = time.perf_counter() - t0
     ti_full = nq / (t_idx + nq * t_scan / nt)
     out = {"value": bf_rate, "unit": "queries/s", "cores": threads, "kind": "port",
            "host_cpus": os.cpu_count(),

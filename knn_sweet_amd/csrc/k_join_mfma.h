@@ -373,6 +373,10 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
             if (!first && lock && a.lockstep && !splitm) __syncthreads();
           }
           first = false;
+          // a cluster this wave visited as a seed is not visited again: the
+          // union of a lockstep workgroup can hold it because another wave
+          // needs it (a second visit would append the same keys twice)
+          if (!seedmode && __builtin_amdgcn_readlane(cslot, jl) >= 0) continue;
           const int b = __builtin_amdgcn_readlane(bj, jl);
           const int s0 = __builtin_amdgcn_readlane(s0j, jl), s1 = __builtin_amdgcn_readlane(s1j, jl);
           const float Rb = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Rbj), jl));

@@ -152,12 +152,14 @@ def test_filter_error_within_its_bound(ctx, d, k, prec, kern):
 @pytest.mark.parametrize("kcap,mfma_max,prec", [
     (64, 64, kj.PREC_BF16X1),    # overflow -> straight to the VALU kernel at 1024
     (64, 512, kj.PREC_BF16X1),   # MFMA at 128, 256, ... in split mode, then VALU if needed
-    (64, 128, kj.PREC_BF16),     # compensated filter, one MFMA stage then VALU
-    (64, 128, kj.PREC_FP32),     # FP32 GEMM-form filter
+    (64, 128, kj.PREC_BF16X1),   # one MFMA stage, then VALU
 ])
 def test_overflow_ladder_stages(ctx, kcap, mfma_max, prec):
-    """A first-pass capacity barely above k makes most queries overflow;
-    every re-run stage must still give the oracle's answer."""
+    """A first-pass capacity barely above k makes most queries overflow with
+    the single-product BF16 filter (its bound is ~2^-7 relative: many
+    candidates tie within it); every re-run stage must still give the
+    oracle's answer.  (The compensated and FP32 filters' bounds are tight
+    enough that capacity 64 holds k = 60 without overflowing.)"""
     mix = datagen.mixture64_model(seed=504, d=128, comps=16)
     R = mix.sample(datagen.rng(505), 20_000)
     Q = mix.sample(datagen.rng(506), 600)
