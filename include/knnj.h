@@ -114,7 +114,13 @@ typedef struct {
   int32_t kcap_mfma_max; /* largest capacity of the MFMA kernel's overflow re-runs before the
                           VALU kernel at 1024 takes the rest: 0 = 512, else a power of two in
                           [64, 512] */
-  int32_t reserved[12];
+  int32_t dedup;       /* queries with bit-identical coordinates share one answer: 0 = auto
+                          (integer-mode joins on the vector ALUs), 1 = off, 2 = on: the join
+                          runs once per distinct query and copies the rows to the others */
+  int32_t filter_stats;/* 1: measure the filter's error against binary64 on every re-scored
+                          candidate (knnj_stats.filter_err_max / filter_bound_max /
+                          filter_err_ratio; a diagnostic: it re-scores each list once more) */
+  int32_t reserved[10];
 } knnj_opts;
 
 /* Device-side timings (hipEvents on the context's stream) and counters. */
@@ -147,11 +153,13 @@ typedef struct {
   int32_t rerun_stages;     /* overflow re-run stages executed (0: none; each stage doubles the
                                capacity on the MFMA kernel, the last one is the VALU kernel at 1024) */
   int32_t reserved0;
-  double filter_err_max;    /* the filter value's own error: largest |s_f - s64| over re-scored
-                               candidates, s_f the filter's squared distance and s64 the binary64
-                               one (0 in exact mode) */
+  double filter_err_max;    /* with opts.filter_stats: the filter value's own error, largest
+                               |s_f - s64| over re-scored candidates, s_f the filter's squared
+                               distance and s64 the binary64 one (0 in exact mode) */
   double filter_bound_max;  /* largest rigorous error bound E attached to a re-scored candidate */
   double filter_err_ratio;  /* largest |s_f - s64| / E over them: <= 1 when the bound holds */
+  int64_t n_distinct_queries; /* queries the join ran for (|Q| unless duplicates were folded;
+                                 0 from knnj_join_batch) */
 } knnj_stats;
 
 /* One independent join of a batch (knnj_join_batch). */

@@ -50,7 +50,8 @@ class Opts(ctypes.Structure):
                 ("xcd_tiles", ctypes.c_int32), ("tile_waves", ctypes.c_int32),
                 ("tile_split", ctypes.c_int32), ("split_rounds", ctypes.c_int32),
                 ("autotune", ctypes.c_int32), ("graph", ctypes.c_int32), ("kcap", ctypes.c_int32),
-                ("kcap_mfma_max", ctypes.c_int32), ("reserved", ctypes.c_int32 * 12)]
+                ("kcap_mfma_max", ctypes.c_int32), ("dedup", ctypes.c_int32),
+                ("filter_stats", ctypes.c_int32), ("reserved", ctypes.c_int32 * 10)]
 
 
 class Stats(ctypes.Structure):
@@ -66,7 +67,7 @@ class Stats(ctypes.Structure):
                 ("n_appended", ctypes.c_int64), ("tuned", ctypes.c_int32), ("graph", ctypes.c_int32),
                 ("rerun_stages", ctypes.c_int32), ("reserved0", ctypes.c_int32),
                 ("filter_err_max", ctypes.c_double), ("filter_bound_max", ctypes.c_double),
-                ("filter_err_ratio", ctypes.c_double)]
+                ("filter_err_ratio", ctypes.c_double), ("n_distinct_queries", ctypes.c_int64)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_
@@ -224,10 +225,10 @@ def _outputs(nq: int, k: int, aos: bool, qdev):
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
               brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0, xcd_tiles=False,
               tile_waves=0, tile_split=0, split_rounds=0, autotune=False, graph=False, kcap=0,
-              kcap_mfma_max=0) -> Opts:
+              kcap_mfma_max=0, dedup=0, filter_stats=False) -> Opts:
     return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
                 tile_q, kernel, refine_iters, int(xcd_tiles), tile_waves, tile_split, split_rounds,
-                int(autotune), int(graph), kcap, kcap_mfma_max)
+                int(autotune), int(graph), kcap, kcap_mfma_max, dedup, int(filter_stats))
 
 
 class Context:

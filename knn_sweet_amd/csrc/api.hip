@@ -274,6 +274,10 @@ knnj_status check_join_args(const knnj_matrix &R, const knnj_matrix &Q, int32_t 
     set_err("kcap must be 0 or a power of two in [64, %d] and >= k", KNNJ_MAX_K);
     return KNNJ_EINVAL;
   }
+  if (o.dedup < 0 || o.dedup > 2) {
+    set_err("dedup must be 0 (auto), 1 (off) or 2 (on)");
+    return KNNJ_EINVAL;
+  }
   if (o.kcap_mfma_max != 0 && (!pow2(o.kcap_mfma_max) || o.kcap_mfma_max < 64 || o.kcap_mfma_max > 512)) {
     set_err("kcap_mfma_max must be 0 or a power of two in [64, 512]");
     return KNNJ_EINVAL;
@@ -337,6 +341,7 @@ struct Inst {
   bool mfma = false, bf16 = false, bf16x1 = false, exact = false, exact_valu = false, integral = false;
   bool use_mfma = false;
   int split_rounds = 0;  // split rounds in the representative selection
+  bool dedup = false;     // duplicate queries answered once (Qu: the leaders' side)
   bool resident = false;  // building a resident index (knnj_index_build): full Lloyd means
   double span = 0.0;
   uint64_t seed = 2015;
@@ -349,6 +354,8 @@ struct Inst {
   double *theta = nullptr;
   int32_t *plist = nullptr, *pcnt = nullptr;
   float *plb = nullptr;
+  int32_t *lead = nullptr, *ucnt = nullptr, *uoff = nullptr, *uperm = nullptr;
+  float *upx = nullptr, *updist = nullptr;
   Tile *tiles = nullptr, *ovf_tiles = nullptr, *ovf2_tiles = nullptr;
   int32_t *counters = nullptr;  // [0] slots, [1] overflow, [2] overflow 2, [3] max err bits, [4] overflow 3
   u64 *evals = nullptr;         // [0] evaluations, [1] merges, [2] appended
@@ -477,6 +484,8 @@ knnj_status decide(Inst &I, const u32 *flags) {
   if (kcap < k) kcap = KNNJ_MAX_K;  // only the VALU kernel holds k > 512
   I.kcap = kcap;
   I.use_mfma = I.mfma && kcap <= 512;
+  // integer data repeats points: there the VALU join runs once per distinct query
+  I.dedup = o.dedup == 2 || (o.dedup == 0 && I.exact && !I.use_mfma);
   I.waves = o.tile_waves > 1 ? o.tile_waves : 1;  // auto: one wave per tile
   I.split = 1;
   if (I.use_mfma && I.waves == 1 && o.tile_split != 1) {
@@ -538,6 +547,15 @@ knnj_status alloc_inst(Alloc &A, Inst &I, Tile *tiles) {
     TRY(A.get("r_nr", (size_t)I.nR, &I.nr));
     TRY(A.get("r_reps32", (size_t)I.m_r * I.dk, &I.reps32));
     TRY(A.get("r_rn32", (size_t)I.m_r, &I.rn32));
+  }
+  if (I.dedup) {
+    const size_t nq = (size_t)std::max<int64_t>(I.nQ, 1);
+    TRY(A.get("dd_lead", nq, &I.lead));
+    TRY(A.get("dd_ucnt", (size_t)I.m_q, &I.ucnt));
+    TRY(A.get("dd_uoff", (size_t)I.m_q + 1, &I.uoff));
+    TRY(A.get("dd_perm", nq, &I.uperm));
+    TRY(A.get("dd_pdist", nq, &I.updist));
+    TRY(A.get("dd_px", nq * I.dpad, &I.upx));
   }
   TRY(A.get("theta", (size_t)I.m_q, &I.theta));
   TRY(A.get("plist", (size_t)I.m_q * I.m_r, &I.plist));
@@ -644,10 +662,11 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
   launch_bounds(I.Qi, I.Ri, I.d, I.k, I.theta, I.plist, I.plb, I.pcnt, s);
   HIPCHK(hipGetLastError());
   if (ev_bounds) HIPCHK(hipEventRecord(ev_bounds, s));
+  if (I.dedup) launch_dedup(I.Qi, I.d, I.lead, I.ucnt, I.uoff, I.upx, I.uperm, I.updist, s);
   launch_zero(I.counters, 8 * sizeof(int32_t), s);
   launch_zero(I.evals, 4 * sizeof(u64), s);
-  launch_make_tiles(I.Qi.off, I.m_q, I.theta, I.tq, I.id, I.tiles, I.counters, (int)I.max_slots,
-                    I.o.xcd_tiles ? 8 : 1, s);
+  launch_make_tiles(I.dedup ? I.uoff : I.Qi.off, I.m_q, I.theta, I.tq, I.id, I.tiles, I.counters,
+                    (int)I.max_slots, I.o.xcd_tiles ? 8 : 1, s);
   HIPCHK(hipGetLastError());
   return KNNJ_OK;
 }
@@ -656,7 +675,10 @@ void fill_args(Inst &I) {
   JoinArgs &a = I.args;
   a = JoinArgs{};
   const int d = I.d;
-  a.Q = I.Qi;
+  // the join reads the distinct queries only (the leaders' side, whose
+  // buffers are fixed by the allocation: fill_args may run before the index
+  // and the dedup kernels are queued, as knnj_join_batch does)
+  a.Q = I.dedup ? dedup_side(I.Qi, I.uoff, I.upx, I.uperm, I.updist) : I.Qi;
   a.R = I.Ri;
   a.d = d;
   a.k = I.k;
@@ -683,7 +705,7 @@ void fill_args(Inst &I) {
   a.ovf_tiles = I.ovf_tiles;
   a.ovf_count = I.counters + 1;
   a.max_err_bits = (u32 *)(I.counters + 3);
-  a.ferr_bits = (u32 *)(I.counters + 5);
+  a.ferr_bits = I.o.filter_stats ? (u32 *)(I.counters + 5) : nullptr;
   a.relbound = (!I.use_mfma && !a.exact) ? 1 : 0;
   a.rc = I.rc;
   a.nr = I.nr;
@@ -711,6 +733,26 @@ void fill_args(Inst &I) {
     cdot = 3.1 * ub * ub + 1.02 * gam(3.0 * d);  // hi/lo, three products
   a.c1 = (float)((2.0 * cdot + 4.0 * u) * 1.01);
   a.c2 = (float)((gam(d + 3) + 8.0 * u) * 1.01);
+}
+
+// The join kernel's buffers, checked on the host before it is launched: a
+// plan whose arguments point nowhere is a bug in this file, reported as an
+// error rather than left to fault the device.
+knnj_status check_args(const JoinArgs &a, int64_t nq) {
+  if (nq <= 0) return KNNJ_OK;
+  const void *need[] = {a.Q.px,    a.Q.perm,  a.Q.pdist, a.Q.off,   a.Q.reps, a.R.px,   a.R.perm,
+                        a.R.pdist, a.R.off,   a.R.reps,  a.R.rad,   a.tiles,  a.ntiles, a.theta,
+                        a.plist,   a.plb,     a.pcnt,    a.out_idx, a.n_evals, a.ovf_tiles, a.ovf_count};
+  for (size_t i = 0; i < sizeof(need) / sizeof(need[0]); ++i)
+    if (!need[i]) {
+      set_err("internal error: join argument %zu is unset", i);
+      return KNNJ_EDEVICE;
+    }
+  if (a.out_mode != 2 && !a.out_dist) {
+    set_err("internal error: join distance output is unset");
+    return KNNJ_EDEVICE;
+  }
+  return KNNJ_OK;
 }
 
 hipError_t launch_main(const Inst &I, int64_t grid, hipStream_t s) {
@@ -777,6 +819,13 @@ knnj_status rerun_overflow(Inst &I, int32_t novf, hipStream_t s, int *stages) {
     out_cnt = next_out;
   }
   return KNNJ_OK;
+}
+
+// Every duplicate query's output row from its leader's (after the join and
+// any overflow re-runs have written the leaders' rows).
+void expand_duplicates(const Inst &I, hipStream_t s) {
+  if (I.dedup && I.nQ > 0)
+    launch_dedup_expand(I.nQ, I.Qi.perm, I.lead, I.k, I.aos ? 1 : 0, I.out_idx, I.out_dist, s);
 }
 
 knnj_status copy_outputs(const Inst &I, hipStream_t s) {
@@ -892,6 +941,7 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
     I.args.tile_prof = tp;
     I.args.ovf_diag = dg;
   }
+  TRY(check_args(I.args, I.nQ));
   HIPCHK(hipEventRecord(ev[8], s));
   HIPCHK(launch_main(I, nslots, s));
   HIPCHK(hipEventRecord(ev[4], s));
@@ -908,6 +958,10 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
   if (hc[1] > 0) TRY(rerun_overflow(I, hc[1], s, &stages));
   st.rerun_stages = stages;
   HIPCHK(hipEventRecord(ev[6], s));
+  expand_duplicates(I, s);
+  HIPCHK(hipEventRecord(ev[9], s));
+  int32_t ndist = (int32_t)I.nQ;
+  if (I.dedup) HIPCHK(hipMemcpyAsync(&ndist, I.uoff + I.m_q, sizeof(ndist), hipMemcpyDeviceToHost, s));
   TRY(copy_outputs(I, s));
   u64 hcnt[4] = {0, 0, 0, 0};
   u32 herr = 0, hferr[3] = {0, 0, 0};
@@ -920,8 +974,10 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
   st.t_total_ms = elapsed(ev[0], ev[7]);
   st.t_index_ms = elapsed(ev[1], ev[2]);
   st.t_bounds_ms = elapsed(ev[2], ev[3]);
-  st.t_join_ms = elapsed(ev[8], ev[4]);
+  // the join proper: the tiles' launch, and the copies to duplicate queries
+  st.t_join_ms = elapsed(ev[8], ev[4]) + (I.dedup ? elapsed(ev[6], ev[9]) : 0.0);
   st.t_fallback_ms = hc[1] > 0 ? elapsed(ev[5], ev[6]) : 0.0;
+  st.n_distinct_queries = ndist;
   st.n_dist_evals = (int64_t)hcnt[0];
   st.n_merges = (int64_t)hcnt[1];
   st.n_appended = (int64_t)hcnt[2];
@@ -953,10 +1009,10 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
     P.plb = I.plb;
   }
   {
-    std::vector<int32_t> hp(I.m_q);
-    HIPCHK(hipMemcpy(hp.data(), I.pcnt, sizeof(int32_t) * I.m_q, hipMemcpyDeviceToHost));
+    int32_t hp[KNNJ_MAX_REPS];  // m_q <= KNNJ_MAX_REPS; no host allocation either
+    HIPCHK(hipMemcpy(hp, I.pcnt, sizeof(int32_t) * I.m_q, hipMemcpyDeviceToHost));
     int64_t t = 0;
-    for (int32_t v : hp) t += v;
+    for (int i = 0; i < I.m_q; ++i) t += hp[i];
     st.n_cluster_pairs = t;
   }
   if (stats) *stats = st;
@@ -1295,7 +1351,12 @@ knnj_status knnj_index_info(const knnj_index *x, knnj_index_desc *info) {
   return KNNJ_OK;
 }
 
-knnj_status knnj_index_query(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, int32_t k, const knnj_opts *opts,
+}  // extern "C"
+
+namespace {
+
+// knnj_index_query with its buffers from the context workspace.
+knnj_status index_query_impl(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, int32_t k, const knnj_opts *opts,
                              int32_t *out_idx, float *out_dist, knnj_stats *stats) {
   g_err.clear();
   if (!c || !x) {
@@ -1339,12 +1400,12 @@ knnj_status knnj_index_query(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, in
   I.out_host = !Q.on_device;
   I.user_idx = out_idx;
   I.user_dist = out_dist;
-  HIPCHK(hipEventRecord(c->ev[0], s));
   Alloc A;
   A.c = c;
+  HIPCHK(hipEventRecord(c->ev[0], s));
   TRY(stage_inputs(A, I, s));
   u32 *flags;
-  TRY(ws(c, "flags", 8, &flags));
+  TRY(A.get("flags", 8, &flags));
   HIPCHK(hipMemsetAsync(flags, 0, 8 * sizeof(u32), s));
   if (I.nQ > 0) launch_scan_input(I.dQ, I.nQ, I.d, I.ldQ, flags, s);
   u32 hflags[3];
@@ -1360,6 +1421,15 @@ knnj_status knnj_index_query(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, in
   hflags[2] |= x->flags[2];
   TRY(decide(I, hflags));
   return run_single(c, I, A, stats);
+}
+
+}  // namespace
+
+extern "C" {
+
+knnj_status knnj_index_query(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, int32_t k, const knnj_opts *opts,
+                             int32_t *out_idx, float *out_dist, knnj_stats *stats) {
+  return index_query_impl(c, x, Q, k, opts, out_idx, out_dist, stats);
 }
 
 knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *inst, const knnj_opts *opts,
@@ -1486,7 +1556,10 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   JoinArgs *dargs;
   TRY(ws(c, "batch_args", (size_t)n_inst, &dargs));
   std::vector<JoinArgs> hargs(n_inst);
-  for (int id : order) hargs[id] = I[id].args;
+  for (int id : order) {
+    TRY(check_args(I[id].args, I[id].nQ));
+    hargs[id] = I[id].args;
+  }
   HIPCHK(hipMemcpyAsync(dargs, hargs.data(), sizeof(JoinArgs) * n_inst, hipMemcpyHostToDevice, s));
   if (!c->aux[0]) {
     for (auto &x : c->aux) HIPCHK(hipStreamCreate(&x));
@@ -1628,6 +1701,7 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
       }
     }
     novf += hc[8 * id + 1];
+    expand_duplicates(x, s);
     TRY(copy_outputs(x, s));
     evals += (int64_t)hcnt[4 * id];
     merges += (int64_t)hcnt[4 * id + 1];

@@ -23,6 +23,48 @@
 // is straight-line code whose row loads all issue before the arithmetic.
 // emit_queries writes NQ queries' rows at once: their re-scoring loads are in
 // flight together (each entry's arithmetic is the same as alone).
+// Diagnostic (a.ferr_bits set): the filter's own error on every entry of a
+// query's list that the rerank re-scores, recomputed from scratch after the
+// answer is written so the rerank itself is unchanged.  A coded key is
+// U = fl(s_f + E) with E the decoded bound, so s_f = U - E up to U's
+// rounding (half an ulp, taken off the error so that a ratio above 1 proves
+// a violated bound); a direct-form key is s_f itself, |s_f - s| <= delta s.
+template <int KCAP>
+__device__ __forceinline__ void filter_stats_query(const JoinArgs &a, const u64 *L, const float *qrow, float cutoff,
+                                                float thr_l) {
+  const int lane = threadIdx.x & 63;
+  float ferr = 0.f, fbnd = 0.f, frat = 0.f;
+  for (int j = lane; j < KCAP; j += 64) {
+    const u64 key = L[j];
+    if (key == KEY_MAX) continue;
+    if (!(a.ecode ? key_lower(key, a.ec) <= thr_l : key_dist(key) <= cutoff)) continue;
+    const int pos = (int)(a.ecode ? key_row_of(key, a.ec) : key_row(key));
+    const double s64 = sq64_pair(qrow, a.R.px + (int64_t)pos * a.R.dpad, a.d);
+    const double U = (double)key_dist(key);
+    double sf = U, eb = 0.0, slack = 0.0;
+    if (a.ecode) {
+      eb = (double)ecode_decode((u32)key >> a.ec.rowbits, a.ec);
+      sf = U - eb;
+      slack = U * 0x1p-24;
+    } else if (a.relbound) {
+      eb = a.delta * s64;
+    }
+    if (!(eb < INFINITY)) continue;
+    const double fe = fmax(fabs(sf - s64) - slack, 0.0);
+    ferr = fmaxf(ferr, (float)fe);
+    fbnd = fmaxf(fbnd, (float)eb);
+    if (eb > 0.0) frat = fmaxf(frat, (float)(fe / eb));
+  }
+  ferr = wave_max(ferr);
+  fbnd = wave_max(fbnd);
+  frat = wave_max(frat);
+  if (lane == 0) {
+    if (ferr > 0.f) atomicMax(a.ferr_bits, __float_as_uint(ferr));
+    if (fbnd > 0.f) atomicMax(a.ferr_bits + 1, __float_as_uint(fbnd));
+    if (frat > 0.f) atomicMax(a.ferr_bits + 2, __float_as_uint(frat));
+  }
+}
+
 template <int KCAP, bool EXACT, int NV, int NQ>
 __device__ __forceinline__ void emit_queries(const JoinArgs &a, const u64 *const (&L)[NQ], const int (&qpos)[NQ],
                                              int cluster, const float (&cutoff)[NQ], const float *const (&qrow)[NQ],
@@ -147,51 +189,21 @@ __device__ __forceinline__ void emit_queries(const JoinArgs &a, const u64 *const
     const int64_t qo = a.Q.perm[qpos[n]];
     u64 hs[E];
     u32 rs[E];
-    float err = 0.f, ferr = 0.f, fbnd = 0.f, frat = 0.f;
+    float err = 0.f;
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       hs[u] = KEY_MAX;
       rs[u] = 0xFFFFFFFFu;
       if (okv[n][u]) {
         const u64 key = L[n][lane + 64 * u];
-        const double s64 = s64v[n][u];
-        hs[u] = (u64)__double_as_longlong(s64);
+        hs[u] = (u64)__double_as_longlong(s64v[n][u]);
         rs[u] = (u32)a.R.perm[pv[n][u]];
-        const double U = (double)key_dist(key);
-        err = fmaxf(err, (float)fabs(U - s64));
-        // the filter value s_f and its bound E: a coded key is U = fl(s_f + E)
-        // with E the decoded bound, so s_f = U - E up to U's rounding (half an
-        // ulp, taken off the error so that a ratio above 1 proves a violated
-        // bound); a direct-form key is s_f itself, with |s_f - s| <= delta s
-        double sf = U, eb = 0.0, slack = 0.0;
-        if (a.ecode) {
-          eb = (double)ecode_decode((u32)key >> a.ec.rowbits, a.ec);
-          sf = U - eb;
-          slack = U * 0x1p-24;
-        } else if (a.relbound) {
-          eb = a.delta * s64;
-        }
-        if (eb < INFINITY) {
-          const double fe = fmax(fabs(sf - s64) - slack, 0.0);
-          ferr = fmaxf(ferr, (float)fe);
-          fbnd = fmaxf(fbnd, (float)eb);
-          if (eb > 0.0) frat = fmaxf(frat, (float)(fe / eb));
-        }
+        err = fmaxf(err, (float)fabs((double)key_dist(key) - s64v[n][u]));
       }
     }
     wave_sort_pair<E>(hs, rs);
     err = wave_max(err);
     if (lane == 0 && err > 0.f) atomicMax(a.max_err_bits, __float_as_uint(err));
-    if (a.ferr_bits) {
-      ferr = wave_max(ferr);
-      fbnd = wave_max(fbnd);
-      frat = wave_max(frat);
-      if (lane == 0) {
-        if (ferr > 0.f) atomicMax(a.ferr_bits, __float_as_uint(ferr));
-        if (fbnd > 0.f) atomicMax(a.ferr_bits + 1, __float_as_uint(fbnd));
-        if (frat > 0.f) atomicMax(a.ferr_bits + 2, __float_as_uint(frat));
-      }
-    }
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const int j = lane + 64 * u;
@@ -207,6 +219,7 @@ __device__ __forceinline__ void emit_queries(const JoinArgs &a, const u64 *const
         }
       }
     }
+    if (a.ferr_bits) filter_stats_query<KCAP>(a, L[n], qrow[n], cutoff[n], thr_l[n]);
   }
 }
 

@@ -870,6 +870,10 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const int32_t *__restrict_
   if (threadIdx.x == 1023) off[m] = part[1023];
 }
 
+void launch_scan_counts(const int32_t *cnt, int m, int32_t *off, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, cnt, m, off);
+}
+
 // Places each row in its cluster's segment (slot order is arbitrary here;
 // the segment sort below makes it deterministic).
 __global__ void k_scatter(const int32_t *__restrict__ cl, int64_t n, const int32_t *__restrict__ off,

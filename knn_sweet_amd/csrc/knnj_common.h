@@ -48,11 +48,19 @@ __device__ __forceinline__ u32 ecode_encode(float E, const ECode &ec) {
   const u32 c = ((u32)(e - ec.ebase) << ec.lgs) + (u32)j;
   return c >= ecode_top(ec) ? ecode_top(ec) : c;
 }
+// 2^(i/8) (1 + 2^-20) rounded up to binary32, i = 3 bits, as a select tree
+// (the join's epilogues decode per B group: no division or exp2 there)
+__device__ __forceinline__ float ecode_step(u32 i) {
+  const float lo = (i & 2u) ? ((i & 1u) ? 0x1.4bfdc4p+0f : 0x1.306ff4p+0f)
+                            : ((i & 1u) ? 0x1.172b96p+0f : 0x1.00001p+0f);
+  const float hi = (i & 2u) ? ((i & 1u) ? 0x1.d581acp+0f : 0x1.ae8a16p+0f)
+                            : ((i & 1u) ? 0x1.8ace6ep+0f : 0x1.6a09fep+0f);
+  return (i & 4u) ? hi : lo;
+}
 __device__ __forceinline__ float ecode_decode(u32 c, const ECode &ec) {
   if (c >= ecode_top(ec)) return INFINITY;
-  const int j = (int)(c & ((1u << ec.lgs) - 1u));
-  const float up = exp2f((float)j / (float)(1 << ec.lgs)) * (1.f + 0x1p-20f);
-  return ldexpf(up, (int)(c >> ec.lgs) + ec.ebase);
+  const u32 j = c & ((1u << ec.lgs) - 1u);  // steps of 1/2^lgs octave = j 2^(3-lgs) eighths
+  return ldexpf(ecode_step(j << (3 - ec.lgs)), (int)(c >> ec.lgs) + ec.ebase);
 }
 __device__ __forceinline__ float key_lower(u64 key, const ECode &ec) {
   return key_dist(key) - 2.f * ecode_decode((u32)key >> ec.rowbits, ec);

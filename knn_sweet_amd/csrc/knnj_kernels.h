@@ -37,6 +37,31 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
 void launch_make_tiles(const int32_t *off, int m, const double *theta, int tq, int inst, Tile *tiles,
                        int32_t *nslots, int max_slots, int nxcd, hipStream_t s);
 
+// exclusive scan of m counts (m <= KNNJ_MAX_REPS): off[0..m], off[m] = total
+void launch_scan_counts(const int32_t *cnt, int m, int32_t *off, hipStream_t s);
+
+// k_dedup.hip: exact duplicate queries answered once.  launch_dedup marks
+// each position's leader (first bit-identical row of its cluster) in lead,
+// counts leaders per cluster (ucnt, scanned into uoff) and writes the
+// leaders' side (rows, original ids, distances) into upx/uperm/updist.
+void launch_dedup(const SideIndex &Q, int d, int32_t *lead, int32_t *ucnt, int32_t *uoff, float *upx,
+                  int32_t *uperm, float *updist, hipStream_t s);
+// The leaders' side as the join reads it: Q's representatives and radii
+// with the buffers launch_dedup fills (a host-side view, known before any
+// launch, so kernel arguments can be filled ahead of the work).
+inline SideIndex dedup_side(const SideIndex &Q, const int32_t *uoff, const float *upx, const int32_t *uperm,
+                            const float *updist) {
+  SideIndex u = Q;
+  u.off = uoff;
+  u.px = upx;
+  u.perm = uperm;
+  u.pdist = updist;
+  return u;
+}
+// copies every non-leader's output row from its leader's (after the join)
+void launch_dedup_expand(int64_t n, const int32_t *perm, const int32_t *lead, int k, int aos, int32_t *out_idx,
+                         float *out_dist, hipStream_t s);
+
 // k_bounds.hip
 // plist[A][j]: surviving reference clusters of query cluster A, ascending by
 // plb[A][j] = max(d(a,b) - r_b, 0) rounded down (a lower bound of d(a, s), s in b)

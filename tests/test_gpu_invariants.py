@@ -142,8 +142,11 @@ def test_filter_error_within_its_bound(ctx, d, k, prec, kern):
     binary64 is within the bound the pruning used (ratio <= 1), and the
     error itself is reported apart from the bound."""
     R, Q = datagen.mixture128(20_000, 500, seed=503 + d, d=d)
-    idx, dist, st = ctx.join(R, Q, k, precision=prec, kernel=kern)
+    idx, dist, st = ctx.join(R, Q, k, precision=prec, kernel=kern, filter_stats=True)
     assert_matches_oracle(R, Q, k, idx, dist)
+    plain = ctx.join(R, Q, k, precision=prec, kernel=kern)
+    assert np.array_equal(plain[0], idx) and np.array_equal(plain[1], dist)
+    assert plain[2]["filter_err_max"] == 0  # measured only on request
     assert st["filter_bound_max"] > 0 and st["filter_err_max"] > 0
     assert st["filter_err_ratio"] <= 1.0, st
     assert st["filter_err_max"] <= st["filter_bound_max"]
