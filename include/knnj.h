@@ -282,6 +282,23 @@ knnj_status knnj_index_query(knnj_ctx *ctx, const knnj_index *index, knnj_matrix
                              const knnj_opts *opts, int32_t *out_idx, float *out_dist,
                              knnj_stats *stats);
 
+/* Allocation-free variant of knnj_index_query, with the same arguments and
+ * results.  It has no counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Every device buffer of the call
+ * -- staged host rows, the query side's cluster index, bounds, tile lists,
+ * counters, staged host outputs -- is carved from a fixed scratch array of
+ * KNNJ_NOALLOC_SCRATCH_BYTES per device, which the library holds from load to
+ * process end; the call makes no hipMalloc / hipFree.  A query whose plan
+ * needs more returns KNNJ_ENOMEM before any index work (a 4,096-row batch at
+ * d = 96, k = 50 needs about 10 MB).  All such calls on one device share that
+ * array and the last call's contents stay in it: calls that overlap on one
+ * GPU race, so serialize them. */
+#define KNNJ_NOALLOC_SCRATCH_BYTES (64ll << 20)
+knnj_status knnj_index_query_noalloc(knnj_ctx *ctx, const knnj_index *index, knnj_matrix Q, int32_t k,
+                                     const knnj_opts *opts, int32_t *out_idx, float *out_dist,
+                                     knnj_stats *stats);
+
 /* ---- Serving: continuous batching over a resident index -----------------
  * The reference answers one query set per process (knnjoin.cu:983-1172;
  * its small-batch path knnjoin.cu:1103-1127).  A knnj_server owns `slots`

@@ -128,7 +128,10 @@ EXPORTS = ("knnj_version", "knnj_last_error", "knnj_ctx_create", "knnj_ctx_destr
            "knnj_ctx_set_budget", "knnj_join", "knnj_join_batch", "knnj_index_build",
            "knnj_index_destroy", "knnj_index_info", "knnj_index_query", "knnj_server_create",
            "knnj_server_submit", "knnj_server_wait", "knnj_server_counters", "knnj_server_destroy",
-           "knnj_ctx_last_plan", "knnj_ctx_copy_plan_part")
+           "knnj_ctx_last_plan", "knnj_ctx_copy_plan_part", "knnj_index_query_noalloc")
+
+# bytes of the per-device scratch array of knnj_index_query_noalloc
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 _lib = None
 
@@ -164,6 +167,8 @@ def load_library(path: str = LIB_PATH):
     lib.knnj_index_info.restype = ctypes.c_int
     lib.knnj_index_query.argtypes = [vp, vp, Matrix, i32, ctypes.POINTER(Opts), vp, vp, ctypes.POINTER(Stats)]
     lib.knnj_index_query.restype = ctypes.c_int
+    lib.knnj_index_query_noalloc.argtypes = [vp, vp, Matrix, i32, ctypes.POINTER(Opts), vp, vp, ctypes.POINTER(Stats)]
+    lib.knnj_index_query_noalloc.restype = ctypes.c_int
     lib.knnj_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
     lib.knnj_server_create.restype = ctypes.c_int
     lib.knnj_server_submit.argtypes = [vp, vp, i64, i64, vp, vp, ctypes.POINTER(ctypes.c_uint64)]
@@ -335,14 +340,24 @@ class Index:
         _check(_lib.knnj_index_info(self._h, ctypes.byref(v)))
         return {name: getattr(v, name) for name, _ in v._fields_ if not name.startswith("reserved")}
 
-    def query(self, Q, k: int, ctx: "Context" = None, **opts):
-        """Exact kNN of Q against the indexed R; returns (idx, dist, stats-dict)."""
+    def query(self, Q, k: int, ctx: "Context" = None, no_alloc: bool = False, out=None, **opts):
+        """Exact kNN of Q against the indexed R; returns (idx, dist, stats-dict).
+
+        no_alloc: run knnj_index_query_noalloc (no device allocation in the
+        library; its buffers come from a fixed per-device scratch array).
+        out: optional (idx, dist) to write into instead of new arrays."""
         ctx = ctx or self.ctx
         mQ, keepQ, qdev = _matrix(Q)
         o = make_opts(**opts)
-        idx, dist, pi, pd = _outputs(mQ.n, k, bool(o.out_aos), keepQ if qdev else None)
+        if out is not None:
+            idx, dist = out
+            pi = idx.data_ptr() if _is_torch(idx) else idx.ctypes.data
+            pd = None if dist is None else (dist.data_ptr() if _is_torch(dist) else dist.ctypes.data)
+        else:
+            idx, dist, pi, pd = _outputs(mQ.n, k, bool(o.out_aos), keepQ if qdev else None)
         st = Stats()
-        _check(_lib.knnj_index_query(ctx._h, self._h, mQ, k, ctypes.byref(o), pi, pd, ctypes.byref(st)))
+        fn = _lib.knnj_index_query_noalloc if no_alloc else _lib.knnj_index_query
+        _check(fn(ctx._h, self._h, mQ, k, ctypes.byref(o), pi, pd, ctypes.byref(st)))
         del keepQ
         return idx, dist, st.as_dict()
 

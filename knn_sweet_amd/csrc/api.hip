@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -75,6 +76,8 @@ struct knnj_ctx {
   // of the last plan, and that plan's signature
   hipGraphExec_t batch_exec = nullptr;
   std::vector<unsigned char> batch_sig;
+  // this device's address of the allocation-free entry point's scratch array
+  void *scratch = nullptr;
   // the last single join's index and bounds (knnj_ctx_last_plan): device
   // pointers into this context's workspace or a resident index
   struct Plan {
@@ -151,12 +154,16 @@ knnj_status ws(knnj_ctx *c, const std::string &name, size_t count, T **out) {
 
 // Buffer source: the named context workspace, or consecutive 256-byte
 // aligned carves of one arena.  An arena Alloc with a null base is a sizing
-// pass: it hands out null pointers and only advances `off`.
+// pass: it hands out null pointers and only advances `off`.  cap > 0 bounds
+// an arena (the allocation-free entry point's fixed scratch): a carve past
+// it fails before anything is handed out.
 struct Alloc {
   knnj_ctx *c = nullptr;
   bool arena = false;
   char *base = nullptr;
   size_t off = 0;
+  size_t cap = 0;
+  bool no_alloc = false;  // the caller cannot allocate: no diagnostic buffers either
   std::vector<void *> *owned = nullptr;  // set: a hipMalloc per buffer, recorded here
   template <class T>
   knnj_status get(const std::string &name, size_t count, T **out) {
@@ -174,6 +181,11 @@ struct Alloc {
     }
     if (!arena) return ws(c, name, count, out);
     const size_t bytes = (std::max<size_t>(count * sizeof(T), 1) + 255) & ~(size_t)255;
+    if (cap > 0 && off + bytes > cap) {
+      set_err("the call needs more than the %zu-byte scratch of the allocation-free entry point (%s: %zu bytes "
+              "at offset %zu)", cap, name.c_str(), bytes, off);
+      return KNNJ_ENOMEM;
+    }
     *out = base ? (T *)(base + off) : nullptr;
     off += bytes;
     return KNNJ_OK;
@@ -930,7 +942,7 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
   HIPCHK(hipStreamSynchronize(s));
 
   fill_args(I);
-  const bool tile_prof = std::getenv("KNNJ_TILE_PROFILE") != nullptr;
+  const bool tile_prof = !A.no_alloc && std::getenv("KNNJ_TILE_PROFILE") != nullptr;
   if (tile_prof) {
     u64 *tp;
     float *dg;
@@ -1022,9 +1034,40 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
 
 }  // namespace
 
+// Scratch of the allocation-free entry point (knnj_index_query_noalloc):
+// one fixed array per device, held from library load to process end.  Its
+// name is a host-side stand-in; host code uses only the device address
+// hipGetSymbolAddress returns for it, once per device with that device
+// current (knnj_ctx_create), kept in g_scratch_addr and in each context.
+static __device__ __align__(256) unsigned char scratch[KNNJ_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_addr;
+
+// The current device's scratch address (device `device` must be current).
+knnj_status scratch_address(int device, void **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it == g_scratch_addr.end()) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_err("hipGetSymbolAddress(scratch) failed on device %d: %s", device, hipGetErrorString(e));
+      return KNNJ_EDEVICE;
+    }
+    it = g_scratch_addr.emplace(device, p).first;
+  }
+  *out = it->second;
+  return KNNJ_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
-const char *knnj_version(void) { return "knnj 0.2.0 (gfx950)"; }
+const char *knnj_version(void) { return "knnj 0.3.0 (gfx950)"; }
 
 const char *knnj_last_error(void) { return g_err.c_str(); }
 
@@ -1054,6 +1097,13 @@ knnj_status knnj_ctx_create(int device, knnj_ctx **out) {
       set_err("hipEventCreate failed");
       knnj_ctx_destroy(c);
       return KNNJ_EDEVICE;
+    }
+  }
+  {
+    const knnj_status e = scratch_address(device, &c->scratch);
+    if (e != KNNJ_OK) {
+      knnj_ctx_destroy(c);
+      return e;
     }
   }
   *out = c;
@@ -1355,9 +1405,10 @@ knnj_status knnj_index_info(const knnj_index *x, knnj_index_desc *info) {
 
 namespace {
 
-// knnj_index_query with its buffers from the context workspace.
+// knnj_index_query with its buffers from the context workspace, or (no_alloc)
+// carved from the device's fixed scratch array.
 knnj_status index_query_impl(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, int32_t k, const knnj_opts *opts,
-                             int32_t *out_idx, float *out_dist, knnj_stats *stats) {
+                             int32_t *out_idx, float *out_dist, knnj_stats *stats, bool no_alloc) {
   g_err.clear();
   if (!c || !x) {
     set_err("context or index is NULL");
@@ -1402,6 +1453,16 @@ knnj_status index_query_impl(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, in
   I.user_dist = out_dist;
   Alloc A;
   A.c = c;
+  if (no_alloc) {
+    if (!c->scratch) {
+      set_err("the context has no scratch array");
+      return KNNJ_EDEVICE;
+    }
+    A.arena = true;
+    A.no_alloc = true;
+    A.base = (char *)c->scratch;
+    A.cap = (size_t)KNNJ_NOALLOC_SCRATCH_BYTES;
+  }
   HIPCHK(hipEventRecord(c->ev[0], s));
   TRY(stage_inputs(A, I, s));
   u32 *flags;
@@ -1429,7 +1490,12 @@ extern "C" {
 
 knnj_status knnj_index_query(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, int32_t k, const knnj_opts *opts,
                              int32_t *out_idx, float *out_dist, knnj_stats *stats) {
-  return index_query_impl(c, x, Q, k, opts, out_idx, out_dist, stats);
+  return index_query_impl(c, x, Q, k, opts, out_idx, out_dist, stats, false);
+}
+
+knnj_status knnj_index_query_noalloc(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, int32_t k,
+                                     const knnj_opts *opts, int32_t *out_idx, float *out_dist, knnj_stats *stats) {
+  return index_query_impl(c, x, Q, k, opts, out_idx, out_dist, stats, true);
 }
 
 knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *inst, const knnj_opts *opts,
