@@ -50,7 +50,8 @@ METRIC = "exact kNN queries/s/GPU (BF16-filter vs plain FP32, recall=1.0); p99 m
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
-PREC_NAMES = {1: "fp32", 2: "bf16x3-filter+fp64-rerank", 4: "bf16x1-filter+fp64-rerank"}
+PREC_NAMES = {1: "fp32", 2: "bf16x3-filter+fp64-rerank", 3: "fp8-filter+fp64-rerank",
+              4: "bf16x1-filter+fp64-rerank"}
 KERNEL_NAMES = {1: "valu", 2: "mfma"}
 
 
@@ -231,7 +232,7 @@ def bench_single(args, kj, cfg):
                 refine_iters=args.refine_iters, tile_q=args.tile_q, autotune=not args.no_autotune,
                 kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel],
                 precision={"auto": kj.PREC_AUTO, "fp32": kj.PREC_FP32, "bf16": kj.PREC_BF16,
-                           "bf16x1": kj.PREC_BF16X1}[args.precision])
+                           "bf16x1": kj.PREC_BF16X1, "fp8": kj.PREC_FP8}[args.precision])
     total, times, stats, last = time_joins(ctx, Rd, Qd, k, args.steps, args.warmup, opts)
     nq, nr, d = Q.shape[0], R.shape[0], R.shape[1]
     value = nq * args.steps / total
@@ -242,7 +243,8 @@ def bench_single(args, kj, cfg):
     out = {
         "metric": METRIC, "value": value, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": total * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "bf16" if "bf16" in prec else "f32",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16" if "bf16" in prec else "fp8" if "fp8" in prec else "f32",
         "data": "synthetic (seeded Gaussian mixture, knn_sweet_amd.datagen)",
         "config": {"workload": cfg["workload"], "n_r": nr, "n_q": nq, "d": d, "k": k, "precision": prec,
                    "kernel": KERNEL_NAMES[st["kernel_used"]], "self_join": cfg["self_join"]},
@@ -282,6 +284,19 @@ def bench_single(args, kj, cfg):
                                       ctx, Rd, Qd, k, dict(opts, precision=kj.PREC_FP32, kernel=kj.KERNEL_VALU),
                                       vstats[-1])}
         out["bf16_vs_fp32_valu"] = value / vvalue
+        # the FP8 (e4m3, per-row scaled) filter on the same inputs: its
+        # candidates against the BF16 filter's (survivor inflation), its time
+        eopts = dict(opts, precision=kj.PREC_FP8)
+        etotal, etimes, estats, _ = time_joins(ctx, Rd, Qd, k, fsteps, 1, eopts)
+        est = estats[-1]
+        out["fp8_filter"] = {"value": nq * fsteps / etotal, "steps": fsteps, "ms_per_step": etotal * 1e3 / fsteps,
+                             "join_ms": float(np.mean([x["t_join_ms"] for x in estats])),
+                             "fallback_ms": est["t_fallback_ms"], "n_overflow": est["n_overflow"],
+                             "rerun_stages": est["rerun_stages"], "kcap": est["kcap"],
+                             "appended": est["n_appended"], "dist_evals": est["n_dist_evals"],
+                             "survivor_inflation_vs_bf16": est["n_appended"] / max(st["n_appended"], 1),
+                             "filter_error_vs_fp64": filter_error(ctx, Rd, Qd, k, eopts, est)}
+        out["fp8_vs_bf16"] = out["fp8_filter"]["value"] / value
     if args.config == "c2" and not args.no_compare:
         # the same plain FP32 join on the vector ALUs (direct-form tiles)
         fsteps = max(2, args.steps // 2)
@@ -476,7 +491,7 @@ def main():
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP32 comparison (c3)")
     ap.add_argument("--brute-force", action="store_true", help="one cluster per side")
     ap.add_argument("--kernel", default="auto", choices=["auto", "valu", "mfma"])
-    ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16", "bf16x1"])
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp32", "bf16", "bf16x1", "fp8"])
     ap.add_argument("--tile-waves", type=int, default=0, help="MFMA join: waves per workgroup tile (0 = auto)")
     ap.add_argument("--xcd-tiles", action="store_true", help="place all tiles of a query cluster on one XCD")
     ap.add_argument("--tile-split", type=int, default=0, help="MFMA join split mode (0 auto, 1 off, 4 on)")

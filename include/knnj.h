@@ -51,7 +51,9 @@ typedef enum {
   KNNJ_PREC_FP32 = 1, /* plain FP32 distance tiles */
   KNNJ_PREC_BF16 = 2, /* BF16 MFMA filter, compensated: operands split into bf16 hi + lo,
                          three products; + exact rerank (implies the MFMA kernel) */
-  KNNJ_PREC_FP8 = 3,  /* reserved: FP8 filter (not built) */
+  KNNJ_PREC_FP8 = 3,  /* FP8 (e4m3) MFMA filter, 32 < d <= 128: centred rows scaled per row
+                         by a power of two, one fp8 product per term, with its rigorous
+                         (wide) error bound + exact rerank (implies the MFMA kernel) */
   KNNJ_PREC_BF16X1 = 4 /* BF16 MFMA filter with a single product per term (wider error
                           band, larger candidate lists) + exact rerank */
 } knnj_precision;

@@ -257,8 +257,8 @@ knnj_status check_join_args(const knnj_matrix &R, const knnj_matrix &Q, int32_t 
     set_err("unknown precision %d", o.precision);
     return KNNJ_EINVAL;
   }
-  if (o.precision == KNNJ_PREC_FP8) {
-    set_err("FP8 filter is not built");
+  if (o.precision == KNNJ_PREC_FP8 && (R.d <= 32 || R.d > 128)) {
+    set_err("the FP8 filter supports 32 < d <= 128 (d = %d)", R.d);
     return KNNJ_EUNSUPPORTED;
   }
   if (o.tile_q != 0 && o.tile_q != 1 && o.tile_q != 2 && o.tile_q != 4 && o.tile_q != 8) {
@@ -277,8 +277,9 @@ knnj_status check_join_args(const knnj_matrix &R, const knnj_matrix &Q, int32_t 
     set_err("unknown kernel %d", o.kernel);
     return KNNJ_EINVAL;
   }
-  if ((o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1) && o.kernel == KNNJ_KERNEL_VALU) {
-    set_err("the BF16 filter runs on the MFMA kernel only");
+  if ((o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1 || o.precision == KNNJ_PREC_FP8) &&
+      o.kernel == KNNJ_KERNEL_VALU) {
+    set_err("the BF16 and FP8 filters run on the MFMA kernel only");
     return KNNJ_EINVAL;
   }
   const auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
@@ -319,10 +320,10 @@ struct knnj_index {
   knnj_opts o{};     // build options
   u32 flags[3] = {};  // input scan of R: non-finite, max |x| bits, non-integral
   int kind = 0, dpad = 0, dk = 0, m_r = 0;
-  bool mfma = false, bf16 = false, bf16x1 = false;
+  bool mfma = false, bf16 = false, bf16x1 = false, fp8 = false;
   SideIndex Ri{};
   void *rc = nullptr;
-  float *nr = nullptr, *reps32 = nullptr, *rn32 = nullptr;
+  float *nr = nullptr, *reps32 = nullptr, *rn32 = nullptr, *rinv = nullptr;
   std::vector<void *> owned;
   size_t bytes = 0;
   double t_build_ms = 0.0;
@@ -350,7 +351,8 @@ struct Inst {
   // decisions
   int kind = 0, kcap = 0, tq = 0, dpad = 0, dk = 0, m_r = 0, m_q = 0, trials = 10, refine = 1, waves = 1,
       split = 1;
-  bool mfma = false, bf16 = false, bf16x1 = false, exact = false, exact_valu = false, integral = false;
+  bool mfma = false, bf16 = false, bf16x1 = false, fp8 = false, exact = false, exact_valu = false,
+       integral = false;
   bool use_mfma = false;
   int split_rounds = 0;  // split rounds in the representative selection
   bool dedup = false;     // duplicate queries answered once (Qu: the leaders' side)
@@ -362,7 +364,7 @@ struct Inst {
   SideBufs rb, qb;
   SideIndex Ri{}, Qi{};
   void *rc = nullptr;
-  float *nr = nullptr, *reps32 = nullptr, *rn32 = nullptr;
+  float *nr = nullptr, *reps32 = nullptr, *rn32 = nullptr, *rinv = nullptr;
   double *theta = nullptr;
   int32_t *plist = nullptr, *pcnt = nullptr;
   float *plb = nullptr;
@@ -377,7 +379,7 @@ struct Inst {
   int kernel_exact() const { return use_mfma ? (exact ? 1 : 0) : ((mfma ? exact_valu : exact) ? 1 : 0); }
   // kernel instantiation: joins with equal keys can share one launch
   std::tuple<int, int, int, int, int, int> launch_key() const {
-    return std::make_tuple(use_mfma ? (bf16 ? (bf16x1 ? 3 : 2) : 1) : 0, kcap, kernel_exact(), tq,
+    return std::make_tuple(use_mfma ? (fp8 ? 4 : bf16 ? (bf16x1 ? 3 : 2) : 1) : 0, kcap, kernel_exact(), tq,
                            use_mfma ? dk : 0, split);
   }
 };
@@ -453,7 +455,8 @@ knnj_status decide(Inst &I, const u32 *flags) {
   I.exact_valu = I.integral && (double)d * I.span * I.span < 16777216.0;
   const bool exact_mfma = I.integral && 4.0 * (double)d * I.span * I.span < 16777216.0;
   int kind = o.kernel;
-  if (o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1) kind = KNNJ_KERNEL_MFMA;
+  if (o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1 || o.precision == KNNJ_PREC_FP8)
+    kind = KNNJ_KERNEL_MFMA;
   if (kind == KNNJ_KERNEL_AUTO) kind = (d >= 32 && d <= 256) ? KNNJ_KERNEL_MFMA : KNNJ_KERNEL_VALU;
   if (kind == KNNJ_KERNEL_MFMA && d > 256) {
     set_err("the MFMA kernel supports d <= 256 (d = %d)", d);
@@ -464,7 +467,8 @@ knnj_status decide(Inst &I, const u32 *flags) {
   I.bf16 = I.mfma && (o.precision == KNNJ_PREC_BF16 || o.precision == KNNJ_PREC_BF16X1 ||
                       (o.precision == KNNJ_PREC_AUTO && d >= 64 && !exact_mfma));
   I.bf16x1 = I.bf16 && o.precision == KNNJ_PREC_BF16X1;
-  I.exact = I.mfma ? (!I.bf16 && exact_mfma) : I.exact_valu;
+  I.fp8 = I.mfma && o.precision == KNNJ_PREC_FP8;
+  I.exact = I.mfma ? (!I.bf16 && !I.fp8 && exact_mfma) : I.exact_valu;
   I.dpad = (d + 3) & ~3;
   I.dk = I.mfma ? mfma_dk(d) : 0;
   if (I.pre) {
@@ -474,7 +478,8 @@ knnj_status decide(Inst &I, const u32 *flags) {
     I.mfma = X.mfma;
     I.bf16 = X.bf16;
     I.bf16x1 = X.bf16x1;
-    I.exact = I.mfma ? (!I.bf16 && exact_mfma) : I.exact_valu;
+    I.fp8 = X.fp8;
+    I.exact = I.mfma ? (!I.bf16 && !I.fp8 && exact_mfma) : I.exact_valu;
     I.dk = X.dk;
   }
   const bool brute = o.brute_force != 0;
@@ -489,6 +494,7 @@ knnj_status decide(Inst &I, const u32 *flags) {
   I.refine = o.refine_iters < 0 ? 0 : (o.refine_iters > 0 ? o.refine_iters : 1);
   I.split_rounds = o.split_rounds > 0 ? std::min(o.split_rounds, 16) : 0;  // joins: none by default
   int kcap = I.exact    ? pow2_at_least(std::max(k, 64))
+             : I.fp8    ? pow2_at_least(std::max(4 * k, 64))
              : I.bf16x1 ? pow2_at_least(std::max(2 * k, 64))
                         : pow2_at_least(std::max(k + std::max(8, k / 8), 64));
   if (o.kcap > 0) kcap = o.kcap;  // the caller's first-pass capacity (>= k, checked)
@@ -548,6 +554,7 @@ knnj_status alloc_inst(Alloc &A, Inst &I, Tile *tiles) {
     I.nr = I.pre->nr;
     I.reps32 = I.pre->reps32;
     I.rn32 = I.pre->rn32;
+    I.rinv = I.pre->rinv;
   } else {
     TRY(alloc_side(A, "r_", I.nR, I.m_r, I.dpad, I.d, I.trials, I.rb));
   }
@@ -556,6 +563,8 @@ knnj_status alloc_inst(Alloc &A, Inst &I, Tile *tiles) {
     char *rc;
     TRY(A.get("r_rc", (size_t)I.nR * I.dk * 4, &rc));
     I.rc = rc;
+    // fp8 rows take nR dk bytes of it, their inverse scales the next 4 nR
+    I.rinv = rc ? (float *)(rc + (size_t)I.nR * I.dk) : nullptr;
     TRY(A.get("r_nr", (size_t)I.nR, &I.nr));
     TRY(A.get("r_reps32", (size_t)I.m_r * I.dk, &I.reps32));
     TRY(A.get("r_rn32", (size_t)I.m_r, &I.rn32));
@@ -658,6 +667,9 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
   return KNNJ_OK;
 }
 
+// The operand planes of the centred reference rows (launch_center).
+int center_planes(const Inst &I) { return I.fp8 ? 3 : I.bf16 ? (I.bf16x1 ? 1 : 2) : 0; }
+
 // Index build, centring, bounds and the tile list of one join.  Optional
 // events are recorded after the index (with centring) and after the bounds.
 knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_index, hipEvent_t ev_bounds) {
@@ -667,7 +679,7 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
   else
     TRY(enqueue_side(c, I, false, s));
   if (I.mfma && !I.pre) {
-    launch_center(I.Ri, I.d, I.dk, I.bf16 ? (I.bf16x1 ? 1 : 2) : 0, I.rc, I.nr, I.reps32, I.rn32, s);
+    launch_center(I.Ri, I.d, I.dk, center_planes(I), I.rc, I.nr, I.reps32, I.rn32, I.rinv, s);
     HIPCHK(hipGetLastError());
   }
   if (ev_index) HIPCHK(hipEventRecord(ev_index, s));
@@ -723,6 +735,7 @@ void fill_args(Inst &I) {
   a.nr = I.nr;
   a.reps32 = I.reps32;
   a.rn32 = I.rn32;
+  a.rinv = I.rinv;
   a.dk = I.dk;
   // per-entry error codes in the MFMA filter keys: rows must fit 26 bits;
   // codes span 63 binades below the largest possible squared distance
@@ -739,7 +752,12 @@ void fill_args(Inst &I) {
   auto gam = [&](double n) { return n * u / (1.0 - n * u); };
   // |computed dot - (q-c).(r-c)| <= cdot ||q-c|| ||r-c||
   double cdot = gam(d);  // fp32 operands
-  if (I.bf16x1)
+  if (I.fp8) {
+    // e4m3 rows scaled by powers of two (k_join_mfma.h): relative rounding
+    // 2^-4, subnormal parts <= 2^-16 ||x|| per element
+    const double dl = 0x1p-4 + std::sqrt((double)d) * 0x1p-16;
+    cdot = 2.0 * dl + dl * dl + gam(d) * (1.0 + dl) * (1.0 + dl);
+  } else if (I.bf16x1)
     cdot = 2.0 * ub + ub * ub + gam(d) * (1.0 + ub) * (1.0 + ub);  // one bf16 product
   else if (I.bf16)
     cdot = 3.1 * ub * ub + 1.02 * gam(3.0 * d);  // hi/lo, three products
@@ -769,6 +787,7 @@ knnj_status check_args(const JoinArgs &a, int64_t nq) {
 
 hipError_t launch_main(const Inst &I, int64_t grid, hipStream_t s) {
   if (I.use_mfma) {
+    if (I.fp8) return launch_join_mfma_fp8(I.args, grid, s);
     if (I.bf16) return launch_join_mfma_bf16(I.args, I.bf16x1 ? 1 : 3, grid, s);
     return launch_join_mfma_f32(I.args, grid, s);
   }
@@ -804,7 +823,9 @@ knnj_status rerun_overflow(Inst &I, int32_t novf, hipStream_t s, int *stages) {
       b.kcap = kc;
       const int sw = join_mfma_split_waves(kc, I.dk);
       if (sw > 1) b.split = b.waves = sw;
-      if (I.bf16)
+      if (I.fp8)
+        HIPCHK(launch_join_mfma_fp8(b, n, s));
+      else if (I.bf16)
         HIPCHK(launch_join_mfma_bf16(b, I.bf16x1 ? 1 : 3, n, s));
       else
         HIPCHK(launch_join_mfma_f32(b, n, s));
@@ -851,7 +872,7 @@ knnj_status copy_outputs(const Inst &I, hipStream_t s) {
 
 void plan_stats(const Inst &I, knnj_stats &st) {
   st.exact_mode = I.exact ? 1 : 0;
-  st.precision_used = I.bf16 ? (I.bf16x1 ? KNNJ_PREC_BF16X1 : KNNJ_PREC_BF16) : KNNJ_PREC_FP32;
+  st.precision_used = I.fp8 ? KNNJ_PREC_FP8 : I.bf16 ? (I.bf16x1 ? KNNJ_PREC_BF16X1 : KNNJ_PREC_BF16) : KNNJ_PREC_FP32;
   st.kernel_used = I.kind;
   st.m_q = I.m_q;
   st.m_r = I.m_r;
@@ -1316,6 +1337,7 @@ knnj_status knnj_index_build(knnj_ctx *c, knnj_matrix R, const knnj_opts *opts, 
   x->mfma = I.mfma;
   x->bf16 = I.bf16;
   x->bf16x1 = I.bf16x1;
+  x->fp8 = I.fp8;
   // persistent buffers owned by the index; build scratch from the context
   Alloc P;
   P.c = c;
@@ -1342,6 +1364,7 @@ knnj_status knnj_index_build(knnj_ctx *c, knnj_matrix R, const knnj_opts *opts, 
     char *rc;
     OWN(P.get("rc", (size_t)n * I.dk * 4, &rc));
     I.rc = rc;
+    I.rinv = (float *)(rc + (size_t)n * I.dk);
     OWN(P.get("nr", (size_t)n, &I.nr));
     OWN(P.get("reps32", (size_t)m * I.dk, &I.reps32));
     OWN(P.get("rn32", (size_t)m, &I.rn32));
@@ -1361,7 +1384,7 @@ knnj_status knnj_index_build(knnj_ctx *c, knnj_matrix R, const knnj_opts *opts, 
   }
   OWN(enqueue_side(c, I, true, s));
   if (I.mfma) {
-    launch_center(I.Ri, I.d, I.dk, I.bf16 ? (I.bf16x1 ? 1 : 2) : 0, I.rc, I.nr, I.reps32, I.rn32, s);
+    launch_center(I.Ri, I.d, I.dk, center_planes(I), I.rc, I.nr, I.reps32, I.rn32, I.rinv, s);
     if (hipGetLastError() != hipSuccess) {
       set_err("centring launch failed");
       return fail(KNNJ_EDEVICE);
@@ -1377,6 +1400,7 @@ knnj_status knnj_index_build(knnj_ctx *c, knnj_matrix R, const knnj_opts *opts, 
   x->nr = I.nr;
   x->reps32 = I.reps32;
   x->rn32 = I.rn32;
+  x->rinv = I.rinv;
   x->bytes = P.off;
   x->t_build_ms = elapsed(c->ev[0], c->ev[1]);
   *out = x;
@@ -1393,7 +1417,7 @@ knnj_status knnj_index_info(const knnj_index *x, knnj_index_desc *info) {
   v.d = x->d;
   v.m_r = x->m_r;
   v.kernel = x->kind;
-  v.precision = x->bf16 ? (x->bf16x1 ? KNNJ_PREC_BF16X1 : KNNJ_PREC_BF16) : KNNJ_PREC_FP32;
+  v.precision = x->fp8 ? KNNJ_PREC_FP8 : x->bf16 ? (x->bf16x1 ? KNNJ_PREC_BF16X1 : KNNJ_PREC_BF16) : KNNJ_PREC_FP32;
   v.device = x->device;
   v.bytes = (int64_t)x->bytes;
   v.t_build_ms = x->t_build_ms;
@@ -1422,7 +1446,7 @@ knnj_status index_query_impl(knnj_ctx *c, const knnj_index *x, knnj_matrix Q, in
   if (opts) o = *opts;
   // the index fixed these: a query may not ask for others
   o.kernel = x->kind;
-  o.precision = x->bf16 ? (x->bf16x1 ? KNNJ_PREC_BF16X1 : KNNJ_PREC_BF16) : KNNJ_PREC_FP32;
+  o.precision = x->fp8 ? KNNJ_PREC_FP8 : x->bf16 ? (x->bf16x1 ? KNNJ_PREC_BF16X1 : KNNJ_PREC_BF16) : KNNJ_PREC_FP32;
   o.brute_force = 0;
   knnj_matrix R{};
   R.data = nullptr;
@@ -1669,7 +1693,9 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
       const int nslots = (int)(last - first);
       const Tile *t0 = all_tiles + first;
       if (P.use_mfma) {
-        if (P.bf16)
+        if (P.fp8)
+          HIPCHK(launch_join_mfma_fp8_multi(P.args, dargs, t0, nslots, s));
+        else if (P.bf16)
           HIPCHK(launch_join_mfma_bf16_multi(P.args, P.bf16x1 ? 1 : 3, dargs, t0, nslots, s));
         else
           HIPCHK(launch_join_mfma_f32_multi(P.args, dargs, t0, nslots, s));
@@ -1701,7 +1727,7 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
       const Inst &x = I[id];
       const int64_t v[] = {x.id, (int64_t)(uintptr_t)x.dR, (int64_t)(uintptr_t)x.dQ, x.ldR, x.ldQ, x.nR, x.nQ,
                            x.m_r, x.m_q, x.trials, (int64_t)x.seed, x.refine, x.split_rounds, x.integral,
-                           x.bf16, x.bf16x1, x.mfma, x.tq, x.max_slots, slot0[id], x.self};
+                           x.bf16, x.bf16x1, x.mfma, x.tq, x.max_slots, slot0[id], x.self, x.fp8};
       put(v, sizeof(v));
     }
     put(&all_tiles, sizeof(all_tiles));

@@ -2,10 +2,13 @@
 // the MFMA distance tiles.
 //
 // Row p of cluster b becomes r' = fl32(r - c_b), stored as fp32, as bf16
-// (round to nearest even), or as two bf16 planes hi = bf16(r'),
+// (round to nearest even), as two bf16 planes hi = bf16(r'),
 // lo = bf16(r' - hi) (row layout [hi[dk] lo[dk]]) for the compensated
-// three-product filter, in rows of dk >= d elements (zero padded), together
-// with ||r'||^2 accumulated in binary32 from the fp32 differences.  Centring
+// three-product filter, or as fp8 (e4m3) of r' scaled by a power of two
+// 2^e that brings its largest magnitude into [64, 128) (below the largest
+// finite value of both e4m3 encodings), with 2^-e kept per row in rinv; rows
+// of dk >= d elements (zero padded), together with ||r'||^2 accumulated in
+// binary32 from the fp32 differences.  Centring
 // keeps the operands of the GEMM form ||q-c||^2 + ||r-c||^2 - 2 (q-c).(r-c)
 // small, which is what makes its rounding error small relative to the
 // neighbour distances (the error bound is in k_join_mfma.h).
@@ -20,8 +23,8 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 kc_bf16x8;
 // and stores; 2 x 64/G rows per wave step.
 template <int G>
 __global__ __launch_bounds__(256) void k_center(SideIndex Ri, int d, int dk, int planes, void *rc, float *nr,
-                                                float *reps32, float *rn32) {
-  // planes: 0 fp32 rows, 1 one bf16 plane, 2 hi/lo bf16 planes
+                                                float *reps32, float *rn32, float *rinv) {
+  // planes: 0 fp32 rows, 1 one bf16 plane, 2 hi/lo bf16 planes, 3 scaled fp8
   const int b = blockIdx.x;
   const int s0 = Ri.off[b], s1 = Ri.off[b + 1];
   const float *rep = Ri.reps + (int64_t)b * Ri.dpad;
@@ -59,8 +62,20 @@ __global__ __launch_bounds__(256) void k_center(SideIndex Ri, int d, int dk, int
       xv[4] = xv[5] = xv[6] = xv[7] = 0.f;
     }
   };
-  auto store_row = [&](int p, const float (&v)[8], float part) {
-    if (planes == 2) {
+  auto store_row = [&](int p, const float (&v)[8], float part, float vmax) {
+    if (planes == 3) {
+      // 2^e with vmax 2^e in [64, 128): exact scaling, then round to nearest
+      const int e = vmax > 0.f ? 6 - ilogbf(vmax) : 0;
+      float w[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) w[u] = ldexpf(v[u], e);
+      int lo = __builtin_amdgcn_cvt_pk_fp8_f32(w[0], w[1], 0, false);
+      lo = __builtin_amdgcn_cvt_pk_fp8_f32(w[2], w[3], lo, true);
+      int hi = __builtin_amdgcn_cvt_pk_fp8_f32(w[4], w[5], 0, false);
+      hi = __builtin_amdgcn_cvt_pk_fp8_f32(w[6], w[7], hi, true);
+      *(uint2 *)((unsigned char *)rc + (int64_t)p * dk + 8 * c) = make_uint2((unsigned)lo, (unsigned)hi);
+      if (c == 0) rinv[p] = ldexpf(1.f, -e);
+    } else if (planes == 2) {
       kc_bf16x8 h, l;
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
@@ -88,30 +103,34 @@ __global__ __launch_bounds__(256) void k_center(SideIndex Ri, int d, int dk, int
     if (oka) load_row(pa, xa);
     if (okb) load_row(pb, xb);
     float va[8], vb[8];
-    float parta = 0.f, partb = 0.f;
+    float parta = 0.f, partb = 0.f, ma = 0.f, mb = 0.f;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       va[u] = (oka && 8 * c + u < d) ? xa[u] - rv[u] : 0.f;
       vb[u] = (okb && 8 * c + u < d) ? xb[u] - rv[u] : 0.f;
       parta = __builtin_fmaf(va[u], va[u], parta);
       partb = __builtin_fmaf(vb[u], vb[u], partb);
+      ma = fmaxf(ma, fabsf(va[u]));
+      mb = fmaxf(mb, fabsf(vb[u]));
     }
 #pragma unroll
     for (int msk = 1; msk < G; msk <<= 1) {
       parta += __shfl_xor(parta, msk);
       partb += __shfl_xor(partb, msk);
+      ma = fmaxf(ma, __shfl_xor(ma, msk));
+      mb = fmaxf(mb, __shfl_xor(mb, msk));
     }
-    if (oka) store_row(pa, va, parta);
-    if (okb) store_row(pb, vb, partb);
+    if (oka) store_row(pa, va, parta, ma);
+    if (okb) store_row(pb, vb, partb, mb);
   }
 }
 
 void launch_center(const SideIndex &Ri, int d, int dk, int planes, void *rc, float *nr, float *reps32, float *rn32,
-                   hipStream_t s) {
+                   float *rinv, hipStream_t s) {
   int G = 1;
   while (G < dk / 8) G <<= 1;
 #define KJ_CENTER(GG) \
-  hipLaunchKernelGGL(k_center<GG>, dim3(Ri.m), dim3(256), 0, s, Ri, d, dk, planes, rc, nr, reps32, rn32)
+  hipLaunchKernelGGL(k_center<GG>, dim3(Ri.m), dim3(256), 0, s, Ri, d, dk, planes, rc, nr, reps32, rn32, rinv)
   switch (G) {
     case 4: KJ_CENTER(4); break;
     case 8: KJ_CENTER(8); break;

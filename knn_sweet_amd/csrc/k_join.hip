@@ -39,6 +39,7 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
   const int lane = threadIdx.x;
   const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
   u64 nvisit = 0, nsweepb = 0;  // diagnostics: visits, screened blocks
+  u64 nchunks = 0, mticks = 0;   // diagnostics: 64-row chunks computed, ticks in merges
   const int A = tile.cluster;
   const int nq = tile.count;
   const int d = a.d, dpR = a.R.dpad, dpQ = a.Q.dpad, k = a.k;
@@ -62,6 +63,10 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
     qrow[i] = a.Q.px + (int64_t)(tile.start + (i < nq ? i : 0)) * dpQ;
   }
   u64 evals = 0, nmerge = 0, npass = 0;
+  // the first 16 bytes of each query row, held for the whole tile
+  float4 qv0[TQ];
+#pragma unroll
+  for (int i = 0; i < TQ; ++i) qv0[i] = *(const float4 *)qrow[i];
 
   auto merge = [&](int i) {
     wave_merge<KCAP>(lists + i * KCAP, bufs + i * KNNJ_BCAP, cnt[i], tmp);
@@ -178,21 +183,34 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
       }
       const int cf = cg + __builtin_ctzll(m);
       const int cl = cg + 63 - __builtin_clzll(m);
+      // each lane's first 16 row bytes and row id are loaded one chunk
+      // ahead, so a chunk's distances do not wait on its own loads (rows of
+      // d <= 4 are just those 16 bytes)
+      auto chunk_pos = [&](int c) {
+        const int q = s0 + 64 * c + lane;
+        return q < s1 ? q : s0;
+      };
+      int ppn = chunk_pos(cf);
+      float4 r0n = *(const float4 *)(a.R.px + (int64_t)ppn * dpR);
+      u32 rown = EXACT ? (u32)a.R.perm[ppn] : (u32)ppn;
       for (int c2 = cf; c2 <= cl; ++c2) {
         const int p = s0 + 64 * c2 + lane;
         const bool valid = p < s1;
-        const int pp = valid ? p : s0;
-        const u32 row = EXACT ? (u32)a.R.perm[pp] : (u32)pp;
+        const int pp = ppn;
+        const float4 r0 = r0n;
+        const u32 row = rown;
+        if (c2 < cl) {
+          ppn = chunk_pos(c2 + 1);
+          r0n = *(const float4 *)(a.R.px + (int64_t)ppn * dpR);
+          rown = EXACT ? (u32)a.R.perm[ppn] : (u32)ppn;
+        }
         const float *rp = a.R.px + (int64_t)pp * dpR;
         float acc[TQ];
-#pragma unroll
-        for (int i = 0; i < TQ; ++i) acc[i] = 0.f;
-        for (int t = 0; t < dpR; t += 4) {
-          const float4 r4 = *(const float4 *)(rp + t);
+        auto step4 = [&](const float4 r4, int t) {
 #pragma unroll
           for (int i = 0; i < TQ; ++i) {
             if (use[i]) {
-              const float4 q4 = *(const float4 *)(qrow[i] + t);
+              const float4 q4 = t == 0 ? qv0[i] : *(const float4 *)(qrow[i] + t);
               float df = q4.x - r4.x;
               acc[i] = __builtin_fmaf(df, df, acc[i]);
               df = q4.y - r4.y;
@@ -203,7 +221,11 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
               acc[i] = __builtin_fmaf(df, df, acc[i]);
             }
           }
-        }
+        };
+#pragma unroll
+        for (int i = 0; i < TQ; ++i) acc[i] = 0.f;
+        step4(r0, 0);
+        for (int t = 4; t < dpR; t += 4) step4(*(const float4 *)(rp + t), t);
         const int nvalid = __popcll(wave_ballot(valid));
 #pragma unroll
         for (int i = 0; i < TQ; ++i) {
@@ -219,24 +241,29 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
             evals += nvalid;
           }
         }
+        ++nchunks;
         bool need_merge = false;
 #pragma unroll
         for (int i = 0; i < TQ; ++i) need_merge |= cnt[i] >= 64;
         if (need_merge) {
+          const u64 cm0 = a.tile_prof ? wall_clock64() : 0;
           __syncthreads();
 #pragma unroll
           for (int i = 0; i < TQ; ++i)
             if (cnt[i] >= 64) merge(i);
+          if (a.tile_prof) mticks += wall_clock64() - cm0;
         }
       }
     }
   }
   }
+  const u64 clk_sweep = a.tile_prof ? wall_clock64() : 0;
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < TQ; ++i)
     if (cnt[i] > 0) merge(i);
   __syncthreads();
+  const u64 clk_merge = a.tile_prof ? wall_clock64() : 0;
 
   // ---- results
   for (int i = 0; i < nq; ++i) {
@@ -252,8 +279,12 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
     u64 *tp = a.tile_prof + 8 * (size_t)blockIdx.x;
     tp[0] = clk0;
     tp[1] = wall_clock64();
-    tp[2] = 0;
-    tp[4] = tp[5] = tp[6] = tp[7] = 0;
+    // phase clocks as the MFMA kernel's record: no seed (4 = 7 = start),
+    // sweep end, final merges end; B groups = 64-row chunks
+    tp[2] = (nchunks << 32) | min(mticks, (u64)0xFFFFFFFFull);
+    tp[4] = tp[7] = clk0;
+    tp[5] = clk_sweep;
+    tp[6] = clk_merge;
     tp[3] = (min(nvisit, (u64)0xFFFF) << 48) | (min(nsweepb, (u64)0xFFFF) << 16) | (u64)min(np, 0xFFFF);
   }
   if (lane == 0) {

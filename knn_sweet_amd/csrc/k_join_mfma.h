@@ -35,10 +35,16 @@
 //     fp32 operands:          cdot = gamma_d
 //     one bf16 product:       cdot = 2 u_b + u_b^2 + gamma_d (1 + u_b)^2
 //     bf16 hi/lo, 3 products: cdot = 3.1 u_b^2 + 1.02 gamma_{3d}
+//     fp8 (e4m3), rows scaled:  cdot = 2 dl + dl^2 + gamma_d (1 + dl)^2,
+//                               dl = 2^-4 + sqrt(d) 2^-16
 // (hi = bf16(x), lo = bf16(x - hi) leaves |x - hi - lo| <= u_b^2 |x|; the
 // dropped lo*lo product and the two residuals are <= 3.1 u_b^2 |q_j r_j|;
-// every bf16 x bf16 product is exact in FP32), all scaled by 1.01 on the
-// host.  Integer inputs with 4 d (2 max|x|)^2 <
+// every bf16 x bf16 product is exact in FP32.  FP8: a row x scaled by 2^e
+// into [64, 128) rounds to nearest with 3 fraction bits, |error| <= 2^-4 |x_j|
+// for normal values and <= 2^-10 (subnormals of either e4m3 encoding) in
+// scaled units, i.e. 2^-10 2^-e <= 2^-16 max|x| <= 2^-16 ||x||; so
+// ||x~ - x|| <= dl ||x||, and every fp8 x fp8 product is exact in FP32), all
+// scaled by 1.01 on the host.  Integer inputs with 4 d (2 max|x|)^2 <
 // 2^24 make every FP32 step exact (E = 0, exact mode).
 //
 // List keys are (U = s_f + E, row), where E is the value of the entry's
@@ -113,15 +119,21 @@ template <>
 struct BOp<3> {
   bf16x8_t h, l;
 };
+template <>
+struct BOp<8> {
+  long h;  // 8 e4m3 values
+};
 
 template <int KCAP, bool EXACT, int PREC, int DK, int W>
 __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   // W waves per workgroup; with W > 1 the B groups are double-buffered (the
   // next group's loads are in flight while this one's MFMAs run)
   constexpr bool DB = W > 1;
-  // PREC: 0 fp32 operands, 1 bf16, 3 bf16 hi/lo (three products)
-  constexpr bool BF16 = PREC != 0;
+  // PREC: 0 fp32 operands, 1 bf16, 3 bf16 hi/lo (three products), 8 fp8
+  // (e4m3) rows scaled per row by a power of two
+  constexpr bool BF16 = PREC == 1 || PREC == 3;
   constexpr bool SPLIT = PREC == 3;
+  constexpr bool FP8 = PREC == 8;
   constexpr int NS = DK / 32;  // 32-dimension steps
   // 16-column blocks loaded together (B registers: NB * NS * 4 or 8 VGPRs)
   constexpr int NB = DB ? (DK <= 128 ? 2 : 1) : (DK <= 128 ? 4 : 2);
@@ -401,6 +413,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
           const float *rep = a.reps32 + (int64_t)b * dk + 8 * g;
           typename std::conditional<BF16, bf16x8_t, float[8]>::type av[NS];
           bf16x8_t avl[SPLIT ? NS : 1];
+          long av8[FP8 ? NS : 1];
           float part = 0.f;
   #pragma unroll
           for (int s = 0; s < NS; ++s) {
@@ -429,6 +442,30 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   #pragma unroll
           for (int i = 0; i < 4; ++i) nqr[i] = fsh[4 * g + i];
           wave_sync();
+          // FP8: row c16 of q - c scaled by 2^e so that its largest magnitude
+          // lies in [64, 128), rounded to e4m3; iqr[i] = 2^-e of row 4g + i
+          float iqr[4] = {1.f, 1.f, 1.f, 1.f};
+          if constexpr (FP8) {
+            float mx = 0.f;
+  #pragma unroll
+            for (int s2 = 0; s2 < NS; ++s2)
+  #pragma unroll
+              for (int j = 0; j < 8; ++j) mx = fmaxf(mx, fabsf(av[s2][j]));
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            const int e = mx > 0.f ? 6 - ilogbf(mx) : 0;
+  #pragma unroll
+            for (int s2 = 0; s2 < NS; ++s2) {
+              int lo = __builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(av[s2][0], e), ldexpf(av[s2][1], e), 0, false);
+              lo = __builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(av[s2][2], e), ldexpf(av[s2][3], e), lo, true);
+              int hi = __builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(av[s2][4], e), ldexpf(av[s2][5], e), 0, false);
+              hi = __builtin_amdgcn_cvt_pk_fp8_f32(ldexpf(av[s2][6], e), ldexpf(av[s2][7], e), hi, true);
+              av8[s2] = (long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+            }
+            const float iq = ldexpf(1.f, -e);
+  #pragma unroll
+            for (int i = 0; i < 4; ++i) iqr[i] = __shfl(iq, 4 * g + i);
+          }
 
           // Ei: the rigorous bound; Ed: the bound the key carries (its code's
           // decoded value, >= Ei, with per-entry codes; Ei without)
@@ -475,6 +512,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
             struct Group {
               BT bo[NB][NS];
               float nrv[NB];
+              float isv[FP8 ? NB : 1];  // FP8: the rows' inverse scales
               u32 rowv[NB];
               bool validv[NB];
             };
@@ -492,6 +530,8 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
                     G.bo[nb][s].l = *(const bf16x8_t *)(rr + dk);
                   } else if constexpr (BF16) {
                     G.bo[nb][s].h = *(const bf16x8_t *)((const __bf16 *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s);
+                  } else if constexpr (FP8) {
+                    G.bo[nb][s].h = *(const long *)((const unsigned char *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s);
                   } else {
                     const float *rr = (const float *)a.rc + (int64_t)pp * dk + 8 * g + 32 * s;
                     G.bo[nb][s].f0 = *(const f32x4_t *)rr;
@@ -499,6 +539,7 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
                   }
                 }
                 G.nrv[nb] = a.nr[pp];
+                if constexpr (FP8) G.isv[nb] = a.rinv[pp];
                 G.rowv[nb] = EXACT ? (u32)a.R.perm[pp] : (u32)pp;
               }
             };
@@ -540,6 +581,10 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
   #pragma unroll
                   for (int nb = 0; nb < NB; ++nb)
                     accv[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[s], G.bo[nb][s].h, accv[nb], 0, 0, 0);
+                } else if constexpr (FP8) {
+  #pragma unroll
+                  for (int nb = 0; nb < NB; ++nb)
+                    accv[nb] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av8[s], G.bo[nb][s].h, accv[nb], 0, 0, 0);
                 } else {
   #pragma unroll
                   for (int j = 0; j < 8; ++j)
@@ -562,7 +607,10 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
                 bool anyp = false;
   #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                  const float sf = fmaxf((nqr[i] + nrb) - 2.f * acc[i], 0.f);
+                  // FP8: the dot of the scaled rows, times both inverse scales
+                  // (powers of two: exact)
+                  const float dotv = FP8 ? acc[i] * (iqr[i] * G.isv[FP8 ? nb : 0]) : acc[i];
+                  const float sf = fmaxf((nqr[i] + nrb) - 2.f * dotv, 0.f);
                   if (EXACT) {
                     key[i] = make_key(sf, row);
                     pass[i] = valid && use[i] && key[i] <= thr[i];

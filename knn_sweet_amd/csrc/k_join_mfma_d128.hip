@@ -1,8 +1,9 @@
 // k_join_mfma_d128.hip -- the MFMA join kernels for operand width 128
-// (plain FP32, bf16 and bf16 hi/lo filters), one translation unit per width
+// (plain FP32, bf16, bf16 hi/lo and fp8 filters), one translation unit per width
 // so the widths compile in parallel.
 #include "k_join_mfma.h"
 
 KJ_DK_INST(0, 128)
 KJ_DK_INST(1, 128)
 KJ_DK_INST(3, 128)
+KJ_DK_INST(8, 128)

@@ -98,6 +98,7 @@ struct JoinArgs {
   int dk;                // padded width of rc and reps32: roundup(d, 32)
   const float *reps32;   // [mR][dk] reference representatives, FP32, zero padded
   const float *rn32;     // [mR] their squared norms
+  const float *rinv;     // [nR] fp8 filter: inverse power-of-two scale of each centred row
   float c1, c2;          // error-bound coefficients (see k_join_mfma.h)
   // optional per-tile record {start clock, end clock, clusters listed, clusters visited}
   u64 *tile_prof;
@@ -116,14 +117,19 @@ size_t join_lds_bytes(int tq, int kcap);
 hipError_t launch_join(const JoinArgs &a, int tq, int64_t max_tiles, hipStream_t s);
 
 // k_center.hip
-// planes: 0 fp32 rows, 1 bf16 rows, 2 bf16 hi/lo rows
+// planes: 0 fp32 rows, 1 bf16 rows, 2 bf16 hi/lo rows, 3 fp8 rows scaled per
+// row by a power of two (rinv[nR]: the inverse scales; unused otherwise)
 void launch_center(const SideIndex &Ri, int d, int dk, int planes, void *rc, float *nr, float *reps32, float *rn32,
-                   hipStream_t s);
+                   float *rinv, hipStream_t s);
 
 // k_join_mfma.hip (kernels per operand width in k_join_mfma_d*.hip)
 hipError_t launch_join_mfma_f32(const JoinArgs &a, int64_t max_tiles, hipStream_t s);
 // terms: 1 single bf16 product, 3 compensated hi/lo products
 hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles, hipStream_t s);
+// fp8 (e4m3) filter, rows scaled per row; operand widths 64, 96, 128
+hipError_t launch_join_mfma_fp8(const JoinArgs &a, int64_t max_tiles, hipStream_t s);
+hipError_t launch_join_mfma_fp8_multi(const JoinArgs &proto, const JoinArgs *argv, const Tile *tiles, int nslots,
+                                      hipStream_t s);
 // queries per workgroup tile of the MFMA join (16 per wave; want_waves > 1
 // asks for the multi-wave workgroup)
 int join_mfma_tile_q(int kcap, int dk, int want_waves);

@@ -99,7 +99,7 @@ struct NoDrop {
 // be elsewhere: only wave-level ordering is used).  Returns the wave-wide minimum of
 // drop(key) over the real keys that fell off the end (INFINITY if none).
 //
-// A 64-key buffer into a list of up to 256 is merged by counting: every lane reads the
+// A buffer of up to 128 keys into a list of up to 256 is merged by counting: every lane reads the
 // buffer keys as LDS broadcasts and counts, for its own buffer and list keys,
 // the buffer keys below them; a buffer key's list rank is a binary search.
 // Each key's new position is then its index among the smaller keys of both
