@@ -18,7 +18,13 @@
 //     buffer is bitonic-sorted and rank-merged into the query's sorted
 //     KCAP-key list, and the threshold drops to the list's k-th key.
 // Integer inputs (exact mode): binary32 squared distances are exact, the key
-// row is the original row, the list is the answer.  Real-valued inputs: the
+// row is the original row.  There the buffer is an unsorted pool of
+// pool_cap(KCAP) keys: when it fills, a select by counting (the k-th
+// smallest distance found bit by bit, one ballot per register) drops every
+// key that already has k smaller ones and lowers the threshold, with no
+// sort; the pool is sorted once, at the end, and its first k keys are the
+// answer.  (Sorted merges every 64 appends took 60 % of a tile at d = 3,
+// k = 200.)  Real-valued inputs: the
 // binary32 key is a filter with relative error <= delta; every candidate
 // within fac = (1+delta)/(1-delta) of the k-th filter key is kept, then
 // re-scored with the oracle's binary64 sequence and sorted by (binary64
@@ -30,13 +36,141 @@
 #include "wave_sort.h"
 #include "join_common.h"
 
+// Waves per workgroup, each on its own tile.  One: tile lengths vary
+// several-fold, and a workgroup of several waves holds its slot until its
+// slowest wave ends (measured at d = 3, k = 200: 4 waves per workgroup ran
+// 30 % fewer waves at once than single waves).  The kernel's workgroup
+// barriers order one wave's LDS accesses and rely on this being 1.
+#define VALU_WPB 1
+
+// Exact mode keeps each query's candidates in an unsorted pool of
+// pool_cap(KCAP) keys instead of a sorted list: at least 64 keys above the
+// largest k it serves plus one chunk of 64 appends (KCAP >= max(k, 64)).
+__host__ __device__ constexpr int pool_cap(int kcap) { return 2 * kcap > kcap + 256 ? 2 * kcap : kcap + 256; }
+
+// LDS words (u64) of one wave
+__host__ __device__ constexpr size_t valu_wave_words(int tq, int kcap, bool exact) {
+  return exact ? (size_t)tq * pool_cap(kcap) : (size_t)tq * kcap + (size_t)tq * KNNJ_BCAP + (size_t)kcap;
+}
+
+// number of this wave's values dv[0..EP) (all lanes) that are <= t
+template <int EP>
+__device__ __forceinline__ int wave_count_le(const u32 (&dv)[EP], u32 t) {
+  int c = 0;
+#pragma unroll
+  for (int u = 0; u < EP; ++u) c += __popcll(wave_ballot(dv[u] <= t));
+  return c;
+}
+
+__device__ __forceinline__ u32 wave_max_u32(u32 v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = max(v, (u32)__shfl_xor((int)v, m));
+  return v;
+}
+
+// Exact-mode pool shrink.  pool[0..cnt) holds cnt >= k distinct keys whose
+// high words are exact integer squared distances (binary32).  D*, the k-th
+// smallest distance, is found bit by bit from the top (a count of the
+// pool's distances <= t per bit: one ballot per register).  Every key of
+// distance > D* has at least k smaller keys, so it can never be among the k
+// nearest; the pool keeps the keys of distance <= D* and the pass threshold
+// becomes (D*, any row).  When the ties at D* would leave less than a chunk
+// of room (or exact_k), the (k - #{distance < D*})-th smallest row among the
+// ties is found the same way and exactly the k smallest keys stay, the
+// threshold then being just below the k-th key.  Returns the threshold
+// (a key passes when <= it) and sets cnt.  rowbits: width of the row ids.
+template <int P>
+__device__ __forceinline__ u64 pool_shrink(u64 *pool, int &cnt, int k, int rowbits, bool exact_k) {
+  constexpr int EP = P / 64;
+  const int lane = threadIdx.x & 63;
+  u64 v[EP];
+  u32 dv[EP];
+  u32 dmax = 0;
+#pragma unroll
+  for (int u = 0; u < EP; ++u) {
+    const int e = lane + 64 * u;
+    v[u] = e < cnt ? pool[e] : KEY_MAX;
+    dv[u] = e < cnt ? (u32)key_dist(v[u]) : 0xFFFFFFFFu;
+    if (e < cnt) dmax = max(dmax, dv[u]);
+  }
+  dmax = wave_max_u32(dmax);
+  u32 r = 0;
+  for (int b = 31 - __builtin_clz(dmax | 1u); b >= 0; --b) {
+    const u32 t = r | ((1u << b) - 1u);
+    if (wave_count_le(dv, t) < k) r |= 1u << b;
+  }
+  const int cle = wave_count_le(dv, r);
+  u64 tau, thr;
+  if (!exact_k && cle <= P - 128) {
+    tau = ((u64)__float_as_uint((float)r) << 32) | 0xFFFFFFFFull;
+    thr = tau;
+  } else {
+    const int kk = k - (r > 0 ? wave_count_le(dv, r - 1u) : 0);
+    u32 rr = 0;
+    for (int b = rowbits - 1; b >= 0; --b) {
+      const u32 t = rr | ((1u << b) - 1u);
+      int c = 0;
+#pragma unroll
+      for (int u = 0; u < EP; ++u) c += __popcll(wave_ballot(dv[u] == r && key_row(v[u]) <= t));
+      if (c < kk) rr |= 1u << b;
+    }
+    tau = ((u64)__float_as_uint((float)r) << 32) | rr;
+    thr = tau - 1;
+  }
+  wave_sync();
+  int base = 0;
+#pragma unroll
+  for (int u = 0; u < EP; ++u) {
+    const bool keep = v[u] <= tau;
+    const u64 m = wave_ballot(keep);
+    if (keep) pool[base + (int)lanes_below(m)] = v[u];
+    base += __popcll(m);
+  }
+  wave_sync();
+  cnt = base;
+  return thr;
+}
+
+// Exact mode: row qo of the answer from a query's pool (cnt keys): the k
+// smallest keys, sorted, are the answer (ties by original row).
+template <int KCAP, int P>
+__device__ __forceinline__ void pool_emit(const JoinArgs &a, u64 *pool, int cnt, int64_t qo, int rowbits) {
+  const int lane = threadIdx.x & 63;
+  const int k = a.k;
+  if (cnt > KCAP) pool_shrink<P>(pool, cnt, k, rowbits, true);
+  constexpr int E = KCAP / 64;
+  u64 w[E];
+#pragma unroll
+  for (int u = 0; u < E; ++u) w[u] = lane + 64 * u < cnt ? pool[lane + 64 * u] : KEY_MAX;
+  wave_sort_u64<E>(w);
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int j = lane + 64 * u;
+    if (j < k) {
+      const float s = key_dist(w[u]);
+      const int32_t row = (int32_t)key_row(w[u]);
+      const float dist = a.out_mode == 1 ? s : (float)sqrt((double)s);
+      if (a.out_mode == 2) {
+        a.out_idx[(qo * k + j) * 2] = row;
+        a.out_idx[(qo * k + j) * 2 + 1] = __float_as_int(dist);
+      } else {
+        a.out_idx[qo * k + j] = row;
+        a.out_dist[qo * k + j] = dist;
+      }
+    }
+  }
+}
+
 template <int TQ, int KCAP, bool EXACT>
-__device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
+__device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile, const int slot) {
   extern __shared__ u64 smem[];
-  u64 *lists = smem;                       // [TQ][KCAP]
-  u64 *bufs = smem + TQ * KCAP;            // [TQ][BCAP]
-  u64 *tmp = bufs + TQ * KNNJ_BCAP;        // [KCAP]
-  const int lane = threadIdx.x;
+  constexpr int P = pool_cap(KCAP);
+  u64 *wsm = smem + (threadIdx.x >> 6) * valu_wave_words(TQ, KCAP, EXACT);
+  u64 *lists = wsm;                        // real mode: [TQ][KCAP] sorted lists
+  u64 *bufs = wsm + TQ * KCAP;             // real mode: [TQ][BCAP] buffers
+  u64 *tmp = bufs + TQ * KNNJ_BCAP;        // real mode: [KCAP]
+  u64 *pools = wsm;                        // exact mode: [TQ][P] pools
+  const int lane = threadIdx.x & 63;
   const u64 clk0 = a.tile_prof ? wall_clock64() : 0;
   u64 nvisit = 0, nsweepb = 0;  // diagnostics: visits, screened blocks
   u64 nchunks = 0, mticks = 0;   // diagnostics: 64-row chunks computed, ticks in merges
@@ -45,11 +179,14 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
   const int d = a.d, dpR = a.R.dpad, dpQ = a.Q.dpad, k = a.k;
   const float eps = dist_relerr(d);
 
+  if constexpr (!EXACT) {
 #pragma unroll
-  for (int i = 0; i < TQ; ++i)
+    for (int i = 0; i < TQ; ++i)
 #pragma unroll
-    for (int u = 0; u < KCAP / 64; ++u) lists[i * KCAP + lane + 64 * u] = KEY_MAX;
-  __syncthreads();
+      for (int u = 0; u < KCAP / 64; ++u) lists[i * KCAP + lane + 64 * u] = KEY_MAX;
+    __syncthreads();
+  }
+  const int rowbits = 32 - __builtin_clz((u32)max(a.R.n - 1, (int64_t)1));
 
   const double th = a.theta[A];
   const float T0 = (float)(th * th * (1.0 + a.delta) * (1.0 + 1e-6));
@@ -68,22 +205,26 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
 #pragma unroll
   for (int i = 0; i < TQ; ++i) qv0[i] = *(const float4 *)qrow[i];
 
+  // the candidates appended for query i since its last merge (real mode:
+  // buffer into the sorted list) or shrink (exact mode: pool)
   auto merge = [&](int i) {
-    wave_merge<KCAP>(lists + i * KCAP, bufs + i * KNNJ_BCAP, cnt[i], tmp);
-    cnt[i] = 0;
     ++nmerge;
-    const u64 kth = lists[i * KCAP + k - 1];
-    if (kth != KEY_MAX) {
-      u64 t;
-      if (EXACT) {
-        t = kth - 1;
-      } else {
-        const float T = (float)((double)key_dist(kth) * a.fac);
-        t = ((u64)__float_as_uint(T) << 32) | 0xFFFFFFFFull;
-      }
+    if constexpr (EXACT) {
+      const u64 t = pool_shrink<P>(pools + i * P, cnt[i], k, rowbits, false);
       if (t < thr[i]) thr[i] = t;
+    } else {
+      wave_merge<KCAP>(lists + i * KCAP, bufs + i * KNNJ_BCAP, cnt[i], tmp);
+      cnt[i] = 0;
+      const u64 kth = lists[i * KCAP + k - 1];
+      if (kth != KEY_MAX) {
+        const float T = (float)((double)key_dist(kth) * a.fac);
+        const u64 t = ((u64)__float_as_uint(T) << 32) | 0xFFFFFFFFull;
+        if (t < thr[i]) thr[i] = t;
+      }
     }
   };
+  // appends from one more chunk would not fit
+  auto full = [&](int i) { return EXACT ? cnt[i] > P - 64 : cnt[i] >= 64; };
 
   const int np = a.pcnt[A];
   const int32_t *pl = a.plist + (int64_t)A * a.R.m;
@@ -234,7 +375,7 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
             const bool pass = key <= thr[i];
             const u64 pm = wave_ballot(pass);
             if (pm) {
-              if (pass) bufs[i * KNNJ_BCAP + cnt[i] + (int)lanes_below(pm)] = key;
+              if (pass) (EXACT ? pools + i * P : bufs + i * KNNJ_BCAP)[cnt[i] + (int)lanes_below(pm)] = key;
               cnt[i] += __popcll(pm);
               npass += __popcll(pm);
             }
@@ -244,13 +385,13 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
         ++nchunks;
         bool need_merge = false;
 #pragma unroll
-        for (int i = 0; i < TQ; ++i) need_merge |= cnt[i] >= 64;
+        for (int i = 0; i < TQ; ++i) need_merge |= full(i);
         if (need_merge) {
           const u64 cm0 = a.tile_prof ? wall_clock64() : 0;
           __syncthreads();
 #pragma unroll
           for (int i = 0; i < TQ; ++i)
-            if (cnt[i] >= 64) merge(i);
+            if (full(i)) merge(i);
           if (a.tile_prof) mticks += wall_clock64() - cm0;
         }
       }
@@ -259,24 +400,32 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
   }
   const u64 clk_sweep = a.tile_prof ? wall_clock64() : 0;
   __syncthreads();
+  if constexpr (!EXACT) {
 #pragma unroll
-  for (int i = 0; i < TQ; ++i)
-    if (cnt[i] > 0) merge(i);
-  __syncthreads();
+    for (int i = 0; i < TQ; ++i)
+      if (cnt[i] > 0) merge(i);
+    __syncthreads();
+  }
   const u64 clk_merge = a.tile_prof ? wall_clock64() : 0;
 
   // ---- results
-  for (int i = 0; i < nq; ++i) {
-    // the threshold of query i (register array, selected without dynamic indexing)
-    u64 ti = thr[0];
+  if constexpr (EXACT) {
 #pragma unroll
-    for (int j = 1; j < TQ; ++j)
-      if (j == i) ti = thr[j];
-    emit_query<KCAP, EXACT>(a, lists + i * KCAP, tile.start + i, A, key_dist(ti),
-                            a.Q.px + (int64_t)(tile.start + i) * dpQ);
+    for (int i = 0; i < TQ; ++i)
+      if (i < nq) pool_emit<KCAP, P>(a, pools + i * P, cnt[i], a.Q.perm[tile.start + i], rowbits);
+  } else {
+    for (int i = 0; i < nq; ++i) {
+      // the threshold of query i (register array, selected without dynamic indexing)
+      u64 ti = thr[0];
+#pragma unroll
+      for (int j = 1; j < TQ; ++j)
+        if (j == i) ti = thr[j];
+      emit_query<KCAP, EXACT>(a, lists + i * KCAP, tile.start + i, A, key_dist(ti),
+                              a.Q.px + (int64_t)(tile.start + i) * dpQ);
+    }
   }
   if (lane == 0 && a.tile_prof) {
-    u64 *tp = a.tile_prof + 8 * (size_t)blockIdx.x;
+    u64 *tp = a.tile_prof + 8 * (size_t)slot;
     tp[0] = clk0;
     tp[1] = wall_clock64();
     // phase clocks as the MFMA kernel's record: no seed (4 = 7 = start),
@@ -294,22 +443,30 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile) {
   }
 }
 
+// Wave w of workgroup b runs tile slot VALU_WPB b + w.
+// At most 128 registers (4 waves per SIMD) for tiles of up to 2 queries and
+// capacity 256: the sweep waits on memory, so waves in flight matter more
+// than registers (larger tiles, and the many-join kernel, would spill).
+#define VALU_WAVES_PER_EU(TQ, KCAP) __attribute__((amdgpu_waves_per_eu((TQ) <= 2 && (KCAP) <= 256 ? 4 : 1)))
+
 template <int TQ, int KCAP, bool EXACT>
-__global__ __launch_bounds__(64) void k_join_valu(JoinArgs a) {
-  if ((int)blockIdx.x >= *a.ntiles) return;
-  const Tile tile = a.tiles[blockIdx.x];
+__global__ __launch_bounds__(64 * VALU_WPB) VALU_WAVES_PER_EU(TQ, KCAP) void k_join_valu(JoinArgs a) {
+  const int slot = (int)blockIdx.x * VALU_WPB + (int)(threadIdx.x >> 6);
+  if (slot >= *a.ntiles) return;
+  const Tile tile = a.tiles[slot];
   if (tile.count <= 0) return;
-  valu_tile<TQ, KCAP, EXACT>(a, tile);
+  valu_tile<TQ, KCAP, EXACT>(a, tile, slot);
 }
 
 // Many joins in one launch (see k_join_mfma_multi).
 template <int TQ, int KCAP, bool EXACT>
-__global__ __launch_bounds__(64) void k_join_valu_multi(const JoinArgs *__restrict__ argv,
-                                                        const Tile *__restrict__ tiles, int nslots) {
-  if ((int)blockIdx.x >= nslots) return;
-  const Tile tile = tiles[blockIdx.x];
+__global__ __launch_bounds__(64 * VALU_WPB) void k_join_valu_multi(const JoinArgs *__restrict__ argv,
+                                                                   const Tile *__restrict__ tiles, int nslots) {
+  const int slot = (int)blockIdx.x * VALU_WPB + (int)(threadIdx.x >> 6);
+  if (slot >= nslots) return;
+  const Tile tile = tiles[slot];
   if (tile.count <= 0) return;
-  valu_tile<TQ, KCAP, EXACT>(argv[tile.pad], tile);
+  valu_tile<TQ, KCAP, EXACT>(argv[tile.pad], tile, slot);
 }
 
 int join_tile_q(int kcap) {
@@ -320,27 +477,35 @@ int join_tile_q(int kcap) {
   return 1;
 }
 
-size_t join_lds_bytes(int tq, int kcap) {
-  return sizeof(u64) * ((size_t)tq * kcap + (size_t)tq * KNNJ_BCAP + (size_t)kcap);
+static size_t join_lds_bytes(int tq, int kcap, bool exact) {
+  return sizeof(u64) * VALU_WPB * valu_wave_words(tq, kcap, exact);
+}
+
+template <int TQ, int KCAP, bool EX>
+static hipError_t launch_tq_kcap_ex(const JoinArgs &a, int64_t grid, hipStream_t s, const JoinArgs *argv,
+                                    const Tile *tiles) {
+  const size_t lds = join_lds_bytes(TQ, KCAP, EX);
+  const unsigned blocks = (unsigned)((grid + VALU_WPB - 1) / VALU_WPB);
+  if (argv) {
+    if (lds > 65536)
+      hipFuncSetAttribute((const void *)k_join_valu_multi<TQ, KCAP, EX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds);
+    hipLaunchKernelGGL((k_join_valu_multi<TQ, KCAP, EX>), dim3(blocks), dim3(64 * VALU_WPB), lds, s, argv, tiles,
+                       (int)grid);
+  } else {
+    if (lds > 65536)
+      hipFuncSetAttribute((const void *)k_join_valu<TQ, KCAP, EX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lds);
+    hipLaunchKernelGGL((k_join_valu<TQ, KCAP, EX>), dim3(blocks), dim3(64 * VALU_WPB), lds, s, a);
+  }
+  return hipGetLastError();
 }
 
 template <int TQ, int KCAP>
 static hipError_t launch_tq_kcap(const JoinArgs &a, int64_t grid, hipStream_t s, const JoinArgs *argv,
                                  const Tile *tiles) {
-  const size_t lds = join_lds_bytes(TQ, KCAP);
-  if (argv) {
-    if (a.exact)
-      hipLaunchKernelGGL((k_join_valu_multi<TQ, KCAP, true>), dim3((unsigned)grid), dim3(64), lds, s, argv, tiles,
-                         (int)grid);
-    else
-      hipLaunchKernelGGL((k_join_valu_multi<TQ, KCAP, false>), dim3((unsigned)grid), dim3(64), lds, s, argv, tiles,
-                         (int)grid);
-  } else if (a.exact) {
-    hipLaunchKernelGGL((k_join_valu<TQ, KCAP, true>), dim3((unsigned)grid), dim3(64), lds, s, a);
-  } else {
-    hipLaunchKernelGGL((k_join_valu<TQ, KCAP, false>), dim3((unsigned)grid), dim3(64), lds, s, a);
-  }
-  return hipGetLastError();
+  return a.exact ? launch_tq_kcap_ex<TQ, KCAP, true>(a, grid, s, argv, tiles)
+                 : launch_tq_kcap_ex<TQ, KCAP, false>(a, grid, s, argv, tiles);
 }
 
 template <int KCAP>
