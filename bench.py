@@ -344,7 +344,9 @@ def bench_batch(args, kj, cfg):
     ctx = kj.Context(0)
     # the batch's default (no Lloyd step) for both the batched call and the loop
     bopts = dict(refine_iters=args.refine_iters or -1, split_rounds=args.split_rounds)
-    gopts = dict(bopts, graph=args.graph)
+    # the batched call autotunes itself on the warm-up call (Lloyd step, VALU
+    # tile, k-select capacity, MFMA tile shape); the loop keeps the defaults
+    gopts = dict(bopts, graph=args.graph, autotune=not args.no_autotune)
     outs, _ = ctx.join_batch(dev, **gopts)  # warm-up; with --graph it captures the plan
     for _ in range(max(0, args.warmup - 1)):
         ctx.join_batch(dev, outputs=outs, **gopts)
@@ -373,6 +375,7 @@ def bench_batch(args, kj, cfg):
         "stages_ms": {"index_bounds_tiles": st["t_index_ms"], "join": t_join, "fallback": st["t_fallback_ms"],
                       "total": st["t_total_ms"]},
         "fused_launches": st["n_launches"], "graph": {0: "eager", 1: "captured", 2: "replayed"}[st["graph"]],
+        "autotune_choice": st["tuned"],
         "pruning_ratio": st["pruning_ratio"],
         "dist_evals": st["n_dist_evals"], "n_overflow": st["n_overflow"],
     }

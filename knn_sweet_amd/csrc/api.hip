@@ -70,8 +70,13 @@ struct knnj_ctx {
   static constexpr int kAux = 4;
   hipStream_t aux[kAux] = {};
   hipEvent_t aux_ev[kAux + 1] = {};
-  // autotune: problem shape -> chosen candidate (index into the candidate list)
-  std::map<std::vector<int64_t>, int> tuned;
+  // autotune: problem shape -> the chosen options and their candidate number
+  // (1 + index over the first-phase list, then the second-phase variants)
+  struct Tuned {
+    knnj_opts opts;
+    int id;
+  };
+  std::map<std::vector<int64_t>, Tuned> tuned;
   // knnj_join_batch with opts.graph: the captured index builds + fused joins
   // of the last plan, and that plan's signature
   hipGraphExec_t batch_exec = nullptr;
@@ -618,6 +623,16 @@ void fill_side_index(Inst &I, bool ref_side) {
 
 // One side's cluster index: sampled representatives, Lloyd refinement,
 // assignment, radii, and the cluster-sorted permuted copy.
+// The context's auxiliary streams (default priority) and their events,
+// created on first use.
+knnj_status ensure_aux(knnj_ctx *c) {
+  if (!c->aux[0]) {
+    for (auto &x : c->aux) HIPCHK(hipStreamCreate(&x));
+    for (auto &e : c->aux_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  return KNNJ_OK;
+}
+
 knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
   SideBufs &b = ref_side ? I.rb : I.qb;
   const float *X = ref_side ? I.dR : I.dQ;
@@ -672,12 +687,27 @@ int center_planes(const Inst &I) { return I.fp8 ? 3 : I.bf16 ? (I.bf16x1 ? 1 : 2
 
 // Index build, centring, bounds and the tile list of one join.  Optional
 // events are recorded after the index (with centring) and after the bounds.
-knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_index, hipEvent_t ev_bounds) {
+// fork: when set (a single join with two sides to index), the query side is
+// built on that auxiliary stream while the reference side is built on s;
+// the two sides share no buffer, and s waits for the query side before the
+// bounds.
+knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_index, hipEvent_t ev_bounds,
+                          hipStream_t fork = nullptr) {
+  const bool two = !I.pre && !I.self && fork;
+  if (two) {
+    HIPCHK(hipEventRecord(c->aux_ev[knnj_ctx::kAux], s));
+    HIPCHK(hipStreamWaitEvent(fork, c->aux_ev[knnj_ctx::kAux], 0));
+    TRY(enqueue_side(c, I, false, fork));
+  }
   if (!I.pre) TRY(enqueue_side(c, I, true, s));
   if (I.self)
     I.Qi = I.Ri;
-  else
+  else if (!two)
     TRY(enqueue_side(c, I, false, s));
+  if (two) {
+    HIPCHK(hipEventRecord(c->aux_ev[0], fork));
+    HIPCHK(hipStreamWaitEvent(s, c->aux_ev[0], 0));
+  }
   if (I.mfma && !I.pre) {
     launch_center(I.Ri, I.d, I.dk, center_planes(I), I.rc, I.nr, I.reps32, I.rn32, I.rinv, s);
     HIPCHK(hipGetLastError());
@@ -957,7 +987,14 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
   TRY(alloc_inst(A, I, nullptr));
 
   HIPCHK(hipEventRecord(ev[1], s));
-  TRY(enqueue_index(c, I, s, ev[2], ev[3]));
+  // the query side's index on an auxiliary stream beside the reference
+  // side's (the allocation-free path creates nothing and stays on one stream)
+  hipStream_t fork = nullptr;
+  if (!A.no_alloc && !I.pre && !I.self) {
+    TRY(ensure_aux(c));
+    fork = c->aux[0];
+  }
+  TRY(enqueue_index(c, I, s, ev[2], ev[3], fork));
   int32_t nslots = 0;
   HIPCHK(hipMemcpyAsync(&nslots, I.counters, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -1219,6 +1256,88 @@ std::vector<knnj_opts> tune_candidates(const knnj_opts &o, const knnj_matrix &R,
   return v;
 }
 
+// Second-phase autotune variants of the first phase's winner, for the fields
+// the caller left open: the k-select capacity doubled (kmax: the largest k
+// served; a larger pool or list trades LDS, hence waves in flight, for fewer
+// merges and overflow re-runs) and, where the matrix cores may run, the
+// 4-wave lockstep tile shape (reference rows shared by 64 queries through
+// L2, one barrier per visited cluster).
+std::vector<knnj_opts> tune_variants(const knnj_opts &best, const knnj_opts &o, int kmax, bool mfma) {
+  std::vector<knnj_opts> v;
+  if (!o.kcap) {
+    knnj_opts x = best;
+    x.kcap = pow2_at_least(std::max(2 * kmax, 64));
+    if (x.kcap <= KNNJ_MAX_K) v.push_back(x);
+  }
+  if (!o.tile_waves && mfma) {
+    knnj_opts x = best;
+    x.tile_waves = 4;
+    v.push_back(x);
+  }
+  return v;
+}
+
+bool tune_may_use_mfma(const knnj_opts &o, int d) {
+  if (o.kernel == KNNJ_KERNEL_VALU) return false;
+  return o.kernel == KNNJ_KERNEL_MFMA || (d >= 32 && d <= 256) || o.precision == KNNJ_PREC_BF16 ||
+         o.precision == KNNJ_PREC_BF16X1 || o.precision == KNNJ_PREC_FP8;
+}
+
+// Times run(opts) for each candidate (each writes the same exact answer) and
+// returns the index of the fastest; *st gets the fastest run's statistics.
+template <class Run>
+knnj_status time_candidates(knnj_ctx *c, const std::vector<knnj_opts> &cand, Run run, int *best, double *best_ms,
+                            knnj_stats *st_best) {
+  knnj_stats st{};
+  for (size_t i = 0; i < cand.size(); ++i) {
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    TRY(run(cand[i], &st));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (*best < 0 || ms < *best_ms) {
+      *best = (int)i;
+      *best_ms = ms;
+      if (st_best) *st_best = st;
+    }
+  }
+  return KNNJ_OK;
+}
+
+// The two-phase search: the first-phase list, then the winner's variants;
+// the chosen options are cached under sig for later calls of that shape.
+template <class Run>
+knnj_status autotune_call(knnj_ctx *c, const std::vector<int64_t> &sig, const std::vector<knnj_opts> &cand,
+                          const knnj_opts &o, int kmax, bool mfma, Run run, knnj_stats *stats) {
+  auto it = c->tuned.find(sig);
+  if (it != c->tuned.end()) {
+    const knnj_status e = run(it->second.opts, stats);
+    if (stats) stats->tuned = it->second.id;
+    return e;
+  }
+  int b1 = -1;
+  double ms1 = 0.0;
+  knnj_stats st1{};
+  TRY(time_candidates(c, cand, run, &b1, &ms1, &st1));
+  knnj_ctx::Tuned t{cand[b1], 1 + b1};
+  const std::vector<knnj_opts> var = tune_variants(cand[b1], o, kmax, mfma);
+  int b2 = -1;
+  double ms2 = 0.0;
+  knnj_stats st2{};
+  TRY(time_candidates(c, var, run, &b2, &ms2, &st2));
+  if (b2 >= 0 && ms2 < ms1) {
+    t = knnj_ctx::Tuned{var[b2], 1 + (int)cand.size() + b2};
+    st1 = st2;
+  }
+  c->tuned[sig] = t;
+  if (stats) {
+    *stats = st1;
+    stats->tuned = t.id;
+  }
+  return KNNJ_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1237,36 +1356,14 @@ knnj_status knnj_join(knnj_ctx *c, knnj_matrix R, knnj_matrix Q, int32_t k, cons
   const int64_t ldR = R.ld ? R.ld : R.d, ldQ = Q.ld ? Q.ld : Q.d;
   const bool self = R.data == Q.data && R.n == Q.n && ldR == ldQ && R.on_device == Q.on_device;
   auto lg = [](int64_t n) { return (int64_t)std::ceil(std::log2((double)std::max<int64_t>(n, 1))); };
-  const std::vector<int64_t> sig = {R.d, k, lg(R.n), lg(Q.n), self, o.precision, o.kernel, o.m_r, o.m_q,
-                                    o.refine_iters, o.tile_q, o.brute_force};
+  std::vector<int64_t> sig = {R.d, k, lg(R.n), lg(Q.n), self, o.precision, o.kernel, o.m_r, o.m_q,
+                              o.refine_iters, o.tile_q, o.brute_force};
+  sig.push_back(o.kcap);
+  sig.push_back(o.tile_waves);
   const std::vector<knnj_opts> cand = tune_candidates(o, R, Q, self);
-  auto it = c->tuned.find(sig);
-  if (it != c->tuned.end()) {
-    const knnj_status e = join_once(c, R, Q, k, cand[it->second], out_idx, out_dist, stats);
-    if (stats) stats->tuned = 1 + it->second;
-    return e;
-  }
-  // first call of this shape: time every candidate (each writes the same
-  // exact answer), keep the fastest
-  int best = 0;
-  double best_ms = 0.0;
-  knnj_stats st{};
-  for (size_t i = 0; i < cand.size(); ++i) {
-    HIPCHK(hipSetDevice(c->device));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    const auto t0 = std::chrono::steady_clock::now();
-    TRY(join_once(c, R, Q, k, cand[i], out_idx, out_dist, &st));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (i == 0 || ms < best_ms) {
-      best = (int)i;
-      best_ms = ms;
-      if (stats) *stats = st;
-    }
-  }
-  c->tuned[sig] = best;
-  if (stats) stats->tuned = 1 + best;
-  return KNNJ_OK;
+  return autotune_call(
+      c, sig, cand, o, k, tune_may_use_mfma(o, R.d),
+      [&](const knnj_opts &x, knnj_stats *st) { return join_once(c, R, Q, k, x, out_idx, out_dist, st); }, stats);
 }
 
 void knnj_index_destroy(knnj_index *x) {
@@ -1522,31 +1619,10 @@ knnj_status knnj_index_query_noalloc(knnj_ctx *c, const knnj_index *x, knnj_matr
   return index_query_impl(c, x, Q, k, opts, out_idx, out_dist, stats, true);
 }
 
-knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *inst, const knnj_opts *opts,
-                            knnj_stats *stats) {
-  g_err.clear();
-  if (!c) {
-    set_err("context is NULL");
-    return KNNJ_EINVAL;
-  }
-  if (n_inst < 0 || (n_inst > 0 && !inst)) {
-    set_err("bad instance array");
-    return KNNJ_EINVAL;
-  }
-  knnj_opts o{};
-  if (opts) o = *opts;
+// One batched call with fixed options (arguments already checked).
+static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_instance *inst, const knnj_opts &o,
+                                   knnj_stats *stats) {
   knnj_stats st{};
-  if (n_inst == 0) {
-    if (stats) *stats = st;
-    return KNNJ_OK;
-  }
-  for (int i = 0; i < n_inst; ++i) {
-    const knnj_status e = check_join_args(inst[i].R, inst[i].Q, inst[i].k, o, inst[i].out_idx, inst[i].out_dist);
-    if (e != KNNJ_OK) {
-      prefix_err("instance %d: ", i);
-      return e;
-    }
-  }
   HIPCHK(hipSetDevice(c->device));
   hipStream_t s = c->stream;
   hipEvent_t *ev = c->ev;
@@ -1651,10 +1727,7 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
     hargs[id] = I[id].args;
   }
   HIPCHK(hipMemcpyAsync(dargs, hargs.data(), sizeof(JoinArgs) * n_inst, hipMemcpyHostToDevice, s));
-  if (!c->aux[0]) {
-    for (auto &x : c->aux) HIPCHK(hipStreamCreate(&x));
-    for (auto &e : c->aux_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  }
+  TRY(ensure_aux(c));
 
   int n_launch = 0;
   // index builds, bounds and tile lists: independent per instance, so they
@@ -1830,6 +1903,69 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   if (stats) *stats = st;
   return KNNJ_OK;
 }
+
+knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *inst, const knnj_opts *opts,
+                            knnj_stats *stats) {
+  g_err.clear();
+  if (!c) {
+    set_err("context is NULL");
+    return KNNJ_EINVAL;
+  }
+  if (n_inst < 0 || (n_inst > 0 && !inst)) {
+    set_err("bad instance array");
+    return KNNJ_EINVAL;
+  }
+  knnj_opts o{};
+  if (opts) o = *opts;
+  if (n_inst == 0) {
+    if (stats) *stats = knnj_stats{};
+    return KNNJ_OK;
+  }
+  for (int i = 0; i < n_inst; ++i) {
+    const knnj_status e = check_join_args(inst[i].R, inst[i].Q, inst[i].k, o, inst[i].out_idx, inst[i].out_dist);
+    if (e != KNNJ_OK) {
+      prefix_err("instance %d: ", i);
+      return e;
+    }
+  }
+  if (!o.autotune) return join_batch_once(c, n_inst, inst, o, stats);
+  // Batch autotune: the batch's shape (every instance's sizes, d, k and
+  // self-join flag, and the options the caller fixed) keys the choice.
+  // First phase: the Lloyd step off (the batch default) or on, and where an
+  // instance may run on the vector ALUs, their 2 or 4 queries per tile;
+  // second phase: the doubled k-select capacity and the 4-wave lockstep tile.
+  std::vector<int64_t> sig = {-1, n_inst, o.precision, o.kernel, o.m_r, o.m_q, o.refine_iters, o.tile_q,
+                              o.tile_waves, o.kcap, o.brute_force, o.graph, o.split_rounds};
+  int kmax = 1;
+  bool any_valu = false, any_mfma = false;
+  for (int i = 0; i < n_inst; ++i) {
+    const knnj_matrix &R = inst[i].R, &Q = inst[i].Q;
+    const int64_t ldR = R.ld ? R.ld : R.d, ldQ = Q.ld ? Q.ld : Q.d;
+    const bool self = R.data == Q.data && R.n == Q.n && ldR == ldQ && R.on_device == Q.on_device;
+    sig.insert(sig.end(), {R.n, Q.n, (int64_t)R.d, (int64_t)inst[i].k, (int64_t)self});
+    kmax = std::max(kmax, (int)inst[i].k);
+    const bool mf = tune_may_use_mfma(o, R.d);
+    any_mfma |= mf;
+    any_valu |= !mf;
+  }
+  std::vector<knnj_opts> cand;
+  const int refs[2] = {o.refine_iters, 1};
+  const int nref = o.refine_iters ? 1 : 2;
+  const int tqs[3] = {o.tile_q, 2, 4};
+  const int ntq = (any_valu && !o.tile_q) ? 3 : 1;
+  for (int r = 0; r < nref; ++r)
+    for (int t = 0; t < ntq; ++t) {
+      knnj_opts x = o;
+      x.autotune = 0;
+      x.refine_iters = refs[r];
+      x.tile_q = tqs[t];
+      cand.push_back(x);
+    }
+  return autotune_call(
+      c, sig, cand, o, kmax, any_mfma,
+      [&](const knnj_opts &x, knnj_stats *st) { return join_batch_once(c, n_inst, inst, x, st); }, stats);
+}
+
 
 }  // extern "C"
 

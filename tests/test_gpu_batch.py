@@ -133,3 +133,22 @@ def test_batch_graph_replays_match_eager(ctx):
     assert st["graph"] == 1
     for (R, Q), (idx, dist) in zip(inst[:-1], res):
         assert_matches_oracle(R, Q, 16, _host(idx), _host(dist))
+
+
+def test_batch_autotune_picks_and_reuses_an_exact_plan(ctx):
+    """The batch autotuner times its candidates (Lloyd step, VALU tile, then
+    the doubled k-select capacity and the lockstep MFMA tile) on the first
+    call of a batch shape and replays the choice on later calls; every
+    candidate writes the same exact answer, equal to the untuned batch."""
+    inst = datagen.batch_instances(n_inst=9, seed=99, r_lo=400, r_hi=6000)
+    dev = [(_dev(R), _dev(Q), 16) for R, Q in inst]
+    plain, _ = ctx.join_batch(dev)
+    first, st1 = ctx.join_batch(dev, autotune=True)
+    again, st2 = ctx.join_batch(dev, autotune=True)
+    assert st1["tuned"] >= 1 and st2["tuned"] == st1["tuned"]
+    for i, (p, a, b) in enumerate(zip(plain, first, again)):
+        _assert_same(a[0], p[0], f"tuned instance {i} idx")
+        _assert_same(a[1], p[1], f"tuned instance {i} dist")
+        _assert_same(b[0], p[0], f"replayed choice, instance {i} idx")
+    for (R, Q), (idx, dist) in zip(inst, again):
+        assert_matches_oracle(R, Q, 16, _host(idx), _host(dist))
