@@ -70,11 +70,13 @@ struct knnj_ctx {
   static constexpr int kAux = 4;
   hipStream_t aux[kAux] = {};
   hipEvent_t aux_ev[kAux + 1] = {};
-  // autotune: problem shape -> the chosen options and their candidate number
-  // (1 + index over the first-phase list, then the second-phase variants)
+  // autotune: problem shape -> the chosen candidate: its index in the first
+  // phase's list, in the second phase's variants of it (-1: none), and its
+  // number (1 + index over both lists).  The candidates are rebuilt from
+  // each call's own options, so per-call fields (output form, statistics)
+  // are the caller's.
   struct Tuned {
-    knnj_opts opts;
-    int id;
+    int b1, b2, id;
   };
   std::map<std::vector<int64_t>, Tuned> tuned;
   // knnj_join_batch with opts.graph: the captured index builds + fused joins
@@ -1312,22 +1314,25 @@ knnj_status autotune_call(knnj_ctx *c, const std::vector<int64_t> &sig, const st
                           const knnj_opts &o, int kmax, bool mfma, Run run, knnj_stats *stats) {
   auto it = c->tuned.find(sig);
   if (it != c->tuned.end()) {
-    const knnj_status e = run(it->second.opts, stats);
-    if (stats) stats->tuned = it->second.id;
+    const knnj_ctx::Tuned t = it->second;
+    knnj_opts x = cand[t.b1];
+    if (t.b2 >= 0) x = tune_variants(x, o, kmax, mfma)[t.b2];
+    const knnj_status e = run(x, stats);
+    if (stats) stats->tuned = t.id;
     return e;
   }
   int b1 = -1;
   double ms1 = 0.0;
   knnj_stats st1{};
   TRY(time_candidates(c, cand, run, &b1, &ms1, &st1));
-  knnj_ctx::Tuned t{cand[b1], 1 + b1};
+  knnj_ctx::Tuned t{b1, -1, 1 + b1};
   const std::vector<knnj_opts> var = tune_variants(cand[b1], o, kmax, mfma);
   int b2 = -1;
   double ms2 = 0.0;
   knnj_stats st2{};
   TRY(time_candidates(c, var, run, &b2, &ms2, &st2));
   if (b2 >= 0 && ms2 < ms1) {
-    t = knnj_ctx::Tuned{var[b2], 1 + (int)cand.size() + b2};
+    t = knnj_ctx::Tuned{b1, b2, 1 + (int)cand.size() + b2};
     st1 = st2;
   }
   c->tuned[sig] = t;

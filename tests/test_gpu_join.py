@@ -231,6 +231,12 @@ def test_autotune_gives_the_same_answer(ctx, d, k):
     assert sa["tuned"] >= 1 and sb["tuned"] == sa["tuned"]
     assert np.array_equal(a_idx, b_idx) and np.array_equal(a_dist, b_dist)
     assert_matches_oracle(R, Q, k, a_idx, a_dist)
+    # a later call of the shape replays the choice with its own per-call
+    # options (here squared distances out), not the first call's
+    c_idx, c_dist, sc = tctx.join(R, Q, k, autotune=True, out_squared=True)
+    assert sc["tuned"] == sa["tuned"]
+    assert np.array_equal(c_idx, a_idx)
+    np.testing.assert_allclose(np.sqrt(c_dist.astype(np.float64)), a_dist, rtol=1e-6)
     tctx.close()
 
 
