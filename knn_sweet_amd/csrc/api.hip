@@ -373,6 +373,7 @@ struct Inst {
   void *rc = nullptr;
   float *nr = nullptr, *reps32 = nullptr, *rn32 = nullptr, *rinv = nullptr;
   double *theta = nullptr;
+  double *dab = nullptr;  // [m_q][m_r] binary64 representative distances (bounds scratch)
   int32_t *plist = nullptr, *pcnt = nullptr;
   float *plb = nullptr;
   int32_t *lead = nullptr, *ucnt = nullptr, *uoff = nullptr, *uperm = nullptr;
@@ -586,6 +587,7 @@ knnj_status alloc_inst(Alloc &A, Inst &I, Tile *tiles) {
     TRY(A.get("dd_px", nq * I.dpad, &I.upx));
   }
   TRY(A.get("theta", (size_t)I.m_q, &I.theta));
+  TRY(A.get("dab", (size_t)I.m_q * I.m_r, &I.dab));
   TRY(A.get("plist", (size_t)I.m_q * I.m_r, &I.plist));
   TRY(A.get("plb", (size_t)I.m_q * I.m_r, &I.plb));
   TRY(A.get("pcnt", (size_t)I.m_q, &I.pcnt));
@@ -720,7 +722,7 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
     HIPCHK(hipGetLastError());
   }
   if (ev_index) HIPCHK(hipEventRecord(ev_index, s));
-  launch_bounds(I.Qi, I.Ri, I.d, I.k, I.theta, I.plist, I.plb, I.pcnt, s);
+  launch_bounds(I.Qi, I.Ri, I.d, I.k, I.theta, I.plist, I.plb, I.pcnt, I.dab, s);
   HIPCHK(hipGetLastError());
   if (ev_bounds) HIPCHK(hipEventRecord(ev_bounds, s));
   if (I.dedup) launch_dedup(I.Qi, I.d, I.lead, I.ucnt, I.uoff, I.upx, I.uperm, I.updist, s);

@@ -201,8 +201,62 @@ __device__ __forceinline__ u64 wave_min_u64(u64 v) {
   return v;
 }
 
+// Binary64 distances of every (query representative, reference
+// representative) pair, dab[a][b]: tiles of 64 x 64 pairs whose 16-dimension
+// slices are staged through LDS, 4 x 4 pairs per thread.  Each reference
+// representative is read from L2 once per 64 query clusters (k_bounds read
+// all of them once per query cluster: 2 MB per workgroup at m_R = 4,096,
+// d = 128).  (double)x - (double)y is exact; the squares are summed with
+// fused multiply-adds in ascending dimension order.
+#define RD_T 64
+__global__ __launch_bounds__(256) void k_rep_dist(const float *__restrict__ qr, int mq, int qdp,
+                                                  const float *__restrict__ rr, int mr, int rdp, int d,
+                                                  double *__restrict__ dab) {
+  __shared__ float sa[RD_T][17], sb[RD_T][17];  // 16 dimensions + 1 against bank conflicts
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int a0 = blockIdx.y * RD_T, b0 = blockIdx.x * RD_T;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  for (int k0 = 0; k0 < d; k0 += 16) {
+    for (int e = threadIdx.x; e < RD_T * 16; e += 256) {
+      const int r = e >> 4, c = e & 15, dim = k0 + c;
+      sa[r][c] = (a0 + r < mq && dim < d) ? qr[(int64_t)(a0 + r) * qdp + dim] : 0.f;
+      sb[r][c] = (b0 + r < mr && dim < d) ? rr[(int64_t)(b0 + r) * rdp + dim] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int c = 0; c < 16; ++c) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        av[i] = (double)sa[ty + 16 * i][c];
+        bv[i] = (double)sb[tx + 16 * i][c];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double t = av[i] - bv[j];
+          acc[i][j] = __builtin_fma(t, t, acc[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int a = a0 + ty + 16 * i, b = b0 + tx + 16 * j;
+      if (a < mq && b < mr) dab[(int64_t)a * mr + b] = sqrt(acc[i][j]);
+    }
+}
+
 __global__ __launch_bounds__(BD_THREADS, 2) void k_bounds(SideIndex Qi, SideIndex Ri, int d, int k, double *theta,
-                                                          int32_t *plist, float *plb, int32_t *pcnt) {
+                                                          int32_t *plist, float *plb, int32_t *pcnt,
+                                                          const double *__restrict__ dab_all) {
   __shared__ double dab[KNNJ_MAX_REPS];
   __shared__ u64 keys[KNNJ_MAX_REPS];  // cluster keys, then candidate bounds, then the surviving list
   __shared__ u64 wmin[BD_THREADS / 64];
@@ -219,41 +273,8 @@ __global__ __launch_bounds__(BD_THREADS, 2) void k_bounds(SideIndex Qi, SideInde
   }
   const double rA = Qi.rad[A];
 
-  // 1. d(a, b) in binary64: 4 lanes per representative, 16-byte loads, the
-  // lanes' partial sums added at the end (rows are dpad = roundup(d, 4)
-  // floats); two representatives per lane group in flight
-  {
-    const bd_f32x4 *ra4 = (const bd_f32x4 *)(Qi.reps + (int64_t)A * Qi.dpad);
-    const int c = tid & 3, dp4 = Qi.dpad >> 2;
-    constexpr int BS = BD_THREADS / 4;
-    for (int b0 = tid >> 2; b0 < mR; b0 += 2 * BS) {
-      const int b1 = b0 + BS;
-      const bool has1 = b1 < mR;
-      const bd_f32x4 *rb4 = (const bd_f32x4 *)(Ri.reps + (int64_t)b0 * Ri.dpad);
-      const bd_f32x4 *rc4 = (const bd_f32x4 *)(Ri.reps + (int64_t)(has1 ? b1 : b0) * Ri.dpad);
-      double acc0 = 0.0, acc1 = 0.0;
-      for (int j = c; j < dp4; j += 4) {
-        const bd_f32x4 x = ra4[j], y = rb4[j], z = rc4[j];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if (4 * j + e < d) {
-            const double t0 = (double)x[e] - (double)y[e];
-            const double t1 = (double)x[e] - (double)z[e];
-            acc0 = __builtin_fma(t0, t0, acc0);
-            acc1 = __builtin_fma(t1, t1, acc1);
-          }
-        }
-      }
-      acc0 += __shfl_xor(acc0, 1);
-      acc0 += __shfl_xor(acc0, 2);
-      acc1 += __shfl_xor(acc1, 1);
-      acc1 += __shfl_xor(acc1, 2);
-      if (c == 0) {
-        dab[b0] = sqrt(acc0);
-        if (has1) dab[b1] = sqrt(acc1);
-      }
-    }
-  }
+  // 1. d(a, b) in binary64 (row A of k_rep_dist's matrix)
+  for (int b = tid; b < mR; b += BD_THREADS) dab[b] = dab_all[(int64_t)A * mR + b];
   __syncthreads();
   // the clusters' keys r_A + d(a,b) + min_s d(s,b) (empty clusters: never taken)
   for (int b = tid; b < mR; b += BD_THREADS) {
@@ -349,7 +370,9 @@ __global__ __launch_bounds__(BD_THREADS, 2) void k_bounds(SideIndex Qi, SideInde
 }
 
 void launch_bounds(const SideIndex &Qi, const SideIndex &Ri, int d, int k, double *theta, int32_t *plist,
-                   float *plb, int32_t *pcnt, hipStream_t s) {
-  hipLaunchKernelGGL(k_bounds, dim3(Qi.m), dim3(BD_THREADS), 0, s, Qi, Ri, d, k, theta, plist, plb, pcnt);
+                   float *plb, int32_t *pcnt, double *dab, hipStream_t s) {
+  hipLaunchKernelGGL(k_rep_dist, dim3((unsigned)((Ri.m + RD_T - 1) / RD_T), (unsigned)((Qi.m + RD_T - 1) / RD_T)),
+                     dim3(256), 0, s, Qi.reps, Qi.m, Qi.dpad, Ri.reps, Ri.m, Ri.dpad, d, dab);
+  hipLaunchKernelGGL(k_bounds, dim3(Qi.m), dim3(BD_THREADS), 0, s, Qi, Ri, d, k, theta, plist, plb, pcnt, dab);
   hipLaunchKernelGGL(k_bounds_full, dim3(Qi.m), dim3(BD_THREADS), 0, s, Qi, Ri, d, k, theta, plist, plb, pcnt);
 }

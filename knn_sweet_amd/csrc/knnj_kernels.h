@@ -64,9 +64,10 @@ void launch_dedup_expand(int64_t n, const int32_t *perm, const int32_t *lead, in
 
 // k_bounds.hip
 // plist[A][j]: surviving reference clusters of query cluster A, ascending by
-// plb[A][j] = max(d(a,b) - r_b, 0) rounded down (a lower bound of d(a, s), s in b)
+// plb[A][j] = max(d(a,b) - r_b, 0) rounded down (a lower bound of d(a, s), s in b);
+// dab: [Qi.m][Ri.m] binary64 scratch for the representative distances
 void launch_bounds(const SideIndex &Qi, const SideIndex &Ri, int d, int k, double *theta, int32_t *plist,
-                   float *plb, int32_t *pcnt, hipStream_t s);
+                   float *plb, int32_t *pcnt, double *dab, hipStream_t s);
 
 // k_join.hip
 struct JoinArgs {
