@@ -74,13 +74,13 @@ __device__ __forceinline__ u32 wave_max_u32(u32 v) {
 // pool's distances <= t per bit: one ballot per register).  Every key of
 // distance > D* has at least k smaller keys, so it can never be among the k
 // nearest; the pool keeps the keys of distance <= D* and the pass threshold
-// becomes (D*, any row).  When the ties at D* would leave less than a chunk
-// of room (or exact_k), the (k - #{distance < D*})-th smallest row among the
-// ties is found the same way and exactly the k smallest keys stay, the
-// threshold then being just below the k-th key.  Returns the threshold
-// (a key passes when <= it) and sets cnt.  rowbits: width of the row ids.
+// becomes (D*, any row).  When those are more than keep_max keys (ties at
+// D*), the (k - #{distance < D*})-th smallest row among the ties is found the
+// same way and exactly the k smallest keys stay, the threshold then being
+// just below the k-th key.  Returns the threshold (a key passes when <= it)
+// and sets cnt.  rowbits: width of the row ids.
 template <int P>
-__device__ __forceinline__ u64 pool_shrink(u64 *pool, int &cnt, int k, int rowbits, bool exact_k) {
+__device__ __forceinline__ u64 pool_shrink(u64 *pool, int &cnt, int k, int rowbits, int keep_max) {
   constexpr int EP = P / 64;
   const int lane = threadIdx.x & 63;
   u64 v[EP];
@@ -101,7 +101,7 @@ __device__ __forceinline__ u64 pool_shrink(u64 *pool, int &cnt, int k, int rowbi
   }
   const int cle = wave_count_le(dv, r);
   u64 tau, thr;
-  if (!exact_k && cle <= P - 128) {
+  if (cle <= keep_max) {
     tau = ((u64)__float_as_uint((float)r) << 32) | 0xFFFFFFFFull;
     thr = tau;
   } else {
@@ -132,12 +132,14 @@ __device__ __forceinline__ u64 pool_shrink(u64 *pool, int &cnt, int k, int rowbi
 }
 
 // Exact mode: row qo of the answer from a query's pool (cnt keys): the k
-// smallest keys, sorted, are the answer (ties by original row).
+// smallest keys, sorted, are the answer (ties by original row).  The pool is
+// first cut to at most KCAP keys (the ties at the k-th distance kept when
+// they fit, so the row search runs only when they do not).
 template <int KCAP, int P>
 __device__ __forceinline__ void pool_emit(const JoinArgs &a, u64 *pool, int cnt, int64_t qo, int rowbits) {
   const int lane = threadIdx.x & 63;
   const int k = a.k;
-  if (cnt > KCAP) pool_shrink<P>(pool, cnt, k, rowbits, true);
+  if (cnt > KCAP) pool_shrink<P>(pool, cnt, k, rowbits, KCAP);
   constexpr int E = KCAP / 64;
   u64 w[E];
 #pragma unroll
@@ -210,7 +212,8 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile, co
   auto merge = [&](int i) {
     ++nmerge;
     if constexpr (EXACT) {
-      const u64 t = pool_shrink<P>(pools + i * P, cnt[i], k, rowbits, false);
+      // ties kept while a chunk of room remains above them
+      const u64 t = pool_shrink<P>(pools + i * P, cnt[i], k, rowbits, P - 128);
       if (t < thr[i]) thr[i] = t;
     } else {
       wave_merge<KCAP>(lists + i * KCAP, bufs + i * KNNJ_BCAP, cnt[i], tmp);
