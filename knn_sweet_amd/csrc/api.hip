@@ -625,8 +625,6 @@ void fill_side_index(Inst &I, bool ref_side) {
   out.px = b.px;
 }
 
-// One side's cluster index: sampled representatives, Lloyd refinement,
-// assignment, radii, and the cluster-sorted permuted copy.
 // The context's auxiliary streams (default priority) and their events,
 // created on first use.
 knnj_status ensure_aux(knnj_ctx *c) {
@@ -637,6 +635,8 @@ knnj_status ensure_aux(knnj_ctx *c) {
   return KNNJ_OK;
 }
 
+// One side's cluster index: sampled representatives, Lloyd refinement,
+// assignment, radii, and the cluster-sorted permuted copy.
 knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
   SideBufs &b = ref_side ? I.rb : I.qb;
   const float *X = ref_side ? I.dR : I.dQ;
@@ -1315,7 +1315,8 @@ knnj_status time_candidates(knnj_ctx *c, const std::vector<knnj_opts> &cand, Run
 }
 
 // The two-phase search: the first-phase list, then the winner's variants;
-// the chosen options are cached under sig for later calls of that shape.
+// the chosen candidate is cached under sig for later calls of that shape,
+// which rebuild it from their own options (cand, o) and run it.
 template <class Run>
 knnj_status autotune_call(knnj_ctx *c, const std::vector<int64_t> &sig, const std::vector<knnj_opts> &cand,
                           const knnj_opts &o, int kmax, bool mfma, Run run, knnj_stats *stats) {
