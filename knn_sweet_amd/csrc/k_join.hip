@@ -254,6 +254,8 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile, co
     const int bj = lv ? pl[pj] : pl[pb];
     const float rbj = (float)a.R.rad[bj] * (1.f + 2.4e-7f);
     const float *repj = a.R.reps + (int64_t)bj * dpR;
+    // member range of each listed cluster, loaded beside its representative
+    const int s0j = a.R.off[bj], s1j = a.R.off[bj + 1];
     float dql[TQ];
 #pragma unroll
     for (int i = 0; i < TQ; ++i) dql[i] = 0.f;
@@ -284,11 +286,23 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile, co
       }
     }
     u64 pending = wave_ballot(lv && needj);
+    // The member distances at the chunk starts of the next pending cluster
+    // (lane c: chunk c; lane 63 also chunk 64) are loaded one visit ahead, so
+    // that a visit's window test does not wait on loads of its own.
+    float wc = INFINITY, wn = INFINITY;
+    auto window_load = [&](const int jl) {
+      const int s0 = __builtin_amdgcn_readlane(s0j, jl), s1 = __builtin_amdgcn_readlane(s1j, jl);
+      const int nch = (s1 - s0 + 63) >> 6;
+      wc = lane < nch ? a.R.pdist[s0 + 64 * lane] : INFINITY;
+      wn = (lane == 63 && 64 < nch) ? a.R.pdist[s0 + 64 * 64] : INFINITY;
+    };
+    if (pending) window_load(__builtin_ctzll(pending));
     while (pending) {
     const int jl = __builtin_ctzll(pending);
     pending &= pending - 1;
-    const int b = __builtin_amdgcn_readlane(bj, jl);
-    const int s0 = a.R.off[b], s1 = a.R.off[b + 1];
+    const float wc0 = wc, wn0 = wn;
+    if (pending) window_load(__builtin_ctzll(pending));
+    const int s0 = __builtin_amdgcn_readlane(s0j, jl), s1 = __builtin_amdgcn_readlane(s1j, jl);
     const float rb = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, rbj), jl));
     bool use[TQ];
     bool any = false;
@@ -316,9 +330,16 @@ __device__ __forceinline__ void valu_tile(const JoinArgs &a, const Tile tile, co
     const int nch = (s1 - s0 + 63) >> 6;
     for (int cg = 0; cg < nch; cg += 64) {
       const int c = cg + lane;
-      const float cmin = c < nch ? a.R.pdist[s0 + 64 * c] : INFINITY;
-      float nmin = __shfl_down(cmin, 1);
-      if (lane == 63) nmin = (c + 1 < nch) ? a.R.pdist[s0 + 64 * (c + 1)] : INFINITY;
+      float cmin, nmin;
+      if (cg == 0) {
+        cmin = wc0;
+        nmin = __shfl_down(wc0, 1);
+        if (lane == 63) nmin = wn0;
+      } else {
+        cmin = c < nch ? a.R.pdist[s0 + 64 * c] : INFINITY;
+        nmin = __shfl_down(cmin, 1);
+        if (lane == 63) nmin = (c + 1 < nch) ? a.R.pdist[s0 + 64 * (c + 1)] : INFINITY;
+      }
       const bool need = (c < nch) && (cmin <= hi) && (nmin >= lo);
       const u64 m = wave_ballot(need);
       if (m == 0) {
@@ -490,15 +511,19 @@ static hipError_t launch_tq_kcap_ex(const JoinArgs &a, int64_t grid, hipStream_t
   const size_t lds = join_lds_bytes(TQ, KCAP, EX);
   const unsigned blocks = (unsigned)((grid + VALU_WPB - 1) / VALU_WPB);
   if (argv) {
-    if (lds > 65536)
-      hipFuncSetAttribute((const void *)k_join_valu_multi<TQ, KCAP, EX>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)lds);
+    if (lds > 65536) {
+      const hipError_t e = hipFuncSetAttribute((const void *)k_join_valu_multi<TQ, KCAP, EX>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL((k_join_valu_multi<TQ, KCAP, EX>), dim3(blocks), dim3(64 * VALU_WPB), lds, s, argv, tiles,
                        (int)grid);
   } else {
-    if (lds > 65536)
-      hipFuncSetAttribute((const void *)k_join_valu<TQ, KCAP, EX>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)lds);
+    if (lds > 65536) {
+      const hipError_t e = hipFuncSetAttribute((const void *)k_join_valu<TQ, KCAP, EX>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL((k_join_valu<TQ, KCAP, EX>), dim3(blocks), dim3(64 * VALU_WPB), lds, s, a);
   }
   return hipGetLastError();

@@ -968,13 +968,15 @@ hipError_t launch_mfma_dk(const JoinArgs &a, int64_t grid, hipStream_t s, const 
 #define KJ_LAUNCH_W(KC, EX, WV)                                                                                \
   do {                                                                                                          \
     if (argv) {                                                                                                 \
-      hipFuncSetAttribute((const void *)k_join_mfma_multi<KC, EX, PREC, DK, WV>,                                \
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                \
+      const hipError_t e_ = hipFuncSetAttribute((const void *)k_join_mfma_multi<KC, EX, PREC, DK, WV>,          \
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);          \
+      if (e_ != hipSuccess) return e_;                                                                          \
       hipLaunchKernelGGL((k_join_mfma_multi<KC, EX, PREC, DK, WV>), dim3((unsigned)grid), dim3(64 * WV), lds, s, \
                          argv, tiles, (int)grid);                                                               \
     } else {                                                                                                    \
-      hipFuncSetAttribute((const void *)k_join_mfma<KC, EX, PREC, DK, WV>,                                      \
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                                \
+      const hipError_t e_ = hipFuncSetAttribute((const void *)k_join_mfma<KC, EX, PREC, DK, WV>,                \
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);          \
+      if (e_ != hipSuccess) return e_;                                                                          \
       hipLaunchKernelGGL((k_join_mfma<KC, EX, PREC, DK, WV>), dim3((unsigned)grid), dim3(64 * WV), lds, s, a);   \
     }                                                                                                           \
   } while (0)
