@@ -695,7 +695,28 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
       bestd[i] = INFINITY;
       bestj[i] = -1;
     }
-    const int np_seed = a.seed_blocks > 0 ? min(np, 16 * a.seed_blocks) : np;
+    const int np_seed = a.qseed ? 0 : a.seed_blocks > 0 ? min(np, 16 * a.seed_blocks) : np;
+    if (a.qseed && np > 0 && nq > 0) {
+      // one block: lane c16's column is the nearest reference representative
+      // of row c16 (from the index build), and each row takes the nearest of
+      // the tile's 16 -- a representative outside the list is still a valid
+      // start (its members are real candidates and the bound below holds for
+      // any cluster)
+      ++nseedb;
+      const bool lv = c16 < nq;
+      const int bj = a.qseed[a.Q.perm[q0 + (lv ? c16 : 0)]];
+      Pre cur;
+      load_pre(bj, cur);
+      const f32x4_t acc = center_dots(cur);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float sqc = (qnr[i] + cur.cn) - 2.f * acc[i];
+        if (lv && sqc < bestd[i]) {
+          bestd[i] = sqc;
+          bestj[i] = bj;
+        }
+      }
+    }
     if (np_seed > 0) {
       int bjc = list_at(c16), bjn = list_at(16 + c16);
       Pre cur;

@@ -110,6 +110,10 @@ struct JoinArgs {
   int lockstep;          // multi-wave workgroup: 1 = waves walk the union of their cluster lists together
   int split;             // multi-wave workgroup: > 1 = its waves split one 16-query tile's cluster list
   int seed_blocks;       // MFMA join: the seed searches the first seed_blocks x 16 listed clusters (0: all)
+  // MFMA join, optional: [original query row] its nearest reference
+  // representative (the index build's assignment of the query rows); the
+  // seed then screens the tile's 16 of these instead of searching the list
+  const int32_t *qseed;
 };
 
 // Picks the wave tile size for a candidate capacity; returns the dynamic LDS bytes.

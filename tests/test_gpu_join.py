@@ -240,6 +240,28 @@ def test_autotune_gives_the_same_answer(ctx, d, k):
     tctx.close()
 
 
+@pytest.mark.parametrize("prec", [kj.PREC_BF16, kj.PREC_FP32])
+def test_seeds_by_assignment(ctx, prec, monkeypatch):
+    """With 512 or more reference representatives the MFMA join starts each
+    query row at its nearest representative from an assignment of the query
+    rows (in a self-join, the row's own cluster) instead of searching its
+    cluster list.  Both starts give the oracle's answer, bit for bit."""
+    mix = datagen.mixture64_model(seed=510, d=64, comps=40)
+    R = mix.sample(datagen.rng(511), 20000)
+    Q = mix.sample(datagen.rng(512), 1500)
+    a_idx, a_dist, sa = ctx.join(R, Q, 32, precision=prec, m_r=1024)
+    assert sa["m_r"] == 1024 and sa["kernel_used"] == kj.KERNEL_MFMA
+    monkeypatch.setenv("KNNJ_SEED_LIST", "1")
+    b_idx, b_dist, _ = ctx.join(R, Q, 32, precision=prec, m_r=1024)
+    monkeypatch.delenv("KNNJ_SEED_LIST")
+    assert np.array_equal(a_idx, b_idx) and np.array_equal(a_dist, b_dist)
+    assert_matches_oracle(R, Q, 32, a_idx, a_dist)
+    X = np.ascontiguousarray(R[:8000])
+    c_idx, c_dist, sc = ctx.join(X, X, 16, precision=prec, m_r=600)
+    assert sc["m_r"] == 600 and sc["kernel_used"] == kj.KERNEL_MFMA
+    assert_matches_oracle(X, X, 16, c_idx, c_dist)
+
+
 @pytest.mark.parametrize("d", [3, 64])
 def test_many_tiny_reference_clusters(ctx, d):
     """About 1.5 rows per reference cluster and k = 500: gathering k bound
