@@ -375,6 +375,8 @@ struct Inst {
   double *theta = nullptr;
   double *dab = nullptr;  // [m_q][m_r] binary64 representative distances (bounds scratch)
   int32_t *qseed = nullptr;  // [nQ] MFMA join: each query row's nearest reference representative
+  __bf16 *qs_rb = nullptr;   // its assignment's bf16 representatives (a resident index's reference side)
+  float *qs_rn = nullptr;
   int32_t *plist = nullptr, *pcnt = nullptr;
   float *plb = nullptr;
   int32_t *lead = nullptr, *ucnt = nullptr, *uoff = nullptr, *uperm = nullptr;
@@ -560,11 +562,10 @@ knnj_status alloc_side(Alloc &A, const std::string &p, int64_t n, int m, int dpa
 // list (measured on the d = 128 mixture, 2M x 200k, 4,096 representatives:
 // the list search was a seventh of a tile).  Small reference sides keep the
 // search, which is cheap there.  A self-join's rows already carry their
-// assignment (the reference side's).
-bool seed_by_assignment(const Inst &I) {
-  return I.mfma && !I.pre && I.m_r >= 512 && !std::getenv("KNNJ_SEED_LIST");
-}
-bool use_qseed(const Inst &I) { return seed_by_assignment(I) && !I.self; }
+// assignment (the reference side's).  A query of a resident index without
+// allocation keeps the search (its scratch array holds no assignment).
+bool seed_by_assignment(const Inst &I) { return I.mfma && I.m_r >= 512 && !std::getenv("KNNJ_SEED_LIST"); }
+bool use_qseed(const Inst &I, const Alloc &A) { return seed_by_assignment(I) && !I.self && !(I.pre && A.no_alloc); }
 
 // Every device buffer of one join after decide(); `tiles` given: the join's
 // slot range of a batch's shared tile array.
@@ -601,7 +602,14 @@ knnj_status alloc_inst(Alloc &A, Inst &I, Tile *tiles) {
   }
   TRY(A.get("theta", (size_t)I.m_q, &I.theta));
   TRY(A.get("dab", (size_t)I.m_q * I.m_r, &I.dab));
-  if (use_qseed(I)) TRY(A.get("qseed", (size_t)std::max<int64_t>(I.nQ, 1), &I.qseed));
+  if (use_qseed(I, A)) {
+    TRY(A.get("qseed", (size_t)std::max<int64_t>(I.nQ, 1), &I.qseed));
+    const int dk = (I.d + 31) & ~31;
+    if (I.pre && I.d >= 16 && dk <= 256) {  // the assignment's bf16 representatives
+      TRY(A.get("qs_rb", (size_t)I.m_r * dk, &I.qs_rb));
+      TRY(A.get("qs_rn", (size_t)I.m_r, &I.qs_rn));
+    }
+  }
   TRY(A.get("plist", (size_t)I.m_q * I.m_r, &I.plist));
   TRY(A.get("plb", (size_t)I.m_q * I.m_r, &I.plb));
   TRY(A.get("pcnt", (size_t)I.m_q, &I.pcnt));
@@ -724,7 +732,10 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
   }
   if (!I.pre) TRY(enqueue_side(c, I, true, s));
   if (I.qseed) {
-    launch_assign(I.dQ, I.nQ, I.ldQ, I.d, I.dpad, I.rb.reps, I.m_r, I.qseed, I.rb.rb, I.rb.rn, s);
+    if (I.pre)
+      launch_assign(I.dQ, I.nQ, I.ldQ, I.d, I.dpad, I.pre->Ri.reps, I.m_r, I.qseed, I.qs_rb, I.qs_rn, s);
+    else
+      launch_assign(I.dQ, I.nQ, I.ldQ, I.d, I.dpad, I.rb.reps, I.m_r, I.qseed, I.rb.rb, I.rb.rn, s);
     HIPCHK(hipGetLastError());
   }
   if (I.self)
@@ -769,7 +780,7 @@ void fill_args(Inst &I) {
   a.lockstep = std::getenv("KNNJ_NOLOCK") ? 0 : 1;
   a.seed_blocks = 0;
   if (const char *sb = std::getenv("KNNJ_SEED_BLOCKS")) a.seed_blocks = std::atoi(sb);
-  a.qseed = I.qseed ? I.qseed : (I.self && seed_by_assignment(I)) ? I.rb.cl : nullptr;
+  a.qseed = I.qseed ? I.qseed : (I.self && !I.pre && seed_by_assignment(I)) ? I.rb.cl : nullptr;
   a.exact = I.kernel_exact();
   const double delta = I.exact ? 0.0 : (double)(d + 4) * 0x1p-23 + 0x1p-40;
   a.delta = delta;

@@ -46,6 +46,27 @@ def test_index_query_equals_join(ctx, served):
     ix.close()
 
 
+def test_index_query_seeds_by_assignment(ctx, served, monkeypatch):
+    """An index of 512 or more representatives starts each query row of the
+    MFMA join at its nearest representative (an assignment of the query rows
+    per call) instead of searching the cluster list; both give the oracle's
+    answer, and the no-allocation entry point (which keeps the search) the
+    same."""
+    R, batches = served
+    ix = ctx.build_index(R, m_r=1024)
+    assert ix.info()["m_r"] == 1024
+    Q = np.concatenate(batches[-3:])
+    a_idx, a_dist, _ = ix.query(Q, 50)
+    monkeypatch.setenv("KNNJ_SEED_LIST", "1")
+    b_idx, b_dist, _ = ix.query(Q, 50)
+    monkeypatch.delenv("KNNJ_SEED_LIST")
+    assert np.array_equal(a_idx, b_idx) and np.array_equal(a_dist, b_dist)
+    c_idx, c_dist, _ = ix.query(Q[:700], 50, no_alloc=True)
+    assert np.array_equal(c_idx, a_idx[:700]) and np.array_equal(c_dist, a_dist[:700])
+    assert_matches_oracle(R, Q, 50, a_idx, a_dist)
+    ix.close()
+
+
 @pytest.mark.parametrize("d,k,prec", [(32, 16, kj.PREC_AUTO), (64, 10, kj.PREC_FP32), (96, 200, kj.PREC_BF16)])
 def test_index_precisions(ctx, d, k, prec):
     mix = datagen.mixture64_model(seed=d + 7, d=d, comps=30)
