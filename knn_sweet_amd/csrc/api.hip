@@ -673,8 +673,7 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
   launch_rep_select(X, ld, d, dpad, b.cand, mc, I.trials, b.partial, rep_score_blocks(mc), b.reps, b.rep_rows, s);
   for (int r = 0; r < rounds; ++r) {
     launch_assign(X, n, ld, d, dpad, b.reps, mc, b.cl, b.rb, b.rn, s);
-    launch_zero(b.radbits, sizeof(u64) * mc, s);
-    launch_zero(b.cnt, sizeof(int32_t) * mc, s);
+    launch_zeros(s, b.radbits, sizeof(u64) * mc, b.cnt, sizeof(int32_t) * mc);
     launch_assign_finish(X, n, ld, d, dpad, b.reps, b.cl, b.d2r, b.radbits, b.cnt, s);
     launch_split_reps(X, n, ld, d, dpad, mc, step, b.cl, b.d2r, b.gkeys, b.reps, s);
     mc += step;
@@ -695,16 +694,13 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
   const int64_t ns = (n + ls - 1) / ls;
   for (int it = 0; it < I.refine; ++it) {
     launch_assign(X, ns, ld * ls, d, dpad, b.reps, m, b.cl, b.rb, b.rn, s);
-    launch_zero(b.cnt, sizeof(int32_t) * m, s);
-    launch_zero(b.fill, sizeof(int32_t) * m, s);
+    launch_zeros(s, b.cnt, sizeof(int32_t) * m, b.fill, sizeof(int32_t) * m);
     launch_count(b.cl, ns, b.cnt, s);
     launch_bucket_rows(b.cl, ns, m, b.cnt, b.off, b.fill, b.perm, b.gkeys, s);
     launch_centroid(X, ld * ls, d, dpad, m, b.off, b.perm, I.integral ? 1 : 0, b.reps, s);
   }
   launch_assign(X, n, ld, d, dpad, b.reps, m, b.cl, b.rb, b.rn, s);
-  launch_zero(b.radbits, sizeof(u64) * m, s);
-  launch_zero(b.cnt, sizeof(int32_t) * m, s);
-  launch_zero(b.fill, sizeof(int32_t) * m, s);
+  launch_zeros(s, b.radbits, sizeof(u64) * m, b.cnt, sizeof(int32_t) * m, b.fill, sizeof(int32_t) * m);
   launch_assign_finish(X, n, ld, d, dpad, b.reps, b.cl, b.d2r, b.radbits, b.cnt, s);
   launch_bucket_sort_permute(X, n, ld, d, dpad, m, b.cl, b.cnt, b.off, b.fill, b.perm, b.d2r, b.pdist, b.gkeys,
                              b.px, s);
@@ -755,8 +751,7 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
   HIPCHK(hipGetLastError());
   if (ev_bounds) HIPCHK(hipEventRecord(ev_bounds, s));
   if (I.dedup) launch_dedup(I.Qi, I.d, I.lead, I.ucnt, I.uoff, I.upx, I.uperm, I.updist, s);
-  launch_zero(I.counters, 8 * sizeof(int32_t), s);
-  launch_zero(I.evals, 4 * sizeof(u64), s);
+  launch_zeros(s, I.counters, 8 * sizeof(int32_t), I.evals, 4 * sizeof(u64));
   launch_make_tiles(I.dedup ? I.uoff : I.Qi.off, I.m_q, I.theta, I.tq, I.id, I.tiles, I.counters,
                     (int)I.max_slots, I.o.xcd_tiles ? 8 : 1, s);
   HIPCHK(hipGetLastError());

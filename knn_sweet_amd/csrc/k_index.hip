@@ -30,6 +30,27 @@ void launch_zero(void *p, size_t bytes, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_zero32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (u32 *)p, n);
 }
 
+// Up to three buffers in one launch (an index build clears several small
+// per-cluster arrays at once; each launch costs the GPU a few microseconds
+// whatever its size, which many small joins pay many times).
+struct ZeroSet {
+  u32 *p[3];
+  int64_t n[3];
+};
+__global__ void k_zero32_set(ZeroSet z) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    if (i < z.n[j]) z.p[j][i] = 0u;
+}
+
+void launch_zeros(hipStream_t s, void *p0, size_t b0, void *p1, size_t b1, void *p2, size_t b2) {
+  ZeroSet z{{(u32 *)p0, (u32 *)p1, (u32 *)p2},
+            {p0 ? (int64_t)(b0 / 4) : 0, p1 ? (int64_t)(b1 / 4) : 0, p2 ? (int64_t)(b2 / 4) : 0}};
+  const int64_t n = std::max(z.n[0], std::max(z.n[1], z.n[2]));
+  if (n > 0) hipLaunchKernelGGL(k_zero32_set, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, z);
+}
+
 // Representative draws: element i is SplitMix64's output number i (the
 // state after i + 1 increments, mixed) modulo n -- the same sequence a host
 // loop of splitmix64() calls gives, computed independently per element.

@@ -7,6 +7,9 @@
 
 // k_index.hip
 void launch_zero(void *p, size_t bytes, hipStream_t s);  // bytes: a multiple of 4
+// zero up to three buffers (null pointers skipped) in one launch
+void launch_zeros(hipStream_t s, void *p0, size_t b0, void *p1 = nullptr, size_t b1 = 0, void *p2 = nullptr,
+                  size_t b2 = 0);
 // cand[i] = SplitMix64 output i (state seed, i + 1 increments) mod n
 void launch_draw(int32_t *cand, int count, uint64_t seed, int64_t n, hipStream_t s);
 void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags, hipStream_t s);
