@@ -384,6 +384,7 @@ struct Inst {
   Tile *tiles = nullptr, *ovf_tiles = nullptr, *ovf2_tiles = nullptr;
   int32_t *counters = nullptr;  // [0] slots, [1] overflow, [2] overflow 2, [3] max err bits, [4] overflow 3
   u64 *evals = nullptr;         // [0] evaluations, [1] merges, [2] appended
+  bool batch_counters = false;  // counters and evals are slices of a batch's arrays, cleared by the batch
   JoinArgs args{};
 
   // exact flag of the kernel that runs the tiles
@@ -619,8 +620,10 @@ knnj_status alloc_inst(Alloc &A, Inst &I, Tile *tiles) {
     TRY(A.get("tiles", (size_t)I.max_slots, &I.tiles));
   TRY(A.get("ovf_tiles", (size_t)I.nQ, &I.ovf_tiles));
   TRY(A.get("ovf2_tiles", (size_t)I.nQ, &I.ovf2_tiles));
-  TRY(A.get("counters", 8, &I.counters));
-  TRY(A.get("evals", 4, &I.evals));
+  if (!I.batch_counters) {
+    TRY(A.get("counters", 8, &I.counters));
+    TRY(A.get("evals", 4, &I.evals));
+  }
   const size_t nout = (size_t)I.nQ * I.k;
   if (I.out_host) {
     TRY(A.get("out_idx", nout * (I.aos ? 2 : 1), &I.out_idx));
@@ -751,7 +754,7 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
   HIPCHK(hipGetLastError());
   if (ev_bounds) HIPCHK(hipEventRecord(ev_bounds, s));
   if (I.dedup) launch_dedup(I.Qi, I.d, I.lead, I.ucnt, I.uoff, I.upx, I.uperm, I.updist, s);
-  launch_zeros(s, I.counters, 8 * sizeof(int32_t), I.evals, 4 * sizeof(u64));
+  if (!I.batch_counters) launch_zeros(s, I.counters, 8 * sizeof(int32_t), I.evals, 4 * sizeof(u64));
   launch_make_tiles(I.dedup ? I.uoff : I.Qi.off, I.m_q, I.theta, I.tq, I.id, I.tiles, I.counters,
                     (int)I.max_slots, I.o.xcd_tiles ? 8 : 1, s);
   HIPCHK(hipGetLastError());
@@ -1728,6 +1731,18 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
   }
   Tile *all_tiles;
   TRY(ws(c, "batch_tiles", (size_t)std::max<int64_t>(total_slots, 1), &all_tiles));
+  // every join's counters in two arrays: cleared in one launch and read back
+  // in one copy each (per-join launches and copies cost as much as the
+  // small joins' kernels)
+  int32_t *all_counters;
+  u64 *all_evals;
+  TRY(ws(c, "batch_counters", (size_t)8 * n_inst, &all_counters));
+  TRY(ws(c, "batch_evals", (size_t)4 * n_inst, &all_evals));
+  for (Inst &x : I) {
+    x.counters = all_counters + 8 * (size_t)x.id;
+    x.evals = all_evals + 4 * (size_t)x.id;
+    x.batch_counters = true;
+  }
   {
     // one arena for every join's buffers: a sizing pass, then the carve
     Alloc sz;
@@ -1772,6 +1787,7 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
   // are spread over the auxiliary streams and overlap on the device (most
   // are many small launches); the fused joins wait for all of them
   auto enqueue_all = [&](bool timed) -> knnj_status {
+    launch_zeros(s, all_counters, sizeof(int32_t) * 8 * (size_t)n_inst, all_evals, sizeof(u64) * 4 * (size_t)n_inst);
     HIPCHK(hipEventRecord(c->aux_ev[knnj_ctx::kAux], s));
     for (int j = 0; j < knnj_ctx::kAux; ++j) HIPCHK(hipStreamWaitEvent(c->aux[j], c->aux_ev[knnj_ctx::kAux], 0));
     {
@@ -1843,6 +1859,8 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
     }
     put(&all_tiles, sizeof(all_tiles));
     put(&dargs, sizeof(dargs));
+    put(&all_counters, sizeof(all_counters));
+    put(&all_evals, sizeof(all_evals));
     if (c->batch_exec && sig == c->batch_sig) {
       HIPCHK(hipGraphLaunch(c->batch_exec, s));
       graphed = true;
@@ -1879,14 +1897,11 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
     TRY(enqueue_all(true));
   }
   HIPCHK(hipEventRecord(ev[4], s));
-  // counters of every join in one copy: the counter and evaluation blocks
-  // sit in the arena, gathered here per join
+  // counters of every join: one copy per array
   std::vector<int32_t> hc((size_t)8 * n_inst, 0);
   std::vector<u64> hcnt((size_t)4 * n_inst, 0);
-  for (int id : order) {
-    HIPCHK(hipMemcpyAsync(&hc[8 * id], I[id].counters, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&hcnt[4 * id], I[id].evals, 4 * sizeof(u64), hipMemcpyDeviceToHost, s));
-  }
+  HIPCHK(hipMemcpyAsync(hc.data(), all_counters, hc.size() * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(hcnt.data(), all_evals, hcnt.size() * sizeof(u64), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
 
   HIPCHK(hipEventRecord(ev[5], s));
