@@ -1697,9 +1697,22 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
   u32 *flags;
   TRY(ws(c, "batch_flags", (size_t)4 * n_inst, &flags));
   HIPCHK(hipMemsetAsync(flags, 0, sizeof(u32) * 4 * n_inst, s));
-  for (const Inst &x : I) {
-    launch_scan_input(x.dR, x.nR, x.d, x.ldR, flags + 4 * x.id, s);
-    if (!x.self && x.nQ > 0) launch_scan_input(x.dQ, x.nQ, x.d, x.ldQ, flags + 4 * x.id, s);
+  // every input scanned in one launch (the descriptors stay alive until the
+  // synchronization below)
+  std::vector<ScanDesc> sd;
+  {
+    int64_t mv = 0;
+    for (const Inst &x : I) {
+      if (x.nR > 0) sd.push_back(ScanDesc{x.dR, x.nR, x.ldR, x.d, 0, flags + 4 * x.id});
+      if (!x.self && x.nQ > 0) sd.push_back(ScanDesc{x.dQ, x.nQ, x.ldQ, x.d, 0, flags + 4 * x.id});
+      mv = std::max(mv, std::max(x.nR, x.self ? 0 : x.nQ) * (int64_t)x.d);
+    }
+    if (!sd.empty()) {
+      ScanDesc *dsd;
+      TRY(ws(c, "batch_scan", sd.size(), &dsd));
+      HIPCHK(hipMemcpyAsync(dsd, sd.data(), sd.size() * sizeof(ScanDesc), hipMemcpyHostToDevice, s));
+      launch_scan_inputs(dsd, (int)sd.size(), mv, s);
+    }
   }
   std::vector<u32> hflags((size_t)4 * n_inst);
   HIPCHK(hipMemcpyAsync(hflags.data(), flags, sizeof(u32) * hflags.size(), hipMemcpyDeviceToHost, s));

@@ -153,6 +153,71 @@ void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags,
   hipLaunchKernelGGL(k_scan_input, dim3((unsigned)blocks), dim3(256), 0, s, X, n, d, ld, flags);
 }
 
+// Many inputs in one launch (a batch of joins): workgroup row y scans input
+// y, its workgroups striding over that input's values as above.
+__global__ __launch_bounds__(256) void k_scan_inputs(const ScanDesc *__restrict__ desc) {
+  const ScanDesc D = desc[blockIdx.y];
+  const int64_t total = D.n * (int64_t)D.d;
+  const bool packed = D.ld == D.d && (total & 3) == 0 && ((uintptr_t)D.X & 15) == 0;
+  const int64_t items = packed ? total / 4 : total;
+  if ((int64_t)blockIdx.x * blockDim.x >= items) return;  // workgroup-uniform
+  u32 bad = 0, nonint = 0;
+  float mx = 0.f;
+  auto take = [&](const float x) {
+    if (!isfinite(x)) {
+      bad = 1;
+    } else {
+      mx = fmaxf(mx, fabsf(x));
+      if (x != rintf(x)) nonint = 1;
+    }
+  };
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if (packed) {
+    const float4 *X4 = (const float4 *)D.X;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < items; e += stride) {
+      const float4 v = X4[e];
+      take(v.x);
+      take(v.y);
+      take(v.z);
+      take(v.w);
+    }
+  } else {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < items; e += stride) {
+      const int64_t r = e / D.d;
+      take(D.X[r * D.ld + (e - r * D.d)]);
+    }
+  }
+  __shared__ u32 sh[3][4];
+  bad = __ballot(bad) != 0;
+  nonint = __ballot(nonint) != 0;
+  mx = wave_max(mx);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][w] = bad;
+    sh[1][w] = __float_as_uint(mx);
+    sh[2][w] = nonint;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 b = 0, x = 0, ni = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+      b |= sh[0][i];
+      x = max(x, sh[1][i]);
+      ni |= sh[2][i];
+    }
+    if (b) atomicOr(&D.flags[0], 1u);
+    atomicMax(&D.flags[1], x);
+    if (ni) atomicOr(&D.flags[2], 1u);
+  }
+}
+
+void launch_scan_inputs(const ScanDesc *desc, int nd, int64_t max_values, hipStream_t s) {
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((max_values + 255) / 256, 256));
+  for (int i = 0; i < nd; i += 65535)  // the grid's y extent
+    hipLaunchKernelGGL(k_scan_inputs, dim3((unsigned)blocks, (unsigned)std::min(65535, nd - i)), dim3(256), 0, s,
+                       desc + i);
+}
+
 // ---------------------------------------------------------------------------
 // Representative sampling score: for trial t, the sum over unordered pairs
 // i < j of the FP32 distance between candidate rows (the reference sums

@@ -13,6 +13,15 @@ void launch_zeros(hipStream_t s, void *p0, size_t b0, void *p1 = nullptr, size_t
 // cand[i] = SplitMix64 output i (state seed, i + 1 increments) mod n
 void launch_draw(int32_t *cand, int count, uint64_t seed, int64_t n, hipStream_t s);
 void launch_scan_input(const float *X, int64_t n, int d, int64_t ld, u32 *flags, hipStream_t s);
+// the same scan of nd inputs in one launch (desc: device array; max_values:
+// the largest n d among them)
+struct ScanDesc {
+  const float *X;
+  int64_t n, ld;
+  int d, pad;
+  u32 *flags;
+};
+void launch_scan_inputs(const ScanDesc *desc, int nd, int64_t max_values, hipStream_t s);
 int rep_score_blocks(int m);  // partial sums per trial of launch_rep_select
 // partial: trials * nblk partial sums + one slot for the winning trial
 void launch_rep_select(const float *X, int64_t ld, int d, int dpad, const int32_t *cand, int m, int trials,
