@@ -375,7 +375,7 @@ struct Inst {
   double *theta = nullptr;
   double *dab = nullptr;  // [m_q][m_r] binary64 representative distances (bounds scratch)
   int32_t *qseed = nullptr;  // [nQ] MFMA join: each query row's nearest reference representative
-  __bf16 *qs_rb = nullptr;   // its assignment's bf16 representatives (a resident index's reference side)
+  __bf16 *qs_rb = nullptr;   // the query rows' assignment's own bf16 representatives
   float *qs_rn = nullptr;
   int32_t *plist = nullptr, *pcnt = nullptr;
   float *plb = nullptr;
@@ -606,7 +606,7 @@ knnj_status alloc_inst(Alloc &A, Inst &I, Tile *tiles) {
   if (use_qseed(I, A)) {
     TRY(A.get("qseed", (size_t)std::max<int64_t>(I.nQ, 1), &I.qseed));
     const int dk = (I.d + 31) & ~31;
-    if (I.pre && I.d >= 16 && dk <= 256) {  // the assignment's bf16 representatives
+    if (I.d >= 16 && dk <= 256) {  // the assignment's own bf16 representatives
       TRY(A.get("qs_rb", (size_t)I.m_r * dk, &I.qs_rb));
       TRY(A.get("qs_rn", (size_t)I.m_r, &I.qs_rn));
     }
@@ -662,7 +662,7 @@ knnj_status ensure_aux(knnj_ctx *c) {
 
 // One side's cluster index: sampled representatives, Lloyd refinement,
 // assignment, radii, and the cluster-sorted permuted copy.
-knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
+knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s, hipEvent_t reps_final = nullptr) {
   SideBufs &b = ref_side ? I.rb : I.qb;
   const float *X = ref_side ? I.dR : I.dQ;
   const int64_t n = ref_side ? I.nR : I.nQ, ld = ref_side ? I.ldR : I.ldQ;
@@ -702,6 +702,7 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s) {
     launch_bucket_rows(b.cl, ns, m, b.cnt, b.off, b.fill, b.perm, b.gkeys, s);
     launch_centroid(X, ld * ls, d, dpad, m, b.off, b.perm, I.integral ? 1 : 0, b.reps, s);
   }
+  if (reps_final) HIPCHK(hipEventRecord(reps_final, s));  // the representatives change no more
   launch_assign(X, n, ld, d, dpad, b.reps, m, b.cl, b.rb, b.rn, s);
   launch_zeros(s, b.radbits, sizeof(u64) * m, b.cnt, sizeof(int32_t) * m, b.fill, sizeof(int32_t) * m);
   launch_assign_finish(X, n, ld, d, dpad, b.reps, b.cl, b.d2r, b.radbits, b.cnt, s);
@@ -729,12 +730,20 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
     HIPCHK(hipStreamWaitEvent(fork, c->aux_ev[knnj_ctx::kAux], 0));
     TRY(enqueue_side(c, I, false, fork));
   }
-  if (!I.pre) TRY(enqueue_side(c, I, true, s));
+  // with the query side on the auxiliary stream, the query rows'
+  // assignment runs there too, once the reference representatives are final
+  // (beside the reference side's own assignment, into buffers of its own)
+  const bool seed_fork = two && I.qseed && I.qs_rb;
+  if (!I.pre) TRY(enqueue_side(c, I, true, s, seed_fork ? c->aux_ev[1] : nullptr));
   if (I.qseed) {
-    if (I.pre)
+    if (I.pre) {
       launch_assign(I.dQ, I.nQ, I.ldQ, I.d, I.dpad, I.pre->Ri.reps, I.m_r, I.qseed, I.qs_rb, I.qs_rn, s);
-    else
+    } else if (seed_fork) {
+      HIPCHK(hipStreamWaitEvent(fork, c->aux_ev[1], 0));
+      launch_assign(I.dQ, I.nQ, I.ldQ, I.d, I.dpad, I.rb.reps, I.m_r, I.qseed, I.qs_rb, I.qs_rn, fork);
+    } else {
       launch_assign(I.dQ, I.nQ, I.ldQ, I.d, I.dpad, I.rb.reps, I.m_r, I.qseed, I.rb.rb, I.rb.rn, s);
+    }
     HIPCHK(hipGetLastError());
   }
   if (I.self)
