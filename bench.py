@@ -346,7 +346,7 @@ def bench_batch(args, kj, cfg):
     bopts = dict(refine_iters=args.refine_iters or -1, split_rounds=args.split_rounds)
     # the batched call autotunes itself on the warm-up call (Lloyd step, VALU
     # tile, k-select capacity, MFMA tile shape); the loop keeps the defaults
-    gopts = dict(bopts, graph=args.graph, autotune=not args.no_autotune)
+    gopts = dict(bopts, graph=args.graph, autotune=not args.no_autotune, work_queue=args.work_queue)
     outs, _ = ctx.join_batch(dev, **gopts)  # warm-up; with --graph it captures the plan
     for _ in range(max(0, args.warmup - 1)):
         ctx.join_batch(dev, outputs=outs, **gopts)
@@ -374,7 +374,8 @@ def bench_batch(args, kj, cfg):
         "p50_ms": float(np.percentile(times, 50)), "p99_ms": float(np.percentile(times, 99)),
         "stages_ms": {"index_bounds_tiles": st["t_index_ms"], "join": t_join, "fallback": st["t_fallback_ms"],
                       "total": st["t_total_ms"]},
-        "fused_launches": st["n_launches"], "graph": {0: "eager", 1: "captured", 2: "replayed"}[st["graph"]],
+        "fused_launches": st["n_launches"], "persistent_launches": st["queue_launches"],
+        "graph": {0: "eager", 1: "captured", 2: "replayed"}[st["graph"]],
         "autotune_choice": st["tuned"],
         "pruning_ratio": st["pruning_ratio"],
         "dist_evals": st["n_dist_evals"], "n_overflow": st["n_overflow"],
@@ -501,6 +502,9 @@ def main():
     ap.add_argument("--split-rounds", type=int, default=0, help="representative split rounds (0 default, -1 none)")
     ap.add_argument("--refine-iters", type=int, default=0, help="Lloyd steps (0 default, -1 none)")
     ap.add_argument("--tile-q", type=int, default=0, help="VALU join: queries per wave tile (0 auto)")
+    ap.add_argument("--work-queue", action="store_true",
+                    help="batched workload: persistent launches fed by per-XCD device work queues "
+                         "(default: the autotuner chooses)")
     ap.add_argument("--graph", action="store_true",
                     help="batched workload: capture the plan once and replay it as a HIP graph")
     ap.add_argument("--no-autotune", action="store_true",

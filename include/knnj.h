@@ -122,7 +122,12 @@ typedef struct {
   int32_t filter_stats;/* 1: measure the filter's error against binary64 on every re-scored
                           candidate (knnj_stats.filter_err_max / filter_bound_max /
                           filter_err_ratio; a diagnostic: it re-scores each list once more) */
-  int32_t reserved[10];
+  int32_t work_queue;  /* knnj_join_batch: 1 = each fused launch of single-wave workgroups
+                          runs persistent -- as many workgroups as the device holds take tile
+                          slots from eight per-XCD device counters until none are left --
+                          instead of one workgroup per slot (0); the batch autotuner tries it
+                          when this is 0 */
+  int32_t reserved[9];
 } knnj_opts;
 
 /* Device-side timings (hipEvents on the context's stream) and counters. */
@@ -154,7 +159,7 @@ typedef struct {
   int32_t graph;            /* knnj_join_batch: 0 eager, 1 captured this call, 2 replayed */
   int32_t rerun_stages;     /* overflow re-run stages executed (0: none; each stage doubles the
                                capacity on the MFMA kernel, the last one is the VALU kernel at 1024) */
-  int32_t reserved0;
+  int32_t queue_launches;   /* knnj_join_batch: fused launches run in the persistent work-queue form */
   double filter_err_max;    /* with opts.filter_stats: the filter value's own error, largest
                                |s_f - s64| over re-scored candidates, s_f the filter's squared
                                distance and s64 the binary64 one (0 in exact mode) */

@@ -51,7 +51,8 @@ class Opts(ctypes.Structure):
                 ("tile_split", ctypes.c_int32), ("split_rounds", ctypes.c_int32),
                 ("autotune", ctypes.c_int32), ("graph", ctypes.c_int32), ("kcap", ctypes.c_int32),
                 ("kcap_mfma_max", ctypes.c_int32), ("dedup", ctypes.c_int32),
-                ("filter_stats", ctypes.c_int32), ("reserved", ctypes.c_int32 * 10)]
+                ("filter_stats", ctypes.c_int32), ("work_queue", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 9)]
 
 
 class Stats(ctypes.Structure):
@@ -65,7 +66,7 @@ class Stats(ctypes.Structure):
                 ("filter_max_abs_err", ctypes.c_double), ("kernel_used", ctypes.c_int32),
                 ("n_launches", ctypes.c_int32), ("n_merges", ctypes.c_int64),
                 ("n_appended", ctypes.c_int64), ("tuned", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("rerun_stages", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("rerun_stages", ctypes.c_int32), ("queue_launches", ctypes.c_int32),
                 ("filter_err_max", ctypes.c_double), ("filter_bound_max", ctypes.c_double),
                 ("filter_err_ratio", ctypes.c_double), ("n_distinct_queries", ctypes.c_int64)]
 
@@ -230,10 +231,10 @@ def _outputs(nq: int, k: int, aos: bool, qdev):
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
               brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0, xcd_tiles=False,
               tile_waves=0, tile_split=0, split_rounds=0, autotune=False, graph=False, kcap=0,
-              kcap_mfma_max=0, dedup=0, filter_stats=False) -> Opts:
+              kcap_mfma_max=0, dedup=0, filter_stats=False, work_queue=False) -> Opts:
     return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
                 tile_q, kernel, refine_iters, int(xcd_tiles), tile_waves, tile_split, split_rounds,
-                int(autotune), int(graph), kcap, kcap_mfma_max, dedup, int(filter_stats))
+                int(autotune), int(graph), kcap, kcap_mfma_max, dedup, int(filter_stats), int(work_queue))
 
 
 class Context:

@@ -1760,6 +1760,9 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
   u64 *all_evals;
   TRY(ws(c, "batch_counters", (size_t)8 * n_inst, &all_counters));
   TRY(ws(c, "batch_evals", (size_t)4 * n_inst, &all_evals));
+  // eight tile-slot counters per fused launch (the persistent launches' work queues)
+  int32_t *queues;
+  TRY(ws(c, "batch_queues", (size_t)8 * (n_inst + 1), &queues));
   for (Inst &x : I) {
     x.counters = all_counters + 8 * (size_t)x.id;
     x.evals = all_evals + 4 * (size_t)x.id;
@@ -1805,11 +1808,21 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
   TRY(ensure_aux(c));
 
   int n_launch = 0;
+  const bool use_queue = o.work_queue != 0;
+  // fused launches run persistent (multi-wave MFMA workgroups keep one workgroup per slot)
+  int n_queued = 0;
+  if (use_queue)
+    for (size_t g = 0; g < order.size(); ++g)
+      if (g == 0 || !(I[order[g]].launch_key() == I[order[g - 1]].launch_key())) {
+        const Inst &P = I[order[g]];
+        if (!P.use_mfma || join_mfma_tile_q(P.args.kcap, P.args.dk, P.args.waves) == 16) ++n_queued;
+      }
   // index builds, bounds and tile lists: independent per instance, so they
   // are spread over the auxiliary streams and overlap on the device (most
   // are many small launches); the fused joins wait for all of them
   auto enqueue_all = [&](bool timed) -> knnj_status {
-    launch_zeros(s, all_counters, sizeof(int32_t) * 8 * (size_t)n_inst, all_evals, sizeof(u64) * 4 * (size_t)n_inst);
+    launch_zeros(s, all_counters, sizeof(int32_t) * 8 * (size_t)n_inst, all_evals, sizeof(u64) * 4 * (size_t)n_inst,
+                 queues, sizeof(int32_t) * 8 * ((size_t)n_inst + 1));
     HIPCHK(hipEventRecord(c->aux_ev[knnj_ctx::kAux], s));
     for (int j = 0; j < knnj_ctx::kAux; ++j) HIPCHK(hipStreamWaitEvent(c->aux[j], c->aux_ev[knnj_ctx::kAux], 0));
     {
@@ -1841,15 +1854,18 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
       const int64_t last = slot0[order[h - 1]] + I[order[h - 1]].max_slots;
       const int nslots = (int)(last - first);
       const Tile *t0 = all_tiles + first;
+      // the persistent form takes the launch's slots from its own eight
+      // counters (multi-wave MFMA workgroups keep one workgroup per slot)
+      int32_t *q = use_queue ? queues + 8 * n_launch : nullptr;
       if (P.use_mfma) {
         if (P.fp8)
-          HIPCHK(launch_join_mfma_fp8_multi(P.args, dargs, t0, nslots, s));
+          HIPCHK(launch_join_mfma_fp8_multi(P.args, dargs, t0, nslots, s, q));
         else if (P.bf16)
-          HIPCHK(launch_join_mfma_bf16_multi(P.args, P.bf16x1 ? 1 : 3, dargs, t0, nslots, s));
+          HIPCHK(launch_join_mfma_bf16_multi(P.args, P.bf16x1 ? 1 : 3, dargs, t0, nslots, s, q));
         else
-          HIPCHK(launch_join_mfma_f32_multi(P.args, dargs, t0, nslots, s));
+          HIPCHK(launch_join_mfma_f32_multi(P.args, dargs, t0, nslots, s, q));
       } else {
-        HIPCHK(launch_join_multi(P.args, P.tq, dargs, t0, nslots, s));
+        HIPCHK(launch_join_multi(P.args, P.tq, dargs, t0, nslots, s, q));
       }
       ++n_launch;
       g = h;
@@ -1883,6 +1899,8 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
     put(&dargs, sizeof(dargs));
     put(&all_counters, sizeof(all_counters));
     put(&all_evals, sizeof(all_evals));
+    put(&queues, sizeof(queues));
+    put(&use_queue, sizeof(use_queue));
     if (c->batch_exec && sig == c->batch_sig) {
       HIPCHK(hipGraphLaunch(c->batch_exec, s));
       graphed = true;
@@ -1973,6 +1991,7 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
   st.filter_max_abs_err = ferr;
   if (!order.empty()) plan_stats(I[order[0]], st);
   st.n_launches = n_groups;
+  st.queue_launches = n_queued;
   st.graph = graph_state;
   (void)n_launch;
   if (stats) *stats = st;
@@ -2010,7 +2029,7 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   // instance may run on the vector ALUs, their 2 or 4 queries per tile;
   // second phase: the doubled k-select capacity and the 4-wave lockstep tile.
   std::vector<int64_t> sig = {-1, n_inst, o.precision, o.kernel, o.m_r, o.m_q, o.refine_iters, o.tile_q,
-                              o.tile_waves, o.kcap, o.brute_force, o.graph, o.split_rounds};
+                              o.tile_waves, o.kcap, o.brute_force, o.graph, o.split_rounds, o.work_queue};
   int kmax = 1;
   bool any_valu = false, any_mfma = false;
   for (int i = 0; i < n_inst; ++i) {
@@ -2036,6 +2055,11 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
       x.tile_q = tqs[t];
       cand.push_back(x);
     }
+  if (!o.work_queue) {  // and the first candidate's persistent form
+    knnj_opts x = cand[0];
+    x.work_queue = 1;
+    cand.push_back(x);
+  }
   return autotune_call(
       c, sig, cand, o, kmax, any_mfma,
       [&](const knnj_opts &x, knnj_stats *st) { return join_batch_once(c, n_inst, inst, x, st); }, stats);

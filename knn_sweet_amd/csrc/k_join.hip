@@ -31,6 +31,9 @@
 // squared distance, original row).  A list whose last key is still within
 // that window may have dropped a needed candidate: the query is reported as
 // overflowed and re-run at a larger capacity by the host.
+#include <cstdio>
+#include <cstdlib>
+
 #include "knnj_common.h"
 #include "knnj_kernels.h"
 #include "wave_sort.h"
@@ -493,6 +496,44 @@ __global__ __launch_bounds__(64 * VALU_WPB) void k_join_valu_multi(const JoinArg
   valu_tile<TQ, KCAP, EXACT>(argv[tile.pad], tile, slot);
 }
 
+// The persistent form of the many-joins launch: a grid of as many
+// single-wave workgroups as the device holds at once, each taking tile slots
+// from the eight device counters queue[0..8) (zero at launch; see
+// queue_next) until none are left.  Every wave leaves the loop once the
+// counters have passed nslots, and no wave waits on another (one wave per
+// workgroup: the tile's barriers are its own).
+template <int TQ, int KCAP, bool EXACT>
+__global__ __launch_bounds__(64 * VALU_WPB) VALU_WAVES_PER_EU(TQ, KCAP) void k_join_valu_queue(
+    const JoinArgs *__restrict__ argv, const Tile *__restrict__ tiles, int nslots, int32_t *queue) {
+  static_assert(VALU_WPB == 1, "queue slots are taken per wave");
+  const int home = queue_home();
+  int turn = 0;
+  for (;;) {
+    const int slot = queue_next(queue, nslots, home, turn);
+    if (slot < 0) break;
+    const Tile tile = tiles[slot];
+    if (tile.count > 0) valu_tile<TQ, KCAP, EXACT>(argv[tile.pad], tile, slot);
+  }
+}
+
+// workgroups of a persistent launch: twice what the occupancy query says
+// the device holds at once (at least two per compute unit), at most one per
+// slot.  Workgroups beyond what fits start as others end and find the queue
+// drained, so asking for more costs nothing, while asking for fewer would
+// leave the device part idle.  KNNJ_QUEUE_DEBUG=1 prints the figures.
+int64_t queue_grid(const void *kernel, int block, size_t lds, int64_t nslots) {
+  int per_cu = 0, dev = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess) per_cu = 1;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    ncu = 256;
+  const int64_t g = std::min<int64_t>(nslots, 2 * (int64_t)std::max(per_cu, 1) * std::max(ncu, 1));
+  if (std::getenv("KNNJ_QUEUE_DEBUG"))
+    fprintf(stderr, "[knnj queue] slots=%lld per_cu=%d cus=%d lds=%zu grid=%lld\n", (long long)nslots, per_cu, ncu,
+            lds, (long long)g);
+  return g;
+}
+
 int join_tile_q(int kcap) {
   // measured on MI355X: 4 queries per wave at capacity 64 (d = 16, k = 16),
   // 2 at 256 (d = 3, k = 200: 4.9 ms vs 6.7 ms at 4)
@@ -507,10 +548,19 @@ static size_t join_lds_bytes(int tq, int kcap, bool exact) {
 
 template <int TQ, int KCAP, bool EX>
 static hipError_t launch_tq_kcap_ex(const JoinArgs &a, int64_t grid, hipStream_t s, const JoinArgs *argv,
-                                    const Tile *tiles) {
+                                    const Tile *tiles, int32_t *queue) {
   const size_t lds = join_lds_bytes(TQ, KCAP, EX);
   const unsigned blocks = (unsigned)((grid + VALU_WPB - 1) / VALU_WPB);
-  if (argv) {
+  if (argv && queue) {
+    const void *kq = (const void *)k_join_valu_queue<TQ, KCAP, EX>;
+    if (lds > 65536) {
+      const hipError_t e = hipFuncSetAttribute(kq, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
+    const int64_t g = queue_grid(kq, 64 * VALU_WPB, lds, grid);
+    hipLaunchKernelGGL((k_join_valu_queue<TQ, KCAP, EX>), dim3((unsigned)g), dim3(64 * VALU_WPB), lds, s, argv, tiles,
+                       (int)grid, queue);
+  } else if (argv) {
     if (lds > 65536) {
       const hipError_t e = hipFuncSetAttribute((const void *)k_join_valu_multi<TQ, KCAP, EX>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -531,48 +581,48 @@ static hipError_t launch_tq_kcap_ex(const JoinArgs &a, int64_t grid, hipStream_t
 
 template <int TQ, int KCAP>
 static hipError_t launch_tq_kcap(const JoinArgs &a, int64_t grid, hipStream_t s, const JoinArgs *argv,
-                                 const Tile *tiles) {
-  return a.exact ? launch_tq_kcap_ex<TQ, KCAP, true>(a, grid, s, argv, tiles)
-                 : launch_tq_kcap_ex<TQ, KCAP, false>(a, grid, s, argv, tiles);
+                                 const Tile *tiles, int32_t *queue) {
+  return a.exact ? launch_tq_kcap_ex<TQ, KCAP, true>(a, grid, s, argv, tiles, queue)
+                 : launch_tq_kcap_ex<TQ, KCAP, false>(a, grid, s, argv, tiles, queue);
 }
 
 template <int KCAP>
 static hipError_t launch_kcap(const JoinArgs &a, int tq, int64_t grid, hipStream_t s, const JoinArgs *argv,
-                              const Tile *tiles) {
+                              const Tile *tiles, int32_t *queue) {
   switch (tq) {
     case 8:
-      if constexpr (KCAP <= 256) return launch_tq_kcap<8, KCAP>(a, grid, s, argv, tiles);
+      if constexpr (KCAP <= 256) return launch_tq_kcap<8, KCAP>(a, grid, s, argv, tiles, queue);
       break;
     case 4:
-      if constexpr (KCAP <= 256) return launch_tq_kcap<4, KCAP>(a, grid, s, argv, tiles);
+      if constexpr (KCAP <= 256) return launch_tq_kcap<4, KCAP>(a, grid, s, argv, tiles, queue);
       break;
     case 2:
-      if constexpr (KCAP <= 512) return launch_tq_kcap<2, KCAP>(a, grid, s, argv, tiles);
+      if constexpr (KCAP <= 512) return launch_tq_kcap<2, KCAP>(a, grid, s, argv, tiles, queue);
       break;
     case 1:
-      return launch_tq_kcap<1, KCAP>(a, grid, s, argv, tiles);
+      return launch_tq_kcap<1, KCAP>(a, grid, s, argv, tiles, queue);
   }
   return hipErrorInvalidValue;
 }
 
 static hipError_t launch_join_any(const JoinArgs &a, int tq, int64_t grid, hipStream_t s, const JoinArgs *argv,
-                                  const Tile *tiles) {
+                                  const Tile *tiles, int32_t *queue) {
   if (grid <= 0) return hipSuccess;
   switch (a.kcap) {
-    case 64: return launch_kcap<64>(a, tq, grid, s, argv, tiles);
-    case 128: return launch_kcap<128>(a, tq, grid, s, argv, tiles);
-    case 256: return launch_kcap<256>(a, tq, grid, s, argv, tiles);
-    case 512: return launch_kcap<512>(a, tq, grid, s, argv, tiles);
-    case 1024: return launch_kcap<1024>(a, tq, grid, s, argv, tiles);
+    case 64: return launch_kcap<64>(a, tq, grid, s, argv, tiles, queue);
+    case 128: return launch_kcap<128>(a, tq, grid, s, argv, tiles, queue);
+    case 256: return launch_kcap<256>(a, tq, grid, s, argv, tiles, queue);
+    case 512: return launch_kcap<512>(a, tq, grid, s, argv, tiles, queue);
+    case 1024: return launch_kcap<1024>(a, tq, grid, s, argv, tiles, queue);
   }
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_join(const JoinArgs &a, int tq, int64_t max_tiles, hipStream_t s) {
-  return launch_join_any(a, tq, max_tiles, s, nullptr, nullptr);
+  return launch_join_any(a, tq, max_tiles, s, nullptr, nullptr, nullptr);
 }
 
 hipError_t launch_join_multi(const JoinArgs &proto, int tq, const JoinArgs *argv, const Tile *tiles, int nslots,
-                             hipStream_t s) {
-  return launch_join_any(proto, tq, nslots, s, argv, tiles);
+                             hipStream_t s, int32_t *queue) {
+  return launch_join_any(proto, tq, nslots, s, argv, tiles, queue);
 }

@@ -970,6 +970,23 @@ __global__ __launch_bounds__(64 * W) void k_join_mfma_multi(const JoinArgs *__re
   mfma_tile<KCAP, EXACT, PREC, DK, W>(argv[tile.pad], tile);
 }
 
+// Persistent form of k_join_mfma_multi for single-wave workgroups (see
+// k_join_valu_queue): tile slots from the eight device counters queue[0..8).  Multi-wave
+// workgroups are not run this way: their waves meet at barriers inside a
+// tile, and a wave that ends a tile early must not go on to the next slot's.
+template <int KCAP, bool EXACT, int PREC, int DK>
+__global__ __launch_bounds__(64) void k_join_mfma_queue(const JoinArgs *__restrict__ argv,
+                                                        const Tile *__restrict__ tiles, int nslots, int32_t *queue) {
+  const int home = queue_home();
+  int turn = 0;
+  for (;;) {
+    const int slot = queue_next(queue, nslots, home, turn);
+    if (slot < 0) break;
+    const Tile tile = tiles[slot];
+    if (tile.count > 0) mfma_tile<KCAP, EXACT, PREC, DK, 1>(argv[tile.pad], tile);
+  }
+}
+
 // waves per workgroup actually launched: 1, or the most that fit when the
 // arguments ask for more than one
 inline int mfma_launch_waves(int kcap, int dkmax, int want) { return want > 1 ? mfma_waves(kcap, dkmax) : 1; }
@@ -983,12 +1000,19 @@ inline size_t mfma_lds(int kcap, int dkmax, int waves) {
 // translation unit (k_join_mfma_d<DK>.hip); k_join_mfma.hip dispatches.
 template <int PREC, int DK>
 hipError_t launch_mfma_dk(const JoinArgs &a, int64_t grid, hipStream_t s, const JoinArgs *argv,
-                          const Tile *tiles) {
+                          const Tile *tiles, int32_t *queue) {
   const int waves = mfma_launch_waves(a.kcap, DK, a.waves);
   const size_t lds = mfma_lds(a.kcap, DK, waves);
 #define KJ_LAUNCH_W(KC, EX, WV)                                                                                \
   do {                                                                                                          \
-    if (argv) {                                                                                                 \
+    if (argv && queue && WV == 1) {                                                                             \
+      const void *kq_ = (const void *)k_join_mfma_queue<KC, EX, PREC, DK>;                                      \
+      const hipError_t e_ = hipFuncSetAttribute(kq_, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);     \
+      if (e_ != hipSuccess) return e_;                                                                          \
+      const int64_t g_ = queue_grid(kq_, 64, lds, grid);                                                        \
+      hipLaunchKernelGGL((k_join_mfma_queue<KC, EX, PREC, DK>), dim3((unsigned)g_), dim3(64), lds, s, argv,     \
+                         tiles, (int)grid, queue);                                                              \
+    } else if (argv) {                                                                                          \
       const hipError_t e_ = hipFuncSetAttribute((const void *)k_join_mfma_multi<KC, EX, PREC, DK, WV>,          \
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);          \
       if (e_ != hipSuccess) return e_;                                                                          \
@@ -1033,7 +1057,9 @@ hipError_t launch_mfma_dk(const JoinArgs &a, int64_t grid, hipStream_t s, const 
 
 // operand widths with a compiled kernel (dk = mfma_dk(d))
 #define KJ_FOR_WIDTHS(X) X(32) X(64) X(96) X(128) X(256)
-#define KJ_DK_INST(P, DKV) \
-  template hipError_t launch_mfma_dk<P, DKV>(const JoinArgs &, int64_t, hipStream_t, const JoinArgs *, const Tile *);
-#define KJ_DK_EXTERN(P, DKV) \
-  extern template hipError_t launch_mfma_dk<P, DKV>(const JoinArgs &, int64_t, hipStream_t, const JoinArgs *, const Tile *);
+#define KJ_DK_INST(P, DKV)                                                                                 \
+  template hipError_t launch_mfma_dk<P, DKV>(const JoinArgs &, int64_t, hipStream_t, const JoinArgs *, const Tile *, \
+                                             int32_t *);
+#define KJ_DK_EXTERN(P, DKV)                                                                                      \
+  extern template hipError_t launch_mfma_dk<P, DKV>(const JoinArgs &, int64_t, hipStream_t, const JoinArgs *,     \
+                                                    const Tile *, int32_t *);

@@ -17,18 +17,18 @@ namespace {
 
 template <int PREC>
 hipError_t launch_mfma_any(const JoinArgs &a, int64_t grid, hipStream_t s, const JoinArgs *argv = nullptr,
-                           const Tile *tiles = nullptr) {
+                           const Tile *tiles = nullptr, int32_t *queue = nullptr) {
   if constexpr (PREC == 8) {
     switch (a.dk) {
-      case 64: return launch_mfma_dk<8, 64>(a, grid, s, argv, tiles);
-      case 96: return launch_mfma_dk<8, 96>(a, grid, s, argv, tiles);
-      case 128: return launch_mfma_dk<8, 128>(a, grid, s, argv, tiles);
+      case 64: return launch_mfma_dk<8, 64>(a, grid, s, argv, tiles, queue);
+      case 96: return launch_mfma_dk<8, 96>(a, grid, s, argv, tiles, queue);
+      case 128: return launch_mfma_dk<8, 128>(a, grid, s, argv, tiles, queue);
       default: return hipErrorInvalidValue;
     }
   }
   switch (a.dk) {
 #define KJ_CASE(DKV) \
-  case DKV: return launch_mfma_dk<PREC, DKV>(a, grid, s, argv, tiles);
+  case DKV: return launch_mfma_dk<PREC, DKV>(a, grid, s, argv, tiles, queue);
     KJ_FOR_WIDTHS(KJ_CASE)
 #undef KJ_CASE
     default: return hipErrorInvalidValue;
@@ -47,9 +47,9 @@ hipError_t launch_join_mfma_f32(const JoinArgs &a, int64_t max_tiles, hipStream_
 }
 
 hipError_t launch_join_mfma_f32_multi(const JoinArgs &proto, const JoinArgs *argv, const Tile *tiles, int nslots,
-                                      hipStream_t s) {
+                                      hipStream_t s, int32_t *queue) {
   if (nslots <= 0) return hipSuccess;
-  return launch_mfma_any<0>(proto, nslots, s, argv, tiles);
+  return launch_mfma_any<0>(proto, nslots, s, argv, tiles, queue);
 }
 
 hipError_t launch_join_mfma_fp8(const JoinArgs &a, int64_t max_tiles, hipStream_t s) {
@@ -58,9 +58,9 @@ hipError_t launch_join_mfma_fp8(const JoinArgs &a, int64_t max_tiles, hipStream_
 }
 
 hipError_t launch_join_mfma_fp8_multi(const JoinArgs &proto, const JoinArgs *argv, const Tile *tiles, int nslots,
-                                      hipStream_t s) {
+                                      hipStream_t s, int32_t *queue) {
   if (nslots <= 0) return hipSuccess;
-  return launch_mfma_any<8>(proto, nslots, s, argv, tiles);
+  return launch_mfma_any<8>(proto, nslots, s, argv, tiles, queue);
 }
 
 hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles, hipStream_t s) {
@@ -69,8 +69,8 @@ hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles
 }
 
 hipError_t launch_join_mfma_bf16_multi(const JoinArgs &proto, int terms, const JoinArgs *argv, const Tile *tiles,
-                                       int nslots, hipStream_t s) {
+                                       int nslots, hipStream_t s, int32_t *queue) {
   if (nslots <= 0) return hipSuccess;
-  return terms == 3 ? launch_mfma_any<3>(proto, nslots, s, argv, tiles)
-                    : launch_mfma_any<1>(proto, nslots, s, argv, tiles);
+  return terms == 3 ? launch_mfma_any<3>(proto, nslots, s, argv, tiles, queue)
+                    : launch_mfma_any<1>(proto, nslots, s, argv, tiles, queue);
 }

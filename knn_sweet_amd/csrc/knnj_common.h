@@ -94,6 +94,54 @@ __device__ __forceinline__ u64 shfl_xor_u64(u64 v, int m) {
   return ((u64)hi << 32) | lo;
 }
 
+// v of lane (lane ^ m), for a mask m known at compile time: DPP moves within
+// rows of 16 lanes (quad permutes: ^1, ^2, ^3; half-row mirror: ^7; row
+// mirror: ^15; two moves: ^4 = ^7^3, ^8 = ^15^7), the swizzle's xor mode
+// within 32 lanes (^16, ^31), a permute through LDS otherwise.  The DPP and
+// swizzle forms need no LDS round trip.
+__device__ __forceinline__ u32 xor_lanes(u32 v, int m) {
+  const int x = (int)v;
+  switch (m) {
+    case 1: return (u32)__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+    case 2: return (u32)__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+    case 3: return (u32)__builtin_amdgcn_mov_dpp(x, 0x1B, 0xF, 0xF, true);  // quad_perm [3,2,1,0]
+    case 4: return (u32)__builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, true), 0x1B, 0xF, 0xF,
+                                                 true);
+    case 7: return (u32)__builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, true);  // row_half_mirror
+    case 8: return (u32)__builtin_amdgcn_mov_dpp(__builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, true), 0x141, 0xF, 0xF,
+                                                 true);
+    case 15: return (u32)__builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, true);  // row_mirror
+    case 16: return (u32)__builtin_amdgcn_ds_swizzle(x, 0x1F | (16 << 10));
+    case 31: return (u32)__builtin_amdgcn_ds_swizzle(x, 0x1F | (31 << 10));
+    default: return (u32)__shfl_xor(x, m);
+  }
+}
+__device__ __forceinline__ u64 xor_lanes_u64(u64 v, int m) {
+  return ((u64)xor_lanes((u32)(v >> 32), m) << 32) | xor_lanes((u32)v, m);
+}
+
+// Persistent launches: eight tile-slot counters q[0..8), one per XCD; slot s
+// belongs to counter s % 8, the XCD the hardware's round-robin would give
+// workgroup s.  A wave takes from its own XCD's counter (home), then from
+// the others in turn (turn), and gets -1 once every counter has passed
+// nslots.  Counters only grow, so every wave reaches -1.
+__device__ __forceinline__ int queue_home() {
+  // HW_REG_XCC_ID (register 20), bits [3:0]
+  return (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7u);
+}
+__device__ __forceinline__ int queue_next(int32_t *q, int nslots, int home, int &turn) {
+  while (turn < 8) {
+    const int x = (home + turn) & 7;
+    int t = 0;
+    if ((threadIdx.x & 63) == 0) t = atomicAdd(q + x, 1);
+    t = __shfl(t, 0);
+    const int64_t s = 8 * (int64_t)t + x;
+    if (s < nslots) return (int)s;
+    ++turn;
+  }
+  return -1;
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);

@@ -146,7 +146,7 @@ hipError_t launch_join_mfma_bf16(const JoinArgs &a, int terms, int64_t max_tiles
 // fp8 (e4m3) filter, rows scaled per row; operand widths 64, 96, 128
 hipError_t launch_join_mfma_fp8(const JoinArgs &a, int64_t max_tiles, hipStream_t s);
 hipError_t launch_join_mfma_fp8_multi(const JoinArgs &proto, const JoinArgs *argv, const Tile *tiles, int nslots,
-                                      hipStream_t s);
+                                      hipStream_t s, int32_t *queue = nullptr);
 // queries per workgroup tile of the MFMA join (16 per wave; want_waves > 1
 // asks for the multi-wave workgroup)
 int join_mfma_tile_q(int kcap, int dk, int want_waves);
@@ -159,9 +159,16 @@ inline int mfma_dk(int d) { return d <= 32 ? 32 : d <= 64 ? 64 : d <= 96 ? 96 : 
 // Fused launches over the tiles of many instances: slot b runs tile tiles[b]
 // of instance tiles[b].pad with arguments argv[tiles[b].pad]; proto carries
 // the template selectors (kcap, exact, dk) shared by all those instances.
+// queue (eight device ints, zero at launch): the persistent form -- as many
+// single-wave workgroups as the device holds take slots from the per-XCD
+// counters until none are left (multi-wave MFMA workgroups keep one
+// workgroup per slot).
 hipError_t launch_join_multi(const JoinArgs &proto, int tq, const JoinArgs *argv, const Tile *tiles, int nslots,
-                             hipStream_t s);
+                             hipStream_t s, int32_t *queue = nullptr);
 hipError_t launch_join_mfma_f32_multi(const JoinArgs &proto, const JoinArgs *argv, const Tile *tiles, int nslots,
-                                      hipStream_t s);
+                                      hipStream_t s, int32_t *queue = nullptr);
 hipError_t launch_join_mfma_bf16_multi(const JoinArgs &proto, int terms, const JoinArgs *argv, const Tile *tiles,
-                                       int nslots, hipStream_t s);
+                                       int nslots, hipStream_t s, int32_t *queue = nullptr);
+// workgroups of a persistent launch of `kernel`: what the device holds at
+// once (at least one per compute unit), at most nslots
+int64_t queue_grid(const void *kernel, int block, size_t lds, int64_t nslots);

@@ -23,7 +23,7 @@ __device__ __forceinline__ void wave_sort_u64(u64 (&v)[E]) {
 #pragma unroll
       for (int u = 0; u < E; ++u) {
         u64 other = v[u ^ um];
-        if (lm) other = shfl_xor_u64(other, lm);
+        if (lm) other = xor_lanes_u64(other, lm);
         const int e = lane + 64 * u;
         const bool lower = e < (e ^ mask);
         const u64 mn = v[u] < other ? v[u] : other;
@@ -53,8 +53,8 @@ __device__ __forceinline__ void wave_sort_pair(u64 (&hi)[E], u32 (&lo)[E]) {
         u64 oh = hi[u ^ um];
         u32 ol = lo[u ^ um];
         if (lm) {
-          oh = shfl_xor_u64(oh, lm);
-          ol = __shfl_xor(ol, lm);
+          oh = xor_lanes_u64(oh, lm);
+          ol = xor_lanes(ol, lm);
         }
         const int e = lane + 64 * u;
         const bool lower = e < (e ^ mask);

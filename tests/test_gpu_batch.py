@@ -152,3 +152,57 @@ def test_batch_autotune_picks_and_reuses_an_exact_plan(ctx):
         _assert_same(b[0], p[0], f"replayed choice, instance {i} idx")
     for (R, Q), (idx, dist) in zip(inst, again):
         assert_matches_oracle(R, Q, 16, _host(idx), _host(dist))
+
+
+@pytest.mark.parametrize("tile_waves", [0, 4])
+def test_batch_work_queue_matches_one_workgroup_per_slot(ctx, tile_waves):
+    """The persistent launches (opts.work_queue: as many single-wave
+    workgroups as the device holds, taking tile slots from eight per-XCD
+    device counters) give the answer of the launches with one workgroup per
+    slot, bit for bit, and the oracle's: uneven real instances of three
+    widths, an integer self-join, twice in a row (the counters start from
+    zero each call), and with multi-wave MFMA tiles (tile_waves=4), which
+    keep one workgroup per slot."""
+    inst = datagen.batch_instances(n_inst=9, seed=120 + tile_waves, r_lo=400, r_hi=15000)
+    X = datagen.skin_like(4000, seed=121)
+    dev = [(_dev(R), _dev(Q), 16) for R, Q in inst] + [(_dev(X), _dev(X), 50)]
+    opts = {"tile_waves": tile_waves} if tile_waves else {}
+    a1, sa = ctx.join_batch(dev, work_queue=True, **opts)
+    a2, _ = ctx.join_batch(dev, work_queue=True, **opts)
+    b, sb = ctx.join_batch(dev, **opts)
+    assert sa["n_launches"] == sb["n_launches"]
+    assert sb["queue_launches"] == 0
+    assert 1 <= sa["queue_launches"] <= sa["n_launches"]
+    if tile_waves:
+        assert sa["queue_launches"] < sa["n_launches"]  # the multi-wave MFMA launches keep their slots
+    for i, ((x1, d1), (x2, d2), (y, e)) in enumerate(zip(a1, a2, b)):
+        _assert_same(x1, y, f"instance {i} idx")
+        _assert_same(d1, e, f"instance {i} dist")
+        _assert_same(x2, y, f"instance {i} idx, second call")
+    for (R, Q), (idx, dist) in zip(inst + [(X, X)], a1):
+        k = 50 if R is X else 16
+        assert_matches_oracle(R, Q, k, _host(idx), _host(dist))
+
+
+def test_batch_work_queue_graph_replays(ctx):
+    """The persistent form captured as a HIP graph: its counters are cleared
+    by a fill kernel inside the graph, so every replay (with eager work and
+    synchronisations between them) equals the eager batch."""
+    import torch
+    inst = datagen.batch_instances(n_inst=7, seed=130, r_lo=500, r_hi=9000)
+    dev = [(_dev(R), _dev(Q), 16) for R, Q in inst]
+    eager, _ = ctx.join_batch(dev)
+    outs, st1 = ctx.join_batch(dev, graph=True, work_queue=True)
+    assert st1["graph"] == 1 and st1["queue_launches"] >= 1
+    X = _dev(inst[0][0])
+    for rep in range(3):
+        for o in outs:
+            o[0].fill_(-1)
+            o[1].fill_(-1.0)
+        ctx.join(X, X, 5)
+        torch.cuda.synchronize()
+        _, st = ctx.join_batch(dev, outputs=outs, graph=True, work_queue=True)
+        assert st["graph"] == 2, rep
+        for i, ((a_i, a_d), (e_i, e_d)) in enumerate(zip(outs, eager)):
+            _assert_same(a_i, e_i, f"replay {rep} instance {i} idx")
+            _assert_same(a_d, e_d, f"replay {rep} instance {i} dist")
