@@ -67,7 +67,7 @@ __device__ __forceinline__ int wave_count_le(const u32 (&dv)[EP], u32 t) {
 
 __device__ __forceinline__ u32 wave_max_u32(u32 v) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = max(v, (u32)__shfl_xor((int)v, m));
+  for (int m = 32; m >= 1; m >>= 1) v = max(v, xor_lanes(v, m));
   return v;
 }
 

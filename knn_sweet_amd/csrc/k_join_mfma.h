@@ -240,8 +240,8 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) qn = __builtin_fmaf(v[j], v[j], qn);
   }
-  qn += __shfl_xor(qn, 16);
-  qn += __shfl_xor(qn, 32);
+  qn += xor_lanes_f(qn, 16);
+  qn += xor_lanes_f(qn, 32);
   if (g == 0) fsh[c16] = qn;
   wave_sync();
   float qnr[4];
@@ -370,8 +370,8 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
           need |= lv && live[i] && (seedmode || cslot < 0) && !(dlo * (1.f - eps) - rbj > thi);
           dqs[(4 * g + i) * 16 + c16] = dlo;
         }
-        need |= __shfl_xor((int)need, 16) != 0;
-        need |= __shfl_xor((int)need, 32) != 0;
+        need |= xor_lanes_i((int)need, 16) != 0;
+        need |= xor_lanes_i((int)need, 32) != 0;
         wave_sync();
         // clusters any wave of the workgroup needs, visited in one order
         const u32 own = (u32)wave_ballot(need) & 0xFFFFu;
@@ -434,8 +434,8 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
               }
             }
           }
-          part += __shfl_xor(part, 16);
-          part += __shfl_xor(part, 32);
+          part += xor_lanes_f(part, 16);
+          part += xor_lanes_f(part, 32);
           if (g == 0) fsh[c16] = part;
           wave_sync();
           float nqr[4];
@@ -451,8 +451,8 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
             for (int s2 = 0; s2 < NS; ++s2)
   #pragma unroll
               for (int j = 0; j < 8; ++j) mx = fmaxf(mx, fabsf(av[s2][j]));
-            mx = fmaxf(mx, __shfl_xor(mx, 16));
-            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            mx = fmaxf(mx, xor_lanes_f(mx, 16));
+            mx = fmaxf(mx, xor_lanes_f(mx, 32));
             const int e = mx > 0.f ? 6 - ilogbf(mx) : 0;
   #pragma unroll
             for (int s2 = 0; s2 < NS; ++s2) {
@@ -745,9 +745,10 @@ __device__ __forceinline__ void mfma_tile(const JoinArgs &a, const Tile tile) {
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
+#pragma unroll
       for (int off = 1; off < 16; off <<= 1) {
-        const float od = __shfl_xor(bestd[i], off);
-        const int oj = __shfl_xor(bestj[i], off);
+        const float od = xor_lanes_f(bestd[i], off);
+        const int oj = xor_lanes_i(bestj[i], off);
         if (od < bestd[i] || (od == bestd[i] && oj < bestj[i])) {
           bestd[i] = od;
           bestj[i] = oj;

@@ -119,6 +119,10 @@ __device__ __forceinline__ u32 xor_lanes(u32 v, int m) {
 __device__ __forceinline__ u64 xor_lanes_u64(u64 v, int m) {
   return ((u64)xor_lanes((u32)(v >> 32), m) << 32) | xor_lanes((u32)v, m);
 }
+__device__ __forceinline__ float xor_lanes_f(float v, int m) {
+  return __builtin_bit_cast(float, xor_lanes(__builtin_bit_cast(u32, v), m));
+}
+__device__ __forceinline__ int xor_lanes_i(int v, int m) { return (int)xor_lanes((u32)v, m); }
 
 // Persistent launches: eight tile-slot counters q[0..8), one per XCD; slot s
 // belongs to counter s % 8, the XCD the hardware's round-robin would give
@@ -142,27 +146,30 @@ __device__ __forceinline__ int queue_next(int32_t *q, int nslots, int home, int 
   return -1;
 }
 
+// butterfly reductions (the same lane pairs as __shfl_xor; the exchanges
+// within 32 lanes as DPP moves and swizzles)
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  for (int m = 32; m >= 1; m >>= 1) v += xor_lanes_f(v, m);
   return v;
 }
 
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  for (int m = 32; m >= 1; m >>= 1)
+    v += __builtin_bit_cast(double, xor_lanes_u64(__builtin_bit_cast(u64, v), m));
   return v;
 }
 
 __device__ __forceinline__ float wave_min(float v) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fminf(v, __shfl_xor(v, m));
+  for (int m = 32; m >= 1; m >>= 1) v = fminf(v, xor_lanes_f(v, m));
   return v;
 }
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m));
+  for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, xor_lanes_f(v, m));
   return v;
 }
 
