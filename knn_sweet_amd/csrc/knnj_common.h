@@ -97,8 +97,9 @@ __device__ __forceinline__ u64 shfl_xor_u64(u64 v, int m) {
 // v of lane (lane ^ m), for a mask m known at compile time: DPP moves within
 // rows of 16 lanes (quad permutes: ^1, ^2, ^3; half-row mirror: ^7; row
 // mirror: ^15; two moves: ^4 = ^7^3, ^8 = ^15^7), the swizzle's xor mode
-// within 32 lanes (^16, ^31), a permute through LDS otherwise.  The DPP and
-// swizzle forms need no LDS round trip.
+// within 32 lanes (^16, ^31), the swap of the two 32-lane halves
+// (v_permlane32_swap: ^32; after a ^31 swizzle: ^63), a permute through LDS
+// otherwise.  Only the permute makes an LDS round trip.
 __device__ __forceinline__ u32 xor_lanes(u32 v, int m) {
   const int x = (int)v;
   switch (m) {
@@ -113,6 +114,18 @@ __device__ __forceinline__ u32 xor_lanes(u32 v, int m) {
     case 15: return (u32)__builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, true);  // row_mirror
     case 16: return (u32)__builtin_amdgcn_ds_swizzle(x, 0x1F | (16 << 10));
     case 31: return (u32)__builtin_amdgcn_ds_swizzle(x, 0x1F | (31 << 10));
+    case 32: {
+      // lanes 32..63 of the first operand trade places with lanes 0..31 of
+      // the second: the second then holds v[lane + 32] below lane 32, the
+      // first v[lane - 32] above it
+      const auto r = __builtin_amdgcn_permlane32_swap((u32)x, (u32)x, false, false);
+      return lane_id() < 32 ? (u32)r[1] : (u32)r[0];
+    }
+    case 63: {
+      const u32 y = (u32)__builtin_amdgcn_ds_swizzle(x, 0x1F | (31 << 10));
+      const auto r = __builtin_amdgcn_permlane32_swap(y, y, false, false);
+      return lane_id() < 32 ? (u32)r[1] : (u32)r[0];
+    }
     default: return (u32)__shfl_xor(x, m);
   }
 }
