@@ -67,7 +67,7 @@ struct knnj_ctx {
   hipEvent_t ev[10] = {};
   // knnj_join_batch: streams the instances' index builds are spread over
   // (default priority, created on first use) and their completion events
-  static constexpr int kAux = 16;  // created on first use; batches spread over aux_streams() of them
+  static constexpr int kAux = 32;  // created on first use; batches spread over aux_streams() of them
   hipStream_t aux[kAux] = {};
   hipEvent_t aux_ev[kAux + 1] = {};
   // autotune: problem shape -> the chosen candidate: its index in the first
@@ -650,28 +650,34 @@ void fill_side_index(Inst &I, bool ref_side) {
   out.px = b.px;
 }
 
-// The context's auxiliary streams (default priority) and their events,
-// created on first use.
-// Auxiliary streams a batch spreads its index builds over: 16, or
-// KNNJ_AUX_STREAMS (1..16).  The phase is some 40 short dependent launches
+// Auxiliary streams a batch spreads its index builds over: 32, or
+// KNNJ_AUX_STREAMS (1..32).  The phase is some 40 short dependent launches
 // per instance, bound by the wait between dependent launches on a stream,
 // so more streams overlap more of them -- also beyond the 4 hardware queues
-// HIP gives a process by default (measured at 512 joins: index phase 102 ms
-// on 4 streams, 71 ms on 16; 76 ms on 16 streams with 16 hardware queues).
+// HIP gives a process by default (measured at 512 joins: index phase 103 ms
+// on 4 streams, 71 ms on 16, 67 ms on 24, 64 ms on 32; 76 ms on 16 streams
+// with 16 hardware queues).
 int aux_streams() {
   static const int n = [] {
     const char *e = std::getenv("KNNJ_AUX_STREAMS");
-    const int v = e ? std::atoi(e) : 16;
+    const int v = e ? std::atoi(e) : 32;
     return std::min(std::max(v, 1), (int)knnj_ctx::kAux);
   }();
   return n;
 }
 
-knnj_status ensure_aux(knnj_ctx *c) {
-  if (!c->aux[0]) {
-    for (auto &x : c->aux) HIPCHK(hipStreamCreate(&x));
-    for (auto &e : c->aux_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  }
+// The context's first n auxiliary streams (default priority) and their
+// events, created on first use (a single join needs two: the query side's
+// stream and the event of the reference representatives, aux_ev[1]; a batch
+// aux_streams()), and the fork event aux_ev[kAux].
+knnj_status ensure_aux(knnj_ctx *c, int n) {
+  n = std::min(n, (int)knnj_ctx::kAux);
+  if (!c->aux_ev[knnj_ctx::kAux]) HIPCHK(hipEventCreateWithFlags(&c->aux_ev[knnj_ctx::kAux], hipEventDisableTiming));
+  for (int j = 0; j < n; ++j)
+    if (!c->aux[j]) {
+      HIPCHK(hipStreamCreate(&c->aux[j]));
+      HIPCHK(hipEventCreateWithFlags(&c->aux_ev[j], hipEventDisableTiming));
+    }
   return KNNJ_OK;
 }
 
@@ -1052,7 +1058,7 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
   // side's (the allocation-free path creates nothing and stays on one stream)
   hipStream_t fork = nullptr;
   if (!A.no_alloc && !I.pre && !I.self) {
-    TRY(ensure_aux(c));
+    TRY(ensure_aux(c, 2));
     fork = c->aux[0];
   }
   TRY(enqueue_index(c, I, s, ev[2], ev[3], fork));
@@ -1820,7 +1826,8 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
     hargs[id] = I[id].args;
   }
   HIPCHK(hipMemcpyAsync(dargs, hargs.data(), sizeof(JoinArgs) * n_inst, hipMemcpyHostToDevice, s));
-  TRY(ensure_aux(c));
+  const int naux = aux_streams();
+  TRY(ensure_aux(c, naux));
 
   int n_launch = 0;
   const bool use_queue = o.work_queue != 0;
@@ -1839,10 +1846,9 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
     launch_zeros(s, all_counters, sizeof(int32_t) * 8 * (size_t)n_inst, all_evals, sizeof(u64) * 4 * (size_t)n_inst,
                  queues, sizeof(int32_t) * 8 * ((size_t)n_inst + 1));
     HIPCHK(hipEventRecord(c->aux_ev[knnj_ctx::kAux], s));
-    for (int j = 0; j < knnj_ctx::kAux; ++j) HIPCHK(hipStreamWaitEvent(c->aux[j], c->aux_ev[knnj_ctx::kAux], 0));
+    for (int j = 0; j < naux; ++j) HIPCHK(hipStreamWaitEvent(c->aux[j], c->aux_ev[knnj_ctx::kAux], 0));
     {
       // largest instances first, each to the stream with the least queued rows
-      const int naux = aux_streams();
       std::vector<int> by_size(order);
       std::stable_sort(by_size.begin(), by_size.end(),
                        [&](int a, int b) { return I[a].nR + I[a].nQ > I[b].nR + I[b].nQ; });
@@ -1853,7 +1859,7 @@ static knnj_status join_batch_once(knnj_ctx *c, int32_t n_inst, const knnj_insta
         TRY(enqueue_index(c, I[id], c->aux[j], nullptr, nullptr));
       }
     }
-    for (int j = 0; j < knnj_ctx::kAux; ++j) {
+    for (int j = 0; j < naux; ++j) {
       HIPCHK(hipEventRecord(c->aux_ev[j], c->aux[j]));
       HIPCHK(hipStreamWaitEvent(s, c->aux_ev[j], 0));
     }
