@@ -346,7 +346,8 @@ def bench_batch(args, kj, cfg):
     bopts = dict(refine_iters=args.refine_iters or -1, split_rounds=args.split_rounds)
     # the batched call autotunes itself on the warm-up call (Lloyd step, VALU
     # tile, k-select capacity, MFMA tile shape); the loop keeps the defaults
-    gopts = dict(bopts, graph=args.graph, autotune=not args.no_autotune, work_queue=args.work_queue)
+    gopts = dict(bopts, graph=args.graph, autotune=not args.no_autotune, work_queue=args.work_queue,
+                 rep_scale=args.rep_scale)
     outs, _ = ctx.join_batch(dev, **gopts)  # warm-up; with --graph it captures the plan
     for _ in range(max(0, args.warmup - 1)):
         ctx.join_batch(dev, outputs=outs, **gopts)
@@ -502,6 +503,9 @@ def main():
     ap.add_argument("--split-rounds", type=int, default=0, help="representative split rounds (0 default, -1 none)")
     ap.add_argument("--refine-iters", type=int, default=0, help="Lloyd steps (0 default, -1 none)")
     ap.add_argument("--tile-q", type=int, default=0, help="VALU join: queries per wave tile (0 auto)")
+    ap.add_argument("--rep-scale", type=int, default=0,
+                    help="batched workload: representatives per instance as a multiple of 2 sqrt(n) "
+                         "(0: the autotuner chooses)")
     ap.add_argument("--work-queue", action="store_true",
                     help="batched workload: persistent launches fed by per-XCD device work queues "
                          "(default: the autotuner chooses)")

@@ -127,7 +127,10 @@ typedef struct {
                           slots from eight per-XCD device counters until none are left --
                           instead of one workgroup per slot (0); the batch autotuner tries it
                           when this is 0 */
-  int32_t reserved[9];
+  int32_t rep_scale;   /* representatives per side where m_r / m_q are 0: rep_scale (1..8) times
+                          the default 2 sqrt(n), at most 4096 (0 = 1; knnj_join_batch: per
+                          instance, and its autotuner tries 2 and 4 when this is 0) */
+  int32_t reserved[8];
 } knnj_opts;
 
 /* Device-side timings (hipEvents on the context's stream) and counters. */

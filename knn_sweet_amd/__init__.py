@@ -52,7 +52,7 @@ class Opts(ctypes.Structure):
                 ("autotune", ctypes.c_int32), ("graph", ctypes.c_int32), ("kcap", ctypes.c_int32),
                 ("kcap_mfma_max", ctypes.c_int32), ("dedup", ctypes.c_int32),
                 ("filter_stats", ctypes.c_int32), ("work_queue", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 9)]
+                ("rep_scale", ctypes.c_int32), ("reserved", ctypes.c_int32 * 8)]
 
 
 class Stats(ctypes.Structure):
@@ -231,10 +231,11 @@ def _outputs(nq: int, k: int, aos: bool, qdev):
 def make_opts(m_q=0, m_r=0, trials=0, precision=PREC_AUTO, seed=0, out_squared=False, out_aos=False,
               brute_force=False, tile_q=0, kernel=KERNEL_AUTO, refine_iters=0, xcd_tiles=False,
               tile_waves=0, tile_split=0, split_rounds=0, autotune=False, graph=False, kcap=0,
-              kcap_mfma_max=0, dedup=0, filter_stats=False, work_queue=False) -> Opts:
+              kcap_mfma_max=0, dedup=0, filter_stats=False, work_queue=False, rep_scale=0) -> Opts:
     return Opts(m_q, m_r, trials, precision, seed, int(out_squared), int(out_aos), int(brute_force),
                 tile_q, kernel, refine_iters, int(xcd_tiles), tile_waves, tile_split, split_rounds,
-                int(autotune), int(graph), kcap, kcap_mfma_max, dedup, int(filter_stats), int(work_queue))
+                int(autotune), int(graph), kcap, kcap_mfma_max, dedup, int(filter_stats), int(work_queue),
+                rep_scale)
 
 
 class Context:

@@ -272,6 +272,10 @@ knnj_status check_join_args(const knnj_matrix &R, const knnj_matrix &Q, int32_t 
     set_err("tile_q must be 0, 1, 2, 4 or 8");
     return KNNJ_EINVAL;
   }
+  if (o.rep_scale < 0 || o.rep_scale > 8) {
+    set_err("rep_scale must be in [0, 8]");
+    return KNNJ_EINVAL;
+  }
   if (o.tile_split < 0 || o.tile_split > 16) {
     set_err("tile_split must be in [0, 16]");
     return KNNJ_EINVAL;
@@ -495,8 +499,11 @@ knnj_status decide(Inst &I, const u32 *flags) {
     I.dk = X.dk;
   }
   const bool brute = o.brute_force != 0;
-  int m_r = brute ? 1 : (o.m_r > 0 ? (int)std::min<int64_t>(o.m_r, I.nR) : default_reps(I.nR));
-  int m_q = brute ? 1 : (o.m_q > 0 ? (int)std::min<int64_t>(o.m_q, I.nQ) : default_reps(I.nQ));
+  // the default count scaled by rep_scale, within [1, min(n, KNNJ_MAX_REPS)]
+  const int rs = std::max(1, o.rep_scale);
+  auto scaled = [&](int64_t n) { return (int)std::min<int64_t>(n, (int64_t)rs * default_reps(n)); };
+  int m_r = brute ? 1 : (o.m_r > 0 ? (int)std::min<int64_t>(o.m_r, I.nR) : scaled(I.nR));
+  int m_q = brute ? 1 : (o.m_q > 0 ? (int)std::min<int64_t>(o.m_q, I.nQ) : scaled(I.nQ));
   I.m_r = std::max(1, std::min(m_r, KNNJ_MAX_REPS));
   I.m_q = std::max(1, std::min(m_q, KNNJ_MAX_REPS));
   if (I.self) I.m_q = I.m_r;
@@ -2051,7 +2058,8 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
   // instance may run on the vector ALUs, their 2 or 4 queries per tile;
   // second phase: the doubled k-select capacity and the 4-wave lockstep tile.
   std::vector<int64_t> sig = {-1, n_inst, o.precision, o.kernel, o.m_r, o.m_q, o.refine_iters, o.tile_q,
-                              o.tile_waves, o.kcap, o.brute_force, o.graph, o.split_rounds, o.work_queue};
+                              o.tile_waves, o.kcap, o.brute_force, o.graph, o.split_rounds, o.work_queue,
+                              o.rep_scale};
   int kmax = 1;
   bool any_valu = false, any_mfma = false;
   for (int i = 0; i < n_inst; ++i) {
@@ -2082,6 +2090,12 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
     x.work_queue = 1;
     cand.push_back(x);
   }
+  if (!o.rep_scale && !o.m_r && !o.m_q && !o.brute_force)  // and twice or four times the representatives
+    for (int sc : {2, 4}) {
+      knnj_opts x = cand[0];
+      x.rep_scale = sc;
+      cand.push_back(x);
+    }
   return autotune_call(
       c, sig, cand, o, kmax, any_mfma,
       [&](const knnj_opts &x, knnj_stats *st) { return join_batch_once(c, n_inst, inst, x, st); }, stats);

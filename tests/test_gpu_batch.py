@@ -206,3 +206,23 @@ def test_batch_work_queue_graph_replays(ctx):
         for i, ((a_i, a_d), (e_i, e_d)) in enumerate(zip(outs, eager)):
             _assert_same(a_i, e_i, f"replay {rep} instance {i} idx")
             _assert_same(a_d, e_d, f"replay {rep} instance {i} dist")
+
+
+def test_batch_rep_scale_matches_default(ctx):
+    """rep_scale multiplies each instance's default representative count
+    (2 sqrt(n), at most 4096): the answers stay the default's, bit for bit,
+    and the oracle's; a single join takes the option too."""
+    inst = datagen.batch_instances(n_inst=6, seed=140, r_lo=800, r_hi=20000)
+    dev = [(_dev(R), _dev(Q), 16) for R, Q in inst]
+    base, _ = ctx.join_batch(dev)
+    for sc in (2, 4):
+        res, _ = ctx.join_batch(dev, rep_scale=sc)
+        for i, ((x, d), (y, e)) in enumerate(zip(res, base)):
+            _assert_same(x, y, f"rep_scale {sc} instance {i} idx")
+            _assert_same(d, e, f"rep_scale {sc} instance {i} dist")
+    R, Q = inst[-1]
+    idx, dist, st = ctx.join(R, Q, 16, rep_scale=4)
+    assert st["m_r"] == min(4096, R.shape[0], 4 * int(round(2 * np.sqrt(R.shape[0]))))
+    assert_matches_oracle(R, Q, 16, idx, dist)
+    with pytest.raises(kj.KnnjError):
+        ctx.join(R, Q, 16, rep_scale=9)
