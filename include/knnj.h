@@ -129,7 +129,7 @@ typedef struct {
                           when this is 0 */
   int32_t rep_scale;   /* representatives per side where m_r / m_q are 0: rep_scale (1..8) times
                           the default 2 sqrt(n), at most 4096 (0 = 1; knnj_join_batch: per
-                          instance, and its autotuner tries 2 and 4 when this is 0) */
+                          instance, and its autotuner tries 2 when this is 0) */
   int32_t reserved[8];
 } knnj_opts;
 

@@ -2090,12 +2090,11 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
     x.work_queue = 1;
     cand.push_back(x);
   }
-  if (!o.rep_scale && !o.m_r && !o.m_q && !o.brute_force)  // and twice or four times the representatives
-    for (int sc : {2, 4}) {
-      knnj_opts x = cand[0];
-      x.rep_scale = sc;
-      cand.push_back(x);
-    }
+  if (!o.rep_scale && !o.m_r && !o.m_q && !o.brute_force) {  // and twice the representatives
+    knnj_opts x = cand[0];
+    x.rep_scale = 2;
+    cand.push_back(x);
+  }
   return autotune_call(
       c, sig, cand, o, kmax, any_mfma,
       [&](const knnj_opts &x, knnj_stats *st) { return join_batch_once(c, n_inst, inst, x, st); }, stats);
