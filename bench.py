@@ -343,7 +343,8 @@ def bench_batch(args, kj, cfg):
     torch.cuda.synchronize()
     ctx = kj.Context(0)
     # the batch's default (no Lloyd step) for both the batched call and the loop
-    bopts = dict(refine_iters=args.refine_iters or -1, split_rounds=args.split_rounds)
+    bopts = dict(refine_iters=args.refine_iters or -1, split_rounds=args.split_rounds,
+                 kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel])
     # the batched call autotunes itself on the warm-up call (Lloyd step, VALU
     # tile, k-select capacity, MFMA tile shape); the loop keeps the defaults
     gopts = dict(bopts, graph=args.graph, autotune=not args.no_autotune, work_queue=args.work_queue,
