@@ -344,7 +344,9 @@ def bench_batch(args, kj, cfg):
     ctx = kj.Context(0)
     # the batch's default (no Lloyd step) for both the batched call and the loop
     bopts = dict(refine_iters=args.refine_iters or -1, split_rounds=args.split_rounds,
-                 kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel])
+                 kernel={"auto": kj.KERNEL_AUTO, "valu": kj.KERNEL_VALU, "mfma": kj.KERNEL_MFMA}[args.kernel],
+                 precision={"auto": kj.PREC_AUTO, "fp32": kj.PREC_FP32, "bf16": kj.PREC_BF16,
+                            "bf16x1": kj.PREC_BF16X1, "fp8": kj.PREC_FP8}[args.precision])
     # the batched call autotunes itself on the warm-up call (Lloyd step, VALU
     # tile, k-select capacity, MFMA tile shape); the loop keeps the defaults
     gopts = dict(bopts, graph=args.graph, autotune=not args.no_autotune, work_queue=args.work_queue,

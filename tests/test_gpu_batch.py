@@ -226,3 +226,15 @@ def test_batch_rep_scale_matches_default(ctx):
     assert_matches_oracle(R, Q, 16, idx, dist)
     with pytest.raises(kj.KnnjError):
         ctx.join(R, Q, 16, rep_scale=9)
+
+
+def test_batch_bf16_filter_on_every_instance(ctx):
+    """The BF16 hi/lo filter forced on every instance (the batch autotuner's
+    precision candidate), d = 16 and 32 included: the oracle's answer."""
+    inst = datagen.batch_instances(n_inst=8, seed=150, r_lo=600, r_hi=15000)
+    dev = [(_dev(R), _dev(Q), 16) for R, Q in inst]
+    res, st = ctx.join_batch(dev, precision=kj.PREC_BF16)
+    base, _ = ctx.join_batch(dev)
+    for i, ((R, Q), (idx, dist), (bi, bd)) in enumerate(zip(inst, res, base)):
+        _assert_same(idx, bi, f"instance {i} idx (d={R.shape[1]})")
+        assert_matches_oracle(R, Q, 16, _host(idx), _host(dist))
