@@ -99,7 +99,8 @@ __device__ __forceinline__ u64 shfl_xor_u64(u64 v, int m) {
 // mirror: ^15; two moves: ^4 = ^7^3, ^8 = ^15^7), the swizzle's xor mode
 // within 32 lanes (^16, ^31), the swap of the two 32-lane halves
 // (v_permlane32_swap: ^32; after a ^31 swizzle: ^63), a permute through LDS
-// otherwise.  Only the permute makes an LDS round trip.
+// otherwise.  Only the permute makes an LDS round trip.  Callers run with
+// all 64 lanes active (the DPP forms read 0 from a disabled source lane).
 __device__ __forceinline__ u32 xor_lanes(u32 v, int m) {
   const int x = (int)v;
   switch (m) {
