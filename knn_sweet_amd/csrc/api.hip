@@ -2077,6 +2077,7 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
                               o.rep_scale};
   int kmax = 1;
   bool any_valu = false, any_mfma = false, any_low_d = false;
+  int min_d = 1 << 30;
   for (int i = 0; i < n_inst; ++i) {
     const knnj_matrix &R = inst[i].R, &Q = inst[i].Q;
     const int64_t ldR = R.ld ? R.ld : R.d, ldQ = Q.ld ? Q.ld : Q.d;
@@ -2087,6 +2088,7 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
     any_mfma |= mf;
     any_valu |= !mf;
     any_low_d |= R.d >= 16 && R.d < 64;
+    min_d = std::min(min_d, (int)R.d);
   }
   std::vector<knnj_opts> cand;
   const int refs[2] = {o.refine_iters, 1};
@@ -2111,11 +2113,11 @@ knnj_status knnj_join_batch(knnj_ctx *c, int32_t n_inst, const knnj_instance *in
     x.rep_scale = 2;
     cand.push_back(x);
   }
-  // and the BF16 hi/lo filter on the matrix cores for every instance: by
-  // default the instances of 16 <= d < 64 run plain FP32 (on the vector ALUs
-  // below d = 32); at 512 uneven joins of d 16, 32, 96 the filter's join
-  // phase took 82 against 109 ms
-  if (any_low_d && low_d_filter_candidate(o, 16)) {
+  // and the BF16 hi/lo filter on the matrix cores for every instance, when
+  // all have d >= 16: by default the instances of 16 <= d < 64 run plain
+  // FP32 (on the vector ALUs below d = 32); at 512 uneven joins of d 16, 32,
+  // 96 the filter's join phase took 82 against 109 ms
+  if (any_low_d && min_d >= 16 && low_d_filter_candidate(o, 16)) {
     knnj_opts x = cand[0];
     x.precision = KNNJ_PREC_BF16;
     cand.push_back(x);
