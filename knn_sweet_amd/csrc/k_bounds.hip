@@ -195,7 +195,7 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float bd_f32x4;
 __device__ __forceinline__ u64 wave_min_u64(u64 v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    const u64 o = __shfl_xor(v, off);
+    const u64 o = xor_lanes_u64(v, off);
     v = o < v ? o : v;
   }
   return v;

@@ -115,10 +115,10 @@ __global__ __launch_bounds__(256) void k_center(SideIndex Ri, int d, int dk, int
     }
 #pragma unroll
     for (int msk = 1; msk < G; msk <<= 1) {
-      parta += __shfl_xor(parta, msk);
-      partb += __shfl_xor(partb, msk);
-      ma = fmaxf(ma, __shfl_xor(ma, msk));
-      mb = fmaxf(mb, __shfl_xor(mb, msk));
+      parta += xor_lanes_f(parta, msk);
+      partb += xor_lanes_f(partb, msk);
+      ma = fmaxf(ma, xor_lanes_f(ma, msk));
+      mb = fmaxf(mb, xor_lanes_f(mb, msk));
     }
     if (oka) store_row(pa, va, parta, ma);
     if (okb) store_row(pb, vb, partb, mb);

@@ -525,8 +525,8 @@ __global__ __launch_bounds__(256, 2) void k_assign_mfma(const float *__restrict_
         }
       // ||x~||^2 of row c16 in all four lanes holding it, then to the
       // accumulator layout (lane g holds rows 4g .. 4g+3)
-      part += __shfl_xor(part, 16);
-      part += __shfl_xor(part, 32);
+      part += xor_lanes_f(part, 16);
+      part += xor_lanes_f(part, 32);
 #pragma unroll
       for (int i = 0; i < 4; ++i) cinit[ab][i] = -0.5f * __shfl(part, 4 * g + i);
     }
@@ -615,8 +615,8 @@ __global__ __launch_bounds__(256, 2) void k_assign_mfma(const float *__restrict_
       int bi = ((kb & lowmask) << 4) + c16;
 #pragma unroll
       for (int msk = 1; msk < 16; msk <<= 1) {
-        const int ok = __shfl_xor(kb, msk);
-        const int oi = __shfl_xor(bi, msk);
+        const int ok = xor_lanes_i(kb, msk);
+        const int oi = xor_lanes_i(bi, msk);
         if (ok < kb || (ok == kb && oi < bi)) {
           kb = ok;
           bi = oi;
@@ -773,7 +773,8 @@ __global__ __launch_bounds__(256) void k_assign_finish4(const float *__restrict_
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
 #pragma unroll
-    for (int msk = 1; msk < L; msk <<= 1) acc[h] += __shfl_xor(acc[h], msk);
+    for (int msk = 1; msk < L; msk <<= 1)
+      acc[h] += __builtin_bit_cast(double, xor_lanes_u64(__builtin_bit_cast(u64, acc[h]), msk));
     if (ok[h] && j == 0) {
       const double dd = sqrt(acc[h]);
       d2r[ir[h]] = (float)dd;
