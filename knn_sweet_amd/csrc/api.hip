@@ -674,9 +674,10 @@ int aux_streams() {
 }
 
 // The context's first n auxiliary streams (default priority) and their
-// events, created on first use (a single join needs two: the query side's
-// stream and the event of the reference representatives, aux_ev[1]; a batch
-// aux_streams()), and the fork event aux_ev[kAux].
+// events, created on first use (a single join needs three: the query side's
+// stream, the events of the reference representatives, aux_ev[1], and of the
+// centred reference rows, aux_ev[2]; a batch aux_streams()), and the fork
+// event aux_ev[kAux].
 knnj_status ensure_aux(knnj_ctx *c, int n) {
   n = std::min(n, (int)knnj_ctx::kAux);
   if (!c->aux_ev[knnj_ctx::kAux]) HIPCHK(hipEventCreateWithFlags(&c->aux_ev[knnj_ctx::kAux], hipEventDisableTiming));
@@ -778,11 +779,23 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
     I.Qi = I.Ri;
   else if (!two)
     TRY(enqueue_side(c, I, false, s));
+  // with two streams the centring of the reference rows (read by the join
+  // only) runs on the auxiliary stream beside the bounds, once the reference
+  // side is indexed; s waits for it after the tile list
+  const bool center = I.mfma && !I.pre;
+  const bool center_fork = two && center;
   if (two) {
-    HIPCHK(hipEventRecord(c->aux_ev[0], fork));
+    HIPCHK(hipEventRecord(c->aux_ev[0], fork));  // query side indexed
+    if (center_fork) {
+      HIPCHK(hipEventRecord(c->aux_ev[knnj_ctx::kAux], s));  // reference side indexed
+      HIPCHK(hipStreamWaitEvent(fork, c->aux_ev[knnj_ctx::kAux], 0));
+      launch_center(I.Ri, I.d, I.dk, center_planes(I), I.rc, I.nr, I.reps32, I.rn32, I.rinv, fork);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c->aux_ev[2], fork));
+    }
     HIPCHK(hipStreamWaitEvent(s, c->aux_ev[0], 0));
   }
-  if (I.mfma && !I.pre) {
+  if (center && !center_fork) {
     launch_center(I.Ri, I.d, I.dk, center_planes(I), I.rc, I.nr, I.reps32, I.rn32, I.rinv, s);
     HIPCHK(hipGetLastError());
   }
@@ -795,6 +808,7 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
   launch_make_tiles(I.dedup ? I.uoff : I.Qi.off, I.m_q, I.theta, I.tq, I.id, I.tiles, I.counters,
                     (int)I.max_slots, I.o.xcd_tiles ? 8 : 1, s);
   HIPCHK(hipGetLastError());
+  if (center_fork) HIPCHK(hipStreamWaitEvent(s, c->aux_ev[2], 0));  // the centred rows
   return KNNJ_OK;
 }
 
@@ -1065,7 +1079,7 @@ knnj_status run_single(knnj_ctx *c, Inst &I, Alloc &A, knnj_stats *stats) {
   // side's (the allocation-free path creates nothing and stays on one stream)
   hipStream_t fork = nullptr;
   if (!A.no_alloc && !I.pre && !I.self) {
-    TRY(ensure_aux(c, 2));
+    TRY(ensure_aux(c, 3));  // aux_ev[1]: representatives final, aux_ev[2]: rows centred
     fork = c->aux[0];
   }
   TRY(enqueue_index(c, I, s, ev[2], ev[3], fork));
