@@ -691,7 +691,8 @@ knnj_status ensure_aux(knnj_ctx *c, int n) {
 
 // One side's cluster index: sampled representatives, Lloyd refinement,
 // assignment, radii, and the cluster-sorted permuted copy.
-knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s, hipEvent_t reps_final = nullptr) {
+knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s, hipEvent_t reps_final = nullptr,
+                         bool defer_permute = false) {
   SideBufs &b = ref_side ? I.rb : I.qb;
   const float *X = ref_side ? I.dR : I.dQ;
   const int64_t n = ref_side ? I.nR : I.nQ, ld = ref_side ? I.ldR : I.ldQ;
@@ -736,7 +737,7 @@ knnj_status enqueue_side(knnj_ctx *c, Inst &I, bool ref_side, hipStream_t s, hip
   launch_zeros(s, b.radbits, sizeof(u64) * m, b.cnt, sizeof(int32_t) * m, b.fill, sizeof(int32_t) * m);
   launch_assign_finish(X, n, ld, d, dpad, b.reps, b.cl, b.d2r, b.radbits, b.cnt, s);
   launch_bucket_sort_permute(X, n, ld, d, dpad, m, b.cl, b.cnt, b.off, b.fill, b.perm, b.d2r, b.pdist, b.gkeys,
-                             b.px, s);
+                             defer_permute ? nullptr : b.px, s);
   HIPCHK(hipGetLastError());
   fill_side_index(I, ref_side);
   return KNNJ_OK;
@@ -763,7 +764,13 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
   // assignment runs there too, once the reference representatives are final
   // (beside the reference side's own assignment, into buffers of its own)
   const bool seed_fork = two && I.qseed && I.qs_rb;
-  if (!I.pre) TRY(enqueue_side(c, I, true, s, seed_fork ? c->aux_ev[1] : nullptr));
+  // with two streams the permuted copy and the centring of the reference
+  // rows (read by the join only) run on the auxiliary stream beside the
+  // bounds, once the reference side is indexed; s waits for them after the
+  // tile list
+  const bool center = I.mfma && !I.pre;
+  const bool center_fork = two && center;
+  if (!I.pre) TRY(enqueue_side(c, I, true, s, seed_fork ? c->aux_ev[1] : nullptr, center_fork));
   if (I.qseed) {
     if (I.pre) {
       launch_assign(I.dQ, I.nQ, I.ldQ, I.d, I.dpad, I.pre->Ri.reps, I.m_r, I.qseed, I.qs_rb, I.qs_rn, s);
@@ -779,16 +786,12 @@ knnj_status enqueue_index(knnj_ctx *c, Inst &I, hipStream_t s, hipEvent_t ev_ind
     I.Qi = I.Ri;
   else if (!two)
     TRY(enqueue_side(c, I, false, s));
-  // with two streams the centring of the reference rows (read by the join
-  // only) runs on the auxiliary stream beside the bounds, once the reference
-  // side is indexed; s waits for it after the tile list
-  const bool center = I.mfma && !I.pre;
-  const bool center_fork = two && center;
   if (two) {
     HIPCHK(hipEventRecord(c->aux_ev[0], fork));  // query side indexed
     if (center_fork) {
       HIPCHK(hipEventRecord(c->aux_ev[knnj_ctx::kAux], s));  // reference side indexed
       HIPCHK(hipStreamWaitEvent(fork, c->aux_ev[knnj_ctx::kAux], 0));
+      launch_permute(I.dR, I.nR, I.ldR, I.d, I.dpad, I.rb.perm, I.rb.px, fork);
       launch_center(I.Ri, I.d, I.dk, center_planes(I), I.rc, I.nr, I.reps32, I.rn32, I.rinv, fork);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(c->aux_ev[2], fork));

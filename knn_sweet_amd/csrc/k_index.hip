@@ -1099,6 +1099,12 @@ void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, in
                      perm);
   hipLaunchKernelGGL(k_segsort_wave, dim3(m), dim3(64), 0, s, off, perm, d2r, pdist);
   hipLaunchKernelGGL(k_segsort, dim3(m), dim3(1024), 0, s, off, perm, d2r, pdist, gkeys);
+  if (px) launch_permute(X, n, ld, d, dpad, perm, px, s);
+}
+
+void launch_permute(const float *X, int64_t n, int64_t ld, int d, int dpad, const int32_t *perm, float *px,
+                    hipStream_t s) {
+  if (n <= 0) return;
   const int64_t tot = n * dpad;
   if (d % 4 == 0 && ld % 4 == 0 && ((uintptr_t)X & 15) == 0 && tot / 4 < ((int64_t)1 << 32)) {
     const int64_t t4 = tot / 4;

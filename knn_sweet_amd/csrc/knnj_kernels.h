@@ -40,10 +40,14 @@ void launch_centroid(const float *X, int64_t ld, int d, int dpad, int m, const i
                      int integral, float *reps, hipStream_t s);
 void launch_bucket_rows(const int32_t *cl, int64_t n, int m, const int32_t *cnt, int32_t *off, int32_t *fill,
                         int32_t *perm, u64 *gkeys, hipStream_t s);
+// (px null: the permuted copy is left to launch_permute)
 void launch_bucket_sort_permute(const float *X, int64_t n, int64_t ld, int d, int dpad, int m,
                                 const int32_t *cl, const int32_t *cnt, int32_t *off, int32_t *fill,
                                 int32_t *perm, const float *d2r, float *pdist, u64 *gkeys, float *px,
                                 hipStream_t s);
+// px[p] = X[perm[p]] (rows zero padded to dpad)
+void launch_permute(const float *X, int64_t n, int64_t ld, int d, int dpad, const int32_t *perm, float *px,
+                    hipStream_t s);
 // XCD-aware tile slots; *nslots = slots used (empty tiles have count 0);
 // every slot's pad field = inst
 void launch_make_tiles(const int32_t *off, int m, const double *theta, int tq, int inst, Tile *tiles,
